@@ -1,0 +1,367 @@
+"""Unstructured triangle meshes for the hydraulic step.
+
+The connectivity contract follows the reference cut-cell mesh
+(braids_core/boundaries/cut_cells.py:83-182, CutFace/CutCellMesh):
+
+* every face has an ``owner``; its unit ``normal`` points out of the owner;
+* ``neighbour`` is -1 on the exterior; ``bid`` names the open-boundary group
+  (-1 marks a reflective wall);
+* the cell->face incidence is CSR (``offset``, ``faces_csr``, ``signs``) with
+  +1 for the owner side and -1 for the neighbour side.  Within a cell the
+  faces are listed in ascending face order, exactly as load_case assembles
+  them (solver/cut_cells/cutcell_solver.f90:5093-5110), so every per-cell
+  floating-point sum runs in the reference order;
+* the faces of each open boundary are listed in ascending face order
+  (cutcell_solver.f90:5111-5126).
+
+Meshes are generated as structured quads split into triangles by one
+diagonal of alternating orientation, with seeded bed fields.  A locality
+renumbering (Hilbert curve over cell centroids) is available; it is a
+permutation with a recorded inverse, so results map back exactly.
+"""
+from __future__ import annotations
+
+import dataclasses
+import math
+from typing import Callable, Optional
+
+import numpy as np
+
+INLET_TOTAL_DISCHARGE = 6   # cut_cell_io.py:470-535 mode map
+OUTLET_STAGE = 2
+
+
+@dataclasses.dataclass
+class Mesh:
+    nc: int
+    nf: int
+    nb: int
+    area: np.ndarray        # [nc]
+    xy: np.ndarray          # [2, nc]
+    owner: np.ndarray       # [nf] int32
+    neighbour: np.ndarray   # [nf] int32, -1 exterior
+    bid: np.ndarray         # [nf] int32, -1 wall / interior
+    normal: np.ndarray      # [2, nf]
+    length: np.ndarray      # [nf]
+    mid: np.ndarray         # [2, nf]
+    manning: np.ndarray     # [nc]
+    z: np.ndarray           # [nc] bed elevation (rock)
+    q0: np.ndarray          # [3, nc] initial h, hu, hv
+    mode: np.ndarray        # [nb] boundary mode
+    forcing_q: np.ndarray   # [nb]
+    forcing_stage: np.ndarray  # [nb]
+    irregular: np.ndarray   # [nc] int32: 1 unless a verified axis-aligned rectangle
+    ii: np.ndarray          # [nc] 1-based background column (deck record)
+    jj: np.ndarray          # [nc] 1-based background row
+    nx: int = 1
+    ny: int = 1
+    dx: float = 1.0
+    dy: float = 1.0
+    series_offset: Optional[np.ndarray] = None   # [nb+1] hydrograph knots
+    series_time: Optional[np.ndarray] = None
+    series_q: Optional[np.ndarray] = None
+    series_stage: Optional[np.ndarray] = None
+    name: str = 'mesh'
+    # derived
+    offset: np.ndarray = None
+    faces_csr: np.ndarray = None
+    signs: np.ndarray = None
+    boundary_offset: np.ndarray = None
+    boundary_faces: np.ndarray = None
+    cell_perm: Optional[np.ndarray] = None   # new index -> original cell
+    face_perm: Optional[np.ndarray] = None   # new index -> original face
+
+    def __post_init__(self):
+        self.build_incidence()
+
+    # -- connectivity ---------------------------------------------------
+    def build_incidence(self):
+        """CSR incidence in load_case order (cutcell_solver.f90:5093-5126)."""
+        nc, nf = self.nc, self.nf
+        own = self.owner.astype(np.int64)
+        nbr = self.neighbour.astype(np.int64)
+        internal = nbr >= 0
+        cells = np.concatenate([own, nbr[internal]])
+        fids = np.concatenate([np.arange(nf), np.nonzero(internal)[0]])
+        sg = np.concatenate([np.ones(nf, np.int32), -np.ones(int(internal.sum()), np.int32)])
+        # ascending face id within each cell: sort by (cell, face id)
+        order = np.lexsort((fids, cells))
+        counts = np.bincount(cells, minlength=nc)
+        self.offset = np.zeros(nc + 1, np.int32)
+        np.cumsum(counts, out=self.offset[1:])
+        self.faces_csr = fids[order].astype(np.int32)
+        self.signs = sg[order].astype(np.int32)
+        b = self.bid.astype(np.int64)
+        has = b >= 0
+        bcount = np.bincount(b[has], minlength=self.nb) if self.nb else np.zeros(0, np.int64)
+        self.boundary_offset = np.zeros(self.nb + 1, np.int32)
+        if self.nb:
+            np.cumsum(bcount, out=self.boundary_offset[1:])
+        bf = np.nonzero(has)[0]
+        self.boundary_faces = bf[np.argsort(b[bf], kind='stable')].astype(np.int32)
+
+    @property
+    def nadj(self):
+        return int(self.offset[-1])
+
+    def max_degree(self):
+        return int(np.max(np.diff(self.offset)))
+
+    # -- renumbering -----------------------------------------------------
+    def renumbered(self, order: str = 'hilbert', bits: int = 16) -> 'Mesh':
+        """Return a copy with cells in Hilbert order and faces sorted by the
+        smaller incident (new) cell index, then the larger one.  Owner,
+        neighbour and normal orientation are preserved per face."""
+        if order == 'identity':
+            cperm = np.arange(self.nc)
+        elif order == 'hilbert':
+            key = hilbert_keys(self.xy[0], self.xy[1], bits)
+            cperm = np.lexsort((np.arange(self.nc), key))
+        else:
+            raise ValueError(f'unknown renumbering {order!r}')
+        inv = np.empty(self.nc, np.int64)
+        inv[cperm] = np.arange(self.nc)
+        own = inv[self.owner]
+        nbr = np.where(self.neighbour >= 0, inv[np.maximum(self.neighbour, 0)], -1)
+        lo = np.where(nbr >= 0, np.minimum(own, nbr), own)
+        hi = np.where(nbr >= 0, np.maximum(own, nbr), np.int64(self.nc))
+        fperm = np.lexsort((np.arange(self.nf), hi, lo))
+        m = Mesh(
+            nc=self.nc, nf=self.nf, nb=self.nb,
+            area=self.area[cperm].copy(), xy=self.xy[:, cperm].copy(),
+            owner=own[fperm].astype(np.int32), neighbour=nbr[fperm].astype(np.int32),
+            bid=self.bid[fperm].copy(), normal=self.normal[:, fperm].copy(),
+            length=self.length[fperm].copy(), mid=self.mid[:, fperm].copy(),
+            manning=self.manning[cperm].copy(), z=self.z[cperm].copy(),
+            q0=self.q0[:, cperm].copy(), mode=self.mode.copy(),
+            forcing_q=self.forcing_q.copy(), forcing_stage=self.forcing_stage.copy(),
+            irregular=self.irregular[cperm].copy(), ii=self.ii[cperm].copy(), jj=self.jj[cperm].copy(),
+            nx=self.nx, ny=self.ny, dx=self.dx, dy=self.dy,
+            series_offset=self.series_offset, series_time=self.series_time,
+            series_q=self.series_q, series_stage=self.series_stage, name=self.name)
+        base_c = self.cell_perm if self.cell_perm is not None else np.arange(self.nc)
+        base_f = self.face_perm if self.face_perm is not None else np.arange(self.nf)
+        m.cell_perm = base_c[cperm]
+        m.face_perm = base_f[fperm]
+        return m
+
+    def to_original_cells(self, a: np.ndarray) -> np.ndarray:
+        """Scatter a cell array ([..., nc]) back to the generator's cell order."""
+        if self.cell_perm is None:
+            return a
+        out = np.empty_like(a)
+        out[..., self.cell_perm] = a
+        return out
+
+    # -- checks ----------------------------------------------------------
+    def metrics(self):
+        """Discrete divergence-theorem closure (cut_cells.py:184-237)."""
+        vec = self.length[None, :] * self.normal
+        closure = np.zeros((2, self.nc))
+        internal = self.neighbour >= 0
+        for a in range(2):
+            closure[a] += np.bincount(self.owner, weights=vec[a], minlength=self.nc)
+            closure[a] -= np.bincount(self.neighbour[internal], weights=vec[a, internal], minlength=self.nc)
+        return dict(closure_linf=float(np.abs(closure).max()),
+                    area_m2=math.fsum(self.area.tolist()),
+                    boundary_faces=int((~internal).sum()))
+
+
+def hilbert_keys(x: np.ndarray, y: np.ndarray, bits: int = 16) -> np.ndarray:
+    """Hilbert-curve index of points quantised to a 2^bits grid over their
+    bounding box (vectorised form of the classic xy->d rotation walk)."""
+    n = 1 << bits
+    x = np.asarray(x, np.float64); y = np.asarray(y, np.float64)
+    x0, x1 = float(x.min()), float(x.max()); y0, y1 = float(y.min()), float(y.max())
+    span = max(x1 - x0, y1 - y0, 1e-300)
+    xi = np.minimum(((x - x0) / span * (n - 1)).astype(np.int64), n - 1)
+    yi = np.minimum(((y - y0) / span * (n - 1)).astype(np.int64), n - 1)
+    d = np.zeros(x.shape, np.int64)
+    s = n >> 1
+    while s > 0:
+        rx = ((xi & s) > 0).astype(np.int64)
+        ry = ((yi & s) > 0).astype(np.int64)
+        d += s * s * ((3 * rx) ^ ry)
+        # rotate the quadrant
+        flip = (ry == 0) & (rx == 1)
+        xi = np.where(flip, s - 1 - xi, xi)
+        yi = np.where(flip, s - 1 - yi, yi)
+        swap = ry == 0
+        xi, yi = np.where(swap, yi, xi), np.where(swap, xi, yi)
+        s >>= 1
+    return d
+
+
+def structured_triangles(nx: int, ny: int, dx: float, dy: float, *,
+                         bed: Callable[[np.ndarray, np.ndarray], np.ndarray],
+                         manning: Callable[[np.ndarray, np.ndarray], np.ndarray],
+                         depth0: Callable[[np.ndarray, np.ndarray, np.ndarray], np.ndarray],
+                         inflow_q: float, outlet_stage: float,
+                         series=None, name='channel') -> Mesh:
+    """Rectangle [0, nx*dx] x [0, ny*dy] of quads, each split by a diagonal
+    whose orientation alternates like a checkerboard; 2*nx*ny triangles.
+    Boundary 0 is the total-discharge inlet at x=0 (mode 6), boundary 1 the
+    stage outlet at x=nx*dx (mode 2); y=0 and y=ny*dy are walls."""
+    I, J = np.meshgrid(np.arange(nx), np.arange(ny))
+    I = I.ravel(); J = J.ravel()                    # quad index q = J*nx + I
+    nq = nx * ny
+    slash = ((I + J) % 2) == 0                      # "/" diagonal (i,j)-(i+1,j+1)
+    x0 = I * dx; y0 = J * dy
+    # triangle vertices: T0 then T1 of each quad
+    # "/": T0=(i,j),(i+1,j),(i+1,j+1)  T1=(i,j),(i+1,j+1),(i,j+1)
+    # "\": T0=(i,j),(i+1,j),(i,j+1)    T1=(i+1,j),(i+1,j+1),(i,j+1)
+    cx = np.empty(2 * nq); cy = np.empty(2 * nq)
+    cx[0::2] = np.where(slash, x0 + 2 * dx / 3, x0 + dx / 3)
+    cy[0::2] = np.where(slash, y0 + dy / 3, y0 + dy / 3)
+    cx[1::2] = np.where(slash, x0 + dx / 3, x0 + 2 * dx / 3)
+    cy[1::2] = np.where(slash, y0 + 2 * dy / 3, y0 + 2 * dy / 3)
+    nc = 2 * nq
+    area = np.full(nc, 0.5 * dx * dy)
+    tri0 = 2 * (J * nx + I); tri1 = tri0 + 1
+    # which triangle touches each side of the quad
+    bottom = tri0
+    top = tri1
+    left = np.where(slash, tri1, tri0)
+    right = np.where(slash, tri0, tri1)
+
+    def quad(i, j):
+        return j * nx + i
+
+    owners = []; nbrs = []; bids = []; nxs = []; nys = []; lens = []; mxs = []; mys = []
+    # internal horizontal edges: between quad (i,j-1) [top] and (i,j) [bottom]
+    ih, jh = np.meshgrid(np.arange(nx), np.arange(1, ny))
+    ih = ih.ravel(); jh = jh.ravel()
+    owners.append(top[quad(ih, jh - 1)]); nbrs.append(bottom[quad(ih, jh)])
+    bids.append(np.full(ih.size, -1)); nxs.append(np.zeros(ih.size)); nys.append(np.ones(ih.size))
+    lens.append(np.full(ih.size, dx)); mxs.append((ih + 0.5) * dx); mys.append(jh * dy)
+    # internal vertical edges: between quad (i-1,j) [right] and (i,j) [left]
+    iv, jv = np.meshgrid(np.arange(1, nx), np.arange(ny))
+    iv = iv.ravel(); jv = jv.ravel()
+    owners.append(right[quad(iv - 1, jv)]); nbrs.append(left[quad(iv, jv)])
+    bids.append(np.full(iv.size, -1)); nxs.append(np.ones(iv.size)); nys.append(np.zeros(iv.size))
+    lens.append(np.full(iv.size, dy)); mxs.append(iv * dx); mys.append((jv + 0.5) * dy)
+    # diagonals: owner T0, neighbour T1
+    dl = math.hypot(dx, dy)
+    owners.append(tri0); nbrs.append(tri1); bids.append(np.full(nq, -1))
+    nxs.append(np.where(slash, -dy / dl, dy / dl)); nys.append(np.full(nq, dx / dl))
+    lens.append(np.full(nq, dl)); mxs.append(x0 + 0.5 * dx); mys.append(y0 + 0.5 * dy)
+    # walls y=0 and y=H
+    ib = np.arange(nx)
+    owners.append(bottom[quad(ib, 0)]); nbrs.append(np.full(nx, -1)); bids.append(np.full(nx, -1))
+    nxs.append(np.zeros(nx)); nys.append(-np.ones(nx)); lens.append(np.full(nx, dx))
+    mxs.append((ib + 0.5) * dx); mys.append(np.zeros(nx))
+    owners.append(top[quad(ib, ny - 1)]); nbrs.append(np.full(nx, -1)); bids.append(np.full(nx, -1))
+    nxs.append(np.zeros(nx)); nys.append(np.ones(nx)); lens.append(np.full(nx, dx))
+    mxs.append((ib + 0.5) * dx); mys.append(np.full(nx, ny * dy))
+    # inlet x=0 (boundary 0) and outlet x=L (boundary 1)
+    jb = np.arange(ny)
+    owners.append(left[quad(0, jb)]); nbrs.append(np.full(ny, -1)); bids.append(np.zeros(ny, int))
+    nxs.append(-np.ones(ny)); nys.append(np.zeros(ny)); lens.append(np.full(ny, dy))
+    mxs.append(np.zeros(ny)); mys.append((jb + 0.5) * dy)
+    owners.append(right[quad(nx - 1, jb)]); nbrs.append(np.full(ny, -1)); bids.append(np.ones(ny, int))
+    nxs.append(np.ones(ny)); nys.append(np.zeros(ny)); lens.append(np.full(ny, dy))
+    mxs.append(np.full(ny, nx * dx)); mys.append((jb + 0.5) * dy)
+
+    owner = np.concatenate(owners).astype(np.int32)
+    neighbour = np.concatenate(nbrs).astype(np.int32)
+    bid = np.concatenate(bids).astype(np.int32)
+    normal = np.vstack([np.concatenate(nxs), np.concatenate(nys)])
+    length = np.concatenate(lens)
+    mid = np.vstack([np.concatenate(mxs), np.concatenate(mys)])
+    xy = np.vstack([cx, cy])
+    z = np.asarray(bed(cx, cy), np.float64)
+    n = np.asarray(manning(cx, cy), np.float64) * np.ones(nc)
+    h = np.maximum(np.asarray(depth0(cx, cy, z), np.float64), 0.0)
+    q0 = np.vstack([h, np.zeros(nc), np.zeros(nc)])
+    ii = np.repeat(I + 1, 2).astype(np.int32)
+    jj = np.repeat(J + 1, 2).astype(np.int32)
+    so = st = sq = ss = None
+    if series is not None:
+        so, st, sq, ss = series
+    return Mesh(nc=nc, nf=owner.size, nb=2, area=area, xy=xy, owner=owner,
+                neighbour=neighbour, bid=bid, normal=normal, length=length, mid=mid,
+                manning=n, z=z, q0=q0,
+                mode=np.array([INLET_TOTAL_DISCHARGE, OUTLET_STAGE], np.int32),
+                forcing_q=np.array([float(inflow_q), 0.0]),
+                forcing_stage=np.array([0.0, float(outlet_stage)]),
+                irregular=np.ones(nc, np.int32), ii=ii, jj=jj, nx=nx, ny=ny, dx=dx, dy=dy,
+                series_offset=so, series_time=st, series_q=sq, series_stage=ss, name=name)
+
+
+# ---------------------------------------------------------------------------
+# Workloads (synthetic inputs; seeded)
+# ---------------------------------------------------------------------------
+
+def plain_channel(nx=500, ny=25, length=2000.0, width=100.0, slope=1e-3, sigma=0.05,
+                  n=0.03, depth=2.0, q=400.0, seed=1) -> Mesh:
+    """Straight rectangular channel 2000 m x 100 m, bed slope 1e-3 plus seeded
+    Gaussian bed noise (sigma 0.05 m), 4 m quads split into 25,000 triangles,
+    Manning n=0.03, 2 m initial depth at rest, inflow Q=400 m^3/s, fixed stage
+    2 m above the outlet bed trend."""
+    dx = length / nx; dy = width / ny
+    rng = np.random.default_rng(seed)
+    noise = rng.normal(0.0, sigma, 2 * nx * ny)
+    cells = {}
+
+    def bed(x, y):
+        return -slope * x + noise
+
+    stage = -slope * length + depth
+    return structured_triangles(nx, ny, dx, dy, bed=bed, manning=lambda x, y: n,
+                                depth0=lambda x, y, z: np.full_like(x, depth),
+                                inflow_q=q, outlet_stage=stage, name='plain_channel')
+
+
+def braided_reach(nx=1768, ny=141, length=5000.0, width=400.0, slope=1e-3, amplitude=60.0,
+                  wavelength=800.0, nbars=200, q=1500.0, seed=2, depth=3.0) -> Mesh:
+    """Braided synthetic reach 5 km x 400 m: sinusoidal thalweg (amplitude
+    60 m, wavelength 800 m), parabolic cross-section, 200 seeded Gaussian bars
+    (height 0.5-2 m, sigma 15-60 m) that create wet/dry fronts, four Manning
+    zones n in {0.025, 0.030, 0.035, 0.040}, inflow Q=1500 m^3/s.  The default
+    2.83 m quads give about 500k triangles."""
+    dx = length / nx; dy = width / ny
+    rng = np.random.default_rng(seed)
+    bx = rng.uniform(0, length, nbars); by = rng.uniform(0, width, nbars)
+    bh = rng.uniform(0.5, 2.0, nbars); bs = rng.uniform(15.0, 60.0, nbars)
+
+    def thalweg(x):
+        return 0.5 * width + amplitude * np.sin(2 * np.pi * x / wavelength)
+
+    def bed(x, y):
+        z = -slope * x + 4.0 * ((y - thalweg(x)) / (0.5 * width)) ** 2
+        for k in range(nbars):
+            z = z + bh[k] * np.exp(-((x - bx[k]) ** 2 + (y - by[k]) ** 2) / (2 * bs[k] ** 2))
+        return z
+
+    def manning(x, y):
+        zone = np.minimum((x / (length / 4)).astype(int), 3)
+        return np.array([0.025, 0.030, 0.035, 0.040])[zone]
+
+    def depth0(x, y, z):
+        return (-slope * x + depth) - z
+
+    return structured_triangles(nx, ny, dx, dy, bed=bed, manning=manning, depth0=depth0,
+                                inflow_q=q, outlet_stage=-slope * length + depth, name='braided_reach')
+
+
+def lake_at_rest(nx=20, ny=6, dx=4.0, dy=4.0, stage=1.0, seed=3, bumps=True) -> Mesh:
+    """Closed basin with a bumpy bed partly above the still-water level:
+    the well-balanced property keeps it exactly at rest."""
+    rng = np.random.default_rng(seed)
+    noise = rng.normal(0.0, 0.3, 2 * nx * ny)
+
+    def bed(x, y):
+        z = 0.02 * x + (noise if bumps else 0.0)
+        return z
+
+    m = structured_triangles(nx, ny, dx, dy, bed=bed, manning=lambda x, y: 0.03,
+                             depth0=lambda x, y, z: stage - z, inflow_q=0.0,
+                             outlet_stage=stage, name='lake_at_rest')
+    # close both ends: walls everywhere
+    m.bid[:] = -1
+    m.nb = 0
+    m.mode = np.zeros(0, np.int32)
+    m.forcing_q = np.zeros(0)
+    m.forcing_stage = np.zeros(0)
+    m.build_incidence()
+    return m

@@ -1,0 +1,119 @@
+"""Reference-solver harness (test infrastructure only).
+
+Writes a hydraulic-only ``case.data`` deck (format version 9, read by
+load_case, solver/cut_cells/cutcell_solver.f90:2961-5520) for a mesh, runs
+the reference executable built from its own sources (oracle/_ref), and reads
+back the accepted state it publishes (output_frame, :17606-17640: ASCII
+``frames.txt`` up to 4096 cells, else stream-binary ``frames.bin``).
+
+Deck v9 records used here (v9 admits mode 6), in load_case order:
+  version / 64-hex fingerprint / nc nf nb nx ny /
+  grav dry bulk manning ws d50 d90 rhos rho z0 sedthr dx dy /
+  endtime dtmax output_dt / scheme bedlaw exchange_mode /
+  mpm_cf mpm_critical / bed_slope alfabs alfabn friction_angle bedload_min_depth /
+  nc x (ii jj area x y rock h hu hv suspended bedmass manning) /
+  nf x (owner neighbour bid nx ny length midx midy)   [1-based, 0 = none] /
+  nb x (mode Q stage concentration bed_input weir_coefficient weir_width) /
+  reversal_policy(1:nb) / nseries(1:nb) / nhydro_series(1:nb) + knots / bed_boundary_mode(1:nb) /
+  cfl_target settling_mode hindered_reference
+"""
+import os
+import subprocess
+from pathlib import Path
+
+import numpy as np
+
+from .swe import reference_executable
+
+FINGERPRINT = 'b' * 64
+
+
+def _f(x):
+    return repr(float(x))
+
+
+def write_deck(path, mesh, *, scheme=5, endtime=1.0, dtmax=1.0, output_dt=None,
+               cfl_target=0.9, grav=9.81, dry=1e-8):
+    m = mesh
+    out = []
+    out.append('9')
+    out.append(FINGERPRINT)
+    out.append(f'{m.nc} {m.nf} {m.nb} {int(m.ii.max())} {int(m.jj.max())}')
+    # bulk, d50, d90, rho, z0 must be positive and rhos > rho even without sediment
+    out.append(' '.join(_f(v) for v in (grav, dry, 1600.0, 0.03, 0.0, 2e-3, 4e-3, 2650.0,
+                                        1000.0, 1e-3, 0.0, m.dx, m.dy)))
+    out.append(' '.join(_f(v) for v in (endtime, dtmax, output_dt if output_dt else endtime)))
+    out.append(f'{scheme} 0 0')
+    out.append('8.0 0.047')
+    out.append('0 0.0 0.0 30.0 0.0')
+    for c in range(m.nc):
+        out.append(f'{int(m.ii[c])} {int(m.jj[c])} {_f(m.area[c])} {_f(m.xy[0, c])} {_f(m.xy[1, c])} '
+                   f'{_f(m.z[c])} {_f(m.q0[0, c])} {_f(m.q0[1, c])} {_f(m.q0[2, c])} 0.0 0.0 '
+                   f'{_f(m.manning[c])}')
+    for f in range(m.nf):
+        nb_ = int(m.neighbour[f]) + 1 if m.neighbour[f] >= 0 else 0
+        b = int(m.bid[f]) + 1 if m.bid[f] >= 0 else 0
+        out.append(f'{int(m.owner[f]) + 1} {nb_} {b} {_f(m.normal[0, f])} {_f(m.normal[1, f])} '
+                   f'{_f(m.length[f])} {_f(m.mid[0, f])} {_f(m.mid[1, f])}')
+    for b in range(m.nb):
+        out.append(f'{int(m.mode[b])} {_f(m.forcing_q[b])} {_f(m.forcing_stage[b])} 0.0 0.0 0.0 0.0')
+    if m.nb:
+        out.append(' '.join(['1'] * m.nb))
+        out.append(' '.join(['0'] * m.nb))
+        if m.series_offset is not None:
+            counts = np.diff(m.series_offset)
+            out.append(' '.join(str(int(k)) for k in counts))
+            for b in range(m.nb):
+                for k in range(m.series_offset[b], m.series_offset[b + 1]):
+                    out.append(f'{_f(m.series_time[k])} {_f(m.series_q[k])} {_f(m.series_stage[k])}')
+        else:
+            out.append(' '.join(['0'] * m.nb))
+        out.append(' '.join(['0'] * m.nb))
+    out.append(f'{_f(cfl_target)} 0 10000.0')
+    Path(path).write_text('\n'.join(out) + '\n')
+
+
+def read_frames(workdir, nc, nb):
+    """Return a list of (time, steps, q[5, nc], ledger[3, nb]) frames."""
+    workdir = Path(workdir)
+    frames = []
+    if (workdir / 'frames.bin').exists():
+        raw = (workdir / 'frames.bin').read_bytes()
+        rec = 8 + 8 + 8 * 5 * nc + 8 * 3 * nb
+        for k in range(len(raw) // rec):
+            base = k * rec
+            t = np.frombuffer(raw, np.float64, 1, base)[0]
+            s = np.frombuffer(raw, np.int64, 1, base + 8)[0]
+            q = np.frombuffer(raw, np.float64, 5 * nc, base + 16).reshape(nc, 5).T.copy()
+            led = np.frombuffer(raw, np.float64, 3 * nb, base + 16 + 40 * nc).reshape(nb, 3).T.copy()
+            frames.append((float(t), int(s), q, led))
+        return frames
+    tokens = (workdir / 'frames.txt').read_text().split()
+    pos = 2   # header: nc nb
+    while pos < len(tokens):
+        t = float(tokens[pos].replace('D', 'E')); s = int(float(tokens[pos + 1])); pos += 2
+        vals = np.array([float(v.replace('D', 'E')) for v in tokens[pos:pos + 5 * nc]]); pos += 5 * nc
+        q = vals.reshape(nc, 5).T.copy()
+        lv = np.array([float(v.replace('D', 'E')) for v in tokens[pos:pos + 3 * nb]]); pos += 3 * nb
+        frames.append((t, s, q, lv.reshape(nb, 3).T.copy() if nb else np.zeros((3, 0))))
+    return frames
+
+
+def run_reference(mesh, workdir, *, scheme=5, endtime=1.0, dtmax=1.0, output_dt=None,
+                  cfl_target=0.9, threads=1, timeout=600):
+    """Run the reference executable on a deck written for ``mesh``."""
+    exe = reference_executable()
+    if exe is None:
+        raise FileNotFoundError('reference executable is not built (make -C oracle ref)')
+    workdir = Path(workdir)
+    workdir.mkdir(parents=True, exist_ok=True)
+    write_deck(workdir / 'case.data', mesh, scheme=scheme, endtime=endtime, dtmax=dtmax,
+               output_dt=output_dt, cfl_target=cfl_target)
+    env = dict(os.environ, OMP_NUM_THREADS=str(threads), OMP_DYNAMIC='FALSE',
+               BRAIDS_TIME_INTEGRATOR='legacy')
+    env.pop('BRAIDS_PROFILE', None)
+    proc = subprocess.run([str(exe), 'case.data'], cwd=workdir, env=env, capture_output=True,
+                          text=True, timeout=timeout)
+    if proc.returncode != 0:
+        raise RuntimeError(f'reference run failed ({proc.returncode}):\n{proc.stdout[-3000:]}\n{proc.stderr[-3000:]}')
+    return read_frames(workdir, mesh.nc, mesh.nb), proc.stdout

@@ -1,0 +1,232 @@
+"""Oracle loader (test infrastructure only).
+
+ctypes bindings for the plain-C FP64 restatement (liboracle.so, built from
+swe_oracle.c) and, where it has been built, for the reference Fortran solver
+compiled from its own sources (_ref/libbraids_ref.so).  Only tests/, the
+smoke check and the CPU-baseline leg of bench.py import this module; the
+product never does.
+"""
+import ctypes
+import os
+import subprocess
+from pathlib import Path
+
+import numpy as np
+
+HERE = Path(__file__).resolve().parent
+LIB = HERE / 'liboracle.so'
+REF_LIB = HERE / '_ref' / 'libbraids_ref.so'
+REF_EXE = HERE / '_ref' / 'xbraids_cutcell'
+
+_D = ctypes.c_double
+_P = ctypes.c_void_p
+_I = ctypes.c_int
+
+
+def build(reference=True):
+    """Compile liboracle.so; also the reference build when its sources exist."""
+    subprocess.run(['make', '-s', '-C', str(HERE)], check=True)
+    if reference and Path('/root/reference/src').is_dir():
+        subprocess.run(['make', '-s', '-C', str(HERE), 'ref'], check=True)
+
+
+class _Mesh(ctypes.Structure):
+    _fields_ = [('nc', _I), ('nf', _I), ('nb', _I), ('nadj', _I), ('scheme', _I),
+                ('grav', _D), ('dry', _D), ('cfl_target', _D), ('dtmax', _D),
+                ('area', _P), ('xy', _P), ('manning', _P), ('irregular', _P),
+                ('owner', _P), ('neighbour', _P), ('bid', _P),
+                ('normal', _P), ('length', _P), ('mid', _P),
+                ('offset', _P), ('faces', _P), ('signs', _P),
+                ('mode', _P), ('boundary_offset', _P), ('boundary_faces', _P),
+                ('forcing_q', _P), ('forcing_stage', _P),
+                ('series_offset', _P), ('series_time', _P), ('series_q', _P), ('series_stage', _P)]
+
+
+def _lib():
+    if not LIB.exists():
+        raise FileNotFoundError(f'{LIB} is not built; run make -C oracle')
+    lib = ctypes.CDLL(str(LIB))
+    lib.om_create.restype = _P
+    lib.om_create.argtypes = [ctypes.POINTER(_Mesh)]
+    lib.om_destroy.argtypes = [_P]
+    lib.om_wave_flux.argtypes = [_P, _P, _D, _D, _P, _P, _P, _P, _P]
+    lib.om_riemanntype.argtypes = [_D, _D, _D, _D, _P, _P, _P, _P, _P]
+    lib.om_euler.argtypes = [_P, _P, _P, _D, _D, _P, _P, _P, _P, _P, _P]
+    lib.om_euler.restype = _I
+    lib.om_rates.argtypes = [_P, _P, _P, _D, _D, _P, _P]
+    lib.om_rates.restype = _I
+    lib.om_prepare_stage.argtypes = [_P, _P]
+    lib.om_manning.argtypes = [_P, _P, _D]
+    lib.om_step_ssp33.argtypes = [_P, _P, _P, _D, _D, _P, _P, _P]
+    lib.om_step_ssp33.restype = _I
+    lib.om_advance.argtypes = [_P, _P, _P, _I, _D, _P, _P, _P, _P]
+    lib.om_advance.restype = _I
+    lib.om_mass.argtypes = [_P, _P]
+    lib.om_mass.restype = _D
+    return lib
+
+
+_LIB = None
+
+
+def lib():
+    global _LIB
+    if _LIB is None:
+        _LIB = _lib()
+    return _LIB
+
+
+def _ptr(a):
+    return None if a is None else a.ctypes.data
+
+
+def wave_flux(left, right, bl, br, n):
+    """Oracle wave_flux: returns (fl[3], fr[3], speed, rightmost)."""
+    L = lib()
+    left = np.ascontiguousarray(left, dtype=np.float64)
+    right = np.ascontiguousarray(right, dtype=np.float64)
+    n = np.ascontiguousarray(n, dtype=np.float64)
+    fl = np.zeros(3); fr = np.zeros(3); s = np.zeros(1); rm = np.zeros(1)
+    L.om_wave_flux(_ptr(left), _ptr(right), float(bl), float(br), _ptr(n),
+                   _ptr(fl), _ptr(fr), _ptr(s), _ptr(rm))
+    return fl, fr, float(s[0]), float(rm[0])
+
+
+def riemanntype(hl, hr, ul, ur):
+    L = lib()
+    out = np.zeros(3); r = np.zeros(2, dtype=np.int32)
+    L.om_riemanntype(hl, hr, ul, ur, _ptr(out[0:]), _ptr(out[1:]), _ptr(out[2:]),
+                     _ptr(r[0:]), _ptr(r[1:]))
+    return float(out[0]), float(out[1]), float(out[2]), int(r[0]), int(r[1])
+
+
+class Oracle:
+    """FP64 restatement bound to one mesh (a braids_hip.mesh.Mesh-like object
+    exposing the connectivity arrays as NumPy attributes)."""
+
+    def __init__(self, mesh, scheme=5, grav=9.81, dry=1e-8, cfl_target=0.9, dtmax=1.0):
+        self.L = lib()
+        m = mesh
+        keep = {}
+
+        def arr(name, a, dt):
+            a = np.ascontiguousarray(a, dtype=dt)
+            keep[name] = a
+            return a.ctypes.data
+
+        s = _Mesh()
+        s.nc, s.nf, s.nb = int(m.nc), int(m.nf), int(m.nb)
+        s.nadj = int(len(m.faces_csr))
+        s.scheme = int(scheme)
+        s.grav, s.dry, s.cfl_target, s.dtmax = grav, dry, cfl_target, dtmax
+        s.area = arr('area', m.area, np.float64)
+        s.xy = arr('xy', m.xy, np.float64)
+        s.manning = arr('manning', m.manning, np.float64)
+        s.irregular = arr('irregular', m.irregular, np.int32)
+        s.owner = arr('owner', m.owner, np.int32)
+        s.neighbour = arr('neighbour', m.neighbour, np.int32)
+        s.bid = arr('bid', m.bid, np.int32)
+        s.normal = arr('normal', m.normal, np.float64)
+        s.length = arr('length', m.length, np.float64)
+        s.mid = arr('mid', m.mid, np.float64)
+        s.offset = arr('offset', m.offset, np.int32)
+        s.faces = arr('faces', m.faces_csr, np.int32)
+        s.signs = arr('signs', m.signs, np.int32)
+        s.mode = arr('mode', np.append(m.mode, 0), np.int32)
+        s.boundary_offset = arr('boundary_offset', m.boundary_offset, np.int32)
+        s.boundary_faces = arr('boundary_faces', np.append(m.boundary_faces, 0), np.int32)
+        s.forcing_q = arr('forcing_q', np.append(m.forcing_q, 0.0), np.float64)
+        s.forcing_stage = arr('forcing_stage', np.append(m.forcing_stage, 0.0), np.float64)
+        if getattr(m, 'series_offset', None) is not None:
+            s.series_offset = arr('series_offset', m.series_offset, np.int32)
+            s.series_time = arr('series_time', np.append(m.series_time, 0.0), np.float64)
+            s.series_q = arr('series_q', np.append(m.series_q, 0.0), np.float64)
+            s.series_stage = arr('series_stage', np.append(m.series_stage, 0.0), np.float64)
+        self._keep = keep
+        self._mesh = s
+        self.nc, self.nf, self.nb = s.nc, s.nf, s.nb
+        self.ctx = self.L.om_create(ctypes.byref(s))
+
+    def __del__(self):
+        ctx = getattr(self, 'ctx', None)
+        if ctx:
+            self.L.om_destroy(ctx)
+            self.ctx = None
+
+    def euler(self, q, z, dt, t=0.0):
+        q = np.ascontiguousarray(q, dtype=np.float64)
+        z = np.ascontiguousarray(z, dtype=np.float64)
+        nxt = np.empty_like(q); water = np.zeros(self.nf); speed = np.zeros(self.nf)
+        ledger = np.zeros(max(self.nb, 1)); peak = np.zeros(1); fail = np.zeros(1, dtype=np.int32)
+        ok = self.L.om_euler(self.ctx, _ptr(q), _ptr(z), float(dt), float(t), _ptr(nxt),
+                             _ptr(water), _ptr(speed), _ptr(ledger), _ptr(peak), _ptr(fail))
+        return dict(ok=bool(ok), next=nxt, water=water, speed=speed, ledger=ledger[:self.nb],
+                    cfl=float(peak[0]), failure=int(fail[0]))
+
+    def rates(self, q, z, dt=0.0, t=0.0):
+        q = np.ascontiguousarray(q, dtype=np.float64)
+        z = np.ascontiguousarray(z, dtype=np.float64)
+        rate = np.empty_like(q); water = np.zeros(self.nf)
+        ok = self.L.om_rates(self.ctx, _ptr(q), _ptr(z), float(dt), float(t), _ptr(rate), _ptr(water))
+        return bool(ok), rate, water
+
+    def step(self, q, z, dt, t=0.0, ledger=None):
+        q = np.array(q, dtype=np.float64, order='C')
+        z = np.ascontiguousarray(z, dtype=np.float64)
+        led = np.zeros(max(self.nb, 1)) if ledger is None else ledger
+        peak = np.zeros(1); fail = np.zeros(1, dtype=np.int32)
+        ok = self.L.om_step_ssp33(self.ctx, _ptr(q), _ptr(z), float(dt), float(t), _ptr(led),
+                                  _ptr(peak), _ptr(fail))
+        return dict(ok=bool(ok), q=q, ledger=led[:self.nb], cfl=float(peak[0]), failure=int(fail[0]))
+
+    def advance(self, q, z, nsteps, tend=1e300, time=0.0, step_hint=None, ledger=None):
+        """Accepted-step loop; returns (q, time, step_hint, accepted, retries, ledger)."""
+        q = np.array(q, dtype=np.float64, order='C')
+        z = np.ascontiguousarray(z, dtype=np.float64)
+        t = np.array([time]); hint = np.array([self._mesh.dtmax if step_hint is None else step_hint])
+        led = np.zeros(max(self.nb, 1)) if ledger is None else ledger
+        retries = np.zeros(1, dtype=np.int32)
+        n = self.L.om_advance(self.ctx, _ptr(q), _ptr(z), int(nsteps), float(tend), _ptr(t),
+                              _ptr(hint), _ptr(led), _ptr(retries))
+        return q, float(t[0]), float(hint[0]), int(n), int(retries[0]), led[:self.nb]
+
+    def mass(self, q):
+        q = np.ascontiguousarray(q, dtype=np.float64)
+        return float(self.L.om_mass(self.ctx, _ptr(q)))
+
+
+class Reference:
+    """The reference Fortran routines compiled from /root/reference (optional)."""
+
+    def __init__(self, grav=9.81, dry=1e-8):
+        if not REF_LIB.exists():
+            raise FileNotFoundError(str(REF_LIB))
+        L = ctypes.CDLL(str(REF_LIB))
+        L.ref_set_constants.argtypes = [_D, _D]
+        L.ref_wave_flux.argtypes = [_P, _P, _D, _D, _P, _P, _P, _P, _P]
+        L.ref_riemanntype.argtypes = [_D, _D, _D, _D, _P, _P, _P, _P, _P]
+        L.ref_set_constants(grav, dry)
+        self.L = L
+
+    def wave_flux(self, left, right, bl, br, n):
+        left = np.ascontiguousarray(left, dtype=np.float64)
+        right = np.ascontiguousarray(right, dtype=np.float64)
+        n = np.ascontiguousarray(n, dtype=np.float64)
+        fl = np.zeros(3); fr = np.zeros(3); s = np.zeros(1); rm = np.zeros(1)
+        self.L.ref_wave_flux(_ptr(left), _ptr(right), float(bl), float(br), _ptr(n),
+                             _ptr(fl), _ptr(fr), _ptr(s), _ptr(rm))
+        return fl, fr, float(s[0]), float(rm[0])
+
+    def riemanntype(self, hl, hr, ul, ur):
+        out = np.zeros(3); r = np.zeros(2, dtype=np.int32)
+        self.L.ref_riemanntype(hl, hr, ul, ur, _ptr(out[0:]), _ptr(out[1:]), _ptr(out[2:]),
+                               _ptr(r[0:]), _ptr(r[1:]))
+        return float(out[0]), float(out[1]), float(out[2]), int(r[0]), int(r[1])
+
+
+def reference_available():
+    return REF_LIB.exists()
+
+
+def reference_executable():
+    return REF_EXE if REF_EXE.exists() else None

@@ -1,0 +1,996 @@
+/*
+ * Oracle (test infrastructure only; see swe_oracle.h).  FP64 restatement of
+ * the hydraulic part of the BRAIDS-HKT cut-cell solver:
+ *   riemanntype / riemann_aug_JCP / rpn2  (clawpack riemann/src, GeoClaw 5.14)
+ *   wave_flux, dynamic_runup_opens_face   (cutcell_solver.f90:9135-9180)
+ *   primitive, gradients, face_value, prepare_wet_dry_traces
+ *                                          (:7787-7817, :8536-8623, :8999-9133)
+ *   traces for modes 2 and 6               (:9814-10270)
+ *   euler face loop, draining, gather+CFL  (:10795-11600, ledger :12058-12066)
+ *   prepare_hydraulic_stage, source        (:8513-8534, :14580-14600)
+ *   advance_ssp33, accepted-step control   (:15858-15986, :17287-17370)
+ * Arithmetic keeps the reference's operation order (Fortran evaluates
+ * a*b*c as (a*b)*c and x**2 as x*x); build with -ffp-contract=off.
+ * Sediment, AMR, secondary flow, WENO (scheme 6) and the section-root
+ * boundary modes are outside this restatement.
+ */
+#include "swe_oracle.h"
+#include <math.h>
+#include <stdlib.h>
+#include <string.h>
+#include <float.h>
+
+#define EPS DBL_EPSILON
+#define SPEED_LIMIT 50.0      /* geoclaw_parameters.f90:9 */
+#define MANNING_MIN_DEPTH 1e-2 /* cutcell_solver.f90:394 */
+
+static double G = 9.81, DRY = 1e-8;   /* module grav / dry_tolerance */
+
+static inline double dmax(double a, double b) { return a > b ? a : (b > a ? b : a); }
+static inline double dmin(double a, double b) { return a < b ? a : (b < a ? b : a); }
+
+struct om_ctx {
+    om_mesh m;
+    double *ls_weight;   /* hydraulic_ls_weight [nadj] */
+    double *ls_delta;    /* bed_ls_delta [2*nadj] */
+    double *ls_matrix;   /* hydraulic_ls_matrix [4*nc] */
+    /* stage workspace */
+    int *active, *hconst;
+    double *v, *grad, *vlo, *vhi, *trace, *fl, *fr, *speed, *theta;
+    double *hydraulic;   /* [2*nb] */
+    double *target_u;    /* [nf] mode-6 face weights */
+    double *qa, *qb, *qc, *la, *lb, *lc;
+};
+
+/* ------------------------------------------------------------------ */
+/* riemanntype: geoclaw_riemann_utils.f:547-658                         */
+void om_riemanntype(double hL, double hR, double uL, double uR, double *hm_out,
+                    double *s1m, double *s2m, int *rare1, int *rare2)
+{
+    const double g = G, drytol = DRY;
+    double h_min = dmin(hR, hL), h_max = dmax(hR, hL), delu = uR - uL, hm, um;
+    (void)um;
+    if (h_min <= drytol) {
+        hm = 0.0;
+        *s1m = uR + uL - 2.0 * sqrt(g * hR) + 2.0 * sqrt(g * hL);
+        *s2m = uR + uL - 2.0 * sqrt(g * hR) + 2.0 * sqrt(g * hL);
+        if (hL <= 0.0) { *rare2 = 1; *rare1 = 0; }
+        else { *rare1 = 1; *rare2 = 0; }
+    } else {
+        double F_min = delu + 2.0 * (sqrt(g * h_min) - sqrt(g * h_max));
+        double F_max = delu + (h_max - h_min) * (sqrt(.5 * g * (h_max + h_min) / (h_max * h_min)));
+        if (F_min > 0.0) {
+            double t = dmax(0.0, -delu + 2.0 * (sqrt(g * hL) + sqrt(g * hR)));
+            hm = (1.0 / (16.0 * g)) * (t * t);
+            *s1m = uL + 2.0 * sqrt(g * hL) - 3.0 * sqrt(g * hm);
+            *s2m = uR - 2.0 * sqrt(g * hR) + 3.0 * sqrt(g * hm);
+            *rare1 = 1; *rare2 = 1;
+        } else if (F_max <= 0.0) {
+            double h0 = h_max;
+            {   /* one Newton iteration on sqrt(h) */
+                double gL = sqrt(.5 * g * (1 / h0 + 1 / hL));
+                double gR = sqrt(.5 * g * (1 / h0 + 1 / hR));
+                double F0 = delu + (h0 - hL) * gL + (h0 - hR) * gR;
+                double dfdh = gL - g * (h0 - hL) / (4.0 * (h0 * h0) * gL)
+                            + gR - g * (h0 - hR) / (4.0 * (h0 * h0) * gR);
+                double slope = 2.0 * sqrt(h0) * dfdh;
+                double t = sqrt(h0) - F0 / slope;
+                h0 = t * t;
+            }
+            hm = h0;
+            double u1m = uL - (hm - hL) * sqrt((.5 * g) * (1 / hm + 1 / hL));
+            double u2m = uR + (hm - hR) * sqrt((.5 * g) * (1 / hm + 1 / hR));
+            *s1m = u1m - sqrt(g * hm);
+            *s2m = u2m + sqrt(g * hm);
+            *rare1 = 0; *rare2 = 0;
+        } else {
+            double h0 = h_min;
+            {
+                double F0 = delu + 2.0 * (sqrt(g * h0) - sqrt(g * h_max))
+                          + (h0 - h_min) * sqrt(.5 * g * (1 / h0 + 1 / h_min));
+                double slope = (F_max - F0) / (h_max - h_min);
+                h0 = h0 - F0 / slope;
+            }
+            hm = h0;
+            if (hL > hR) {
+                *s1m = uL + 2.0 * sqrt(g * hL) - 3.0 * sqrt(g * hm);
+                *s2m = uL + 2.0 * sqrt(g * hL) - sqrt(g * hm);
+                *rare1 = 1; *rare2 = 0;
+            } else {
+                *s2m = uR - 2.0 * sqrt(g * hR) + 3.0 * sqrt(g * hm);
+                *s1m = uR - 2.0 * sqrt(g * hR) + sqrt(g * hm);
+                *rare2 = 1; *rare1 = 0;
+            }
+        }
+    }
+    *hm_out = hm;
+}
+
+/* riemann_aug_JCP with maxiter=1: geoclaw_riemann_utils.f:2-292.
+ * fw[3][3] is fw(component, wave). */
+static void riemann_aug_jcp(double hL, double hR, double huL, double huR,
+                            double hvL, double hvR, double bL, double bR,
+                            double uL, double uR, double vL, double vR,
+                            double phiL, double phiR, double sE1, double sE2,
+                            double sw[3], double fw[3][3])
+{
+    const double g = G, drytol = DRY, rho = 1000.0, pL = 0.0, pR = 0.0;
+    double r[3][3], lambda[3], del[3], beta[3], A[3][3];
+    double delh = hR - hL, delhu = huR - huL, delphi = phiR - phiL;
+    double delb = bR - bL + pR / (rho * g) - pL / (rho * g);
+    double delp = 0.0;
+    double delnorm = delh * delh + delphi * delphi;
+    double hm, s1m, s2m; int rare1, rare2;
+    (void)hvL; (void)hvR; (void)delnorm;
+    om_riemanntype(hL, hR, uL, uR, &hm, &s1m, &s2m, &rare1, &rare2);
+    lambda[0] = dmin(sE1, s2m);
+    lambda[2] = dmax(sE2, s1m);
+    sE1 = lambda[0]; sE2 = lambda[2];
+    lambda[1] = 0.0;
+    double hstarHLL = dmax((huL - huR + sE2 * hR - sE1 * hL) / (sE2 - sE1), 0.0);
+    for (int mw = 0; mw < 3; ++mw) {
+        r[0][mw] = 1.0; r[1][mw] = lambda[mw]; r[2][mw] = lambda[mw] * lambda[mw];
+    }
+    /* rarecorrector is always false in this GeoClaw revision */
+    lambda[1] = 0.5 * (lambda[0] + lambda[2]);
+    r[0][1] = 0.0; r[1][1] = 0.0; r[2][1] = 1.0;
+
+    double criticaltol = dmax(drytol * g, 1e-6);
+    double criticaltol_2 = sqrt(criticaltol);
+    double deldelh, deldelphi;
+    double hLstar = hL, hRstar = hR, uLstar = uL, uRstar = uR;
+    /* iter = 1 */
+    double hbar = dmax(0.5 * (hLstar + hRstar), 0.0);
+    double su = uLstar + uRstar;
+    double s1s2bar = 0.25 * (su * su) - g * hbar;
+    double s1s2tilde = dmax(0.0, uLstar * uRstar) - g * hbar;
+    int sonic = 0;
+    if (fabs(s1s2bar) <= criticaltol) sonic = 1;
+    else if (s1s2bar * s1s2tilde <= criticaltol * criticaltol) sonic = 1;
+    else if (s1s2bar * sE1 * sE2 <= criticaltol * criticaltol) sonic = 1;
+    else if (dmin(fabs(sE1), fabs(sE2)) < criticaltol_2) sonic = 1;
+    else if (sE1 < criticaltol_2 && s1m > -criticaltol_2) sonic = 1;
+    else if (sE2 > -criticaltol_2 && s2m < criticaltol_2) sonic = 1;
+    else if ((uL + sqrt(g * hL)) * (uR + sqrt(g * hR)) < 0.0) sonic = 1;
+    else if ((uL - sqrt(g * hL)) * (uR - sqrt(g * hR)) < 0.0) sonic = 1;
+
+    if (sonic) deldelh = -delb;
+    else deldelh = delb * g * hbar / s1s2bar;
+    if (sE1 < -criticaltol && sE2 > criticaltol) {
+        deldelh = dmin(deldelh, hstarHLL * (sE2 - sE1) / sE2);
+        deldelh = dmax(deldelh, hstarHLL * (sE2 - sE1) / sE1);
+    } else if (sE1 >= criticaltol) {
+        deldelh = dmin(deldelh, hstarHLL * (sE2 - sE1) / sE1);
+        deldelh = dmax(deldelh, -hL);
+    } else if (sE2 <= -criticaltol) {
+        deldelh = dmin(deldelh, hR);
+        deldelh = dmax(deldelh, hstarHLL * (sE2 - sE1) / sE2);
+    }
+    if (sonic) deldelphi = -g * hbar * delb;
+    else deldelphi = -delb * g * hbar * s1s2tilde / s1s2bar;
+    deldelphi = dmin(deldelphi, g * dmax(-hLstar * delb, -hRstar * delb));
+    deldelphi = dmax(deldelphi, g * dmin(-hLstar * delb, -hRstar * delb));
+    (void)delp;
+
+    del[0] = delh - deldelh;
+    del[1] = delhu;
+    del[2] = delphi - deldelphi;
+
+    double det1 = r[0][0] * (r[1][1] * r[2][2] - r[1][2] * r[2][1]);
+    double det2 = r[0][1] * (r[1][0] * r[2][2] - r[1][2] * r[2][0]);
+    double det3 = r[0][2] * (r[1][0] * r[2][1] - r[1][1] * r[2][0]);
+    double determinant = det1 - det2 + det3;
+    for (int k = 0; k < 3; ++k) {
+        for (int mw = 0; mw < 3; ++mw) {
+            A[0][mw] = r[0][mw]; A[1][mw] = r[1][mw]; A[2][mw] = r[2][mw];
+        }
+        A[0][k] = del[0]; A[1][k] = del[1]; A[2][k] = del[2];
+        det1 = A[0][0] * (A[1][1] * A[2][2] - A[1][2] * A[2][1]);
+        det2 = A[0][1] * (A[1][0] * A[2][2] - A[1][2] * A[2][0]);
+        det3 = A[0][2] * (A[1][0] * A[2][1] - A[1][1] * A[2][0]);
+        beta[k] = (det1 - det2 + det3) / determinant;
+    }
+    /* with maxiter=1 the star-state refresh after the convergence test does
+       not reach any output */
+    for (int mw = 0; mw < 3; ++mw) {
+        sw[mw] = lambda[mw];
+        fw[0][mw] = beta[mw] * r[1][mw];
+        fw[1][mw] = beta[mw] * r[2][mw];
+        fw[2][mw] = beta[mw] * r[1][mw];
+    }
+    fw[2][0] = fw[2][0] * vL;
+    fw[2][2] = fw[2][2] * vR;
+    fw[2][1] = 0.0;
+    double hustar_interface = huL + fw[0][0];
+    if (hustar_interface <= 0.0)
+        fw[2][0] = fw[2][0] + (hR * uR * vR - hL * uL * vL - fw[2][0] - fw[2][2]);
+    else
+        fw[2][2] = fw[2][2] + (hR * uR * vR - hL * uL * vL - fw[2][0] - fw[2][2]);
+}
+
+/* rpn2 for one interface (ixy=1): rpn2_geoclaw.f:52-295.  ql/qr are the
+ * (h, normal momentum, tangential momentum) states left and right of the
+ * interface; produces speeds s[3], amdq[3], apdq[3]. */
+static void rpn2_one(const double qleft[3], const double qright[3], double bLin,
+                     double bRin, double s[3], double amdq[3], double apdq[3])
+{
+    const double g = G, drytol = DRY;
+    double fwave[3][3] = {{0}}; /* fwave(component, wave) */
+    double sw[3] = {0, 0, 0}, fw[3][3] = {{0}};
+    double lq[3] = {qleft[0], qleft[1], qleft[2]};
+    double rq[3] = {qright[0], qright[1], qright[2]};
+    s[0] = s[1] = s[2] = 0.0;
+    for (int j = 0; j < 3; ++j) { amdq[j] = 0.0; apdq[j] = 0.0; }
+    if (lq[0] < 0.0) { lq[0] = 0.0; lq[1] = 0.0; lq[2] = 0.0; }
+    if (rq[0] < 0.0) { rq[0] = 0.0; rq[1] = 0.0; rq[2] = 0.0; }
+    if (!(lq[0] <= drytol && rq[0] <= drytol)) {
+        double hL = lq[0], hR = rq[0], huL = lq[1], huR = rq[1];
+        double bL = bLin, bR = bRin, hvL = lq[2], hvR = rq[2];
+        double uL, uR, vL, vR, phiL, phiR, wall[3] = {1.0, 1.0, 1.0};
+        if (hR > drytol) {
+            uR = huR / hR; vR = hvR / hR;
+            phiR = 0.5 * g * (hR * hR) + (huR * huR) / hR;
+        } else { hR = 0.0; huR = 0.0; hvR = 0.0; uR = 0.0; vR = 0.0; phiR = 0.0; }
+        if (hL > drytol) {
+            uL = huL / hL; vL = hvL / hL;
+            phiL = 0.5 * g * (hL * hL) + (huL * huL) / hL;
+        } else { hL = 0.0; huL = 0.0; hvL = 0.0; uL = 0.0; vL = 0.0; phiL = 0.0; }
+        if (hR <= drytol) {
+            double hstar, s1m, s2m; int r1, r2;
+            om_riemanntype(hL, hL, uL, -uL, &hstar, &s1m, &s2m, &r1, &r2);
+            double hstartest = dmax(hL, hstar);
+            if (hstartest + bL < bR) {
+                wall[1] = 0.0; wall[2] = 0.0;
+                hR = hL; huR = -huL; bR = bL; phiR = phiL; uR = -uL; vR = vL;
+            } else if (hL + bL < bR) {
+                bR = hL + bL;
+            }
+        } else if (hL <= drytol) {
+            double hstar, s1m, s2m; int r1, r2;
+            om_riemanntype(hR, hR, -uR, uR, &hstar, &s1m, &s2m, &r1, &r2);
+            double hstartest = dmax(hR, hstar);
+            if (hstartest + bR < bL) {
+                wall[0] = 0.0; wall[1] = 0.0;
+                hL = hR; huL = -huR; bL = bR; phiL = phiR; uL = -uR; vL = vR;
+            } else if (hR + bR < bL) {
+                bL = hR + bR;
+            }
+        }
+        double sL = uL - sqrt(g * hL);
+        double sR = uR + sqrt(g * hR);
+        double uhat = (sqrt(g * hL) * uL + sqrt(g * hR) * uR) / (sqrt(g * hR) + sqrt(g * hL));
+        double chat = sqrt(g * 0.5 * (hR + hL));
+        double sRoe1 = uhat - chat, sRoe2 = uhat + chat;
+        double sE1 = dmin(sL, sRoe1), sE2 = dmax(sR, sRoe2);
+        riemann_aug_jcp(hL, hR, huL, huR, hvL, hvR, bL, bR, uL, uR, vL, vR,
+                        phiL, phiR, sE1, sE2, sw, fw);
+        for (int mw = 0; mw < 3; ++mw) {
+            sw[mw] = sw[mw] * wall[mw];
+            fw[0][mw] = fw[0][mw] * wall[mw];
+            fw[1][mw] = fw[1][mw] * wall[mw];
+            fw[2][mw] = fw[2][mw] * wall[mw];
+        }
+        for (int mw = 0; mw < 3; ++mw) {
+            s[mw] = sw[mw];
+            fwave[0][mw] = fw[0][mw];
+            fwave[1][mw] = fw[1][mw];
+            fwave[2][mw] = fw[2][mw];
+        }
+    }
+    for (int mw = 0; mw < 3; ++mw) {
+        if (s[mw] < -1e-14) {
+            for (int j = 0; j < 3; ++j) amdq[j] = amdq[j] + fwave[j][mw];
+        } else if (s[mw] > 1e-14) {
+            for (int j = 0; j < 3; ++j) apdq[j] = apdq[j] + fwave[j][mw];
+        } else {
+            for (int j = 0; j < 3; ++j) {
+                amdq[j] = amdq[j] + 0.5 * fwave[j][mw];
+                apdq[j] = apdq[j] + 0.5 * fwave[j][mw];
+            }
+        }
+    }
+}
+
+/* wave_flux: cutcell_solver.f90:9135-9159 */
+void om_wave_flux(const double left[3], const double right[3], double bl, double br,
+                  const double n[2], double fl[3], double fr[3], double *speed,
+                  double *rightmost)
+{
+    const double g = G;
+    double t[2] = {-n[1], n[0]};
+    double l[3] = {left[0], left[1] * n[0] + left[2] * n[1], left[1] * t[0] + left[2] * t[1]};
+    double r[3] = {right[0], right[1] * n[0] + right[2] * n[1], right[1] * t[0] + right[2] * t[1]};
+    double s[3], am[3], ap[3], pl[3] = {0, 0, 0}, pr[3] = {0, 0, 0}, fluxl[3], fluxr[3];
+    rpn2_one(l, r, bl, br, s, am, ap);
+    if (l[0] > DRY) { pl[0] = l[1]; pl[1] = (l[1] * l[1]) / l[0] + .5 * g * (l[0] * l[0]); pl[2] = l[1] * l[2] / l[0]; }
+    if (r[0] > DRY) { pr[0] = r[1]; pr[1] = (r[1] * r[1]) / r[0] + .5 * g * (r[0] * r[0]); pr[2] = r[1] * r[2] / r[0]; }
+    for (int j = 0; j < 3; ++j) { fluxl[j] = pl[j] + am[j]; fluxr[j] = pr[j] - ap[j]; }
+    fl[0] = fluxl[0]; fl[1] = fluxl[1] * n[0] + fluxl[2] * t[0]; fl[2] = fluxl[1] * n[1] + fluxl[2] * t[1];
+    fr[0] = fluxr[0]; fr[1] = fluxr[1] * n[0] + fluxr[2] * t[0]; fr[2] = fluxr[1] * n[1] + fluxr[2] * t[1];
+    /* the reference stops on |fl1-fr1| > 1e-10 max(1,|fl1|,|fr1|); the
+       f-wave decomposition makes the two equal up to roundoff */
+    fl[0] = .5 * (fl[0] + fr[0]); fr[0] = fl[0];
+    *speed = dmax(dmax(fabs(s[0]), fabs(s[1])), fabs(s[2]));
+    if (rightmost) *rightmost = dmax(dmax(s[0], s[1]), s[2]);
+}
+
+/* dynamic_runup_opens_face: cutcell_solver.f90:9161-9180 */
+static int runup_opens_face(const double l[3], const double r[3], double bl, double br, const double n[2])
+{
+    double h, approach, hstar, s1m, s2m; int r1, r2;
+    if (l[0] > DRY && r[0] <= DRY) {
+        h = l[0]; approach = (l[1] * n[0] + l[2] * n[1]) / h;
+        if (approach <= 0) return 0;
+        om_riemanntype(h, h, approach, -approach, &hstar, &s1m, &s2m, &r1, &r2);
+        return hstar > h + DRY && hstar + bl > br + DRY;
+    } else if (r[0] > DRY && l[0] <= DRY) {
+        h = r[0]; approach = -(r[1] * n[0] + r[2] * n[1]) / h;
+        if (approach <= 0) return 0;
+        om_riemanntype(h, h, approach, -approach, &hstar, &s1m, &s2m, &r1, &r2);
+        return hstar > h + DRY && hstar + br > bl + DRY;
+    }
+    return 0;
+}
+
+/* ------------------------------------------------------------------ */
+om_ctx *om_create(const om_mesh *mesh)
+{
+    om_ctx *x = (om_ctx *)calloc(1, sizeof(om_ctx));
+    x->m = *mesh;
+    const om_mesh *m = &x->m;
+    int nc = m->nc, nf = m->nf, nadj = m->nadj, nb = m->nb;
+    G = m->grav; DRY = m->dry;
+    x->ls_weight = (double *)calloc((size_t)nadj, sizeof(double));
+    x->ls_delta = (double *)calloc((size_t)2 * nadj, sizeof(double));
+    x->ls_matrix = (double *)calloc((size_t)4 * nc, sizeof(double));
+    /* cache_sediment_geometry (:8663-8691) and cache_hydraulic_geometry (:8639-8661) */
+    for (int c = 0; c < nc; ++c) {
+        double a11 = 0, a12 = 0, a22 = 0;
+        for (int k = m->offset[c]; k < m->offset[c + 1]; ++k) {
+            int f = m->faces[k], n = m->signs[k] < 0 ? m->owner[f] : m->neighbour[f];
+            if (n < 0) continue;
+            double r0 = m->xy[n] - m->xy[c], r1 = m->xy[nc + n] - m->xy[nc + c];
+            x->ls_delta[k] = r0; x->ls_delta[nadj + k] = r1;
+            double w = m->length[f] / (r0 * r0 + r1 * r1);
+            x->ls_weight[k] = w;
+            a11 = a11 + w * (r0 * r0); a12 = a12 + w * r0 * r1; a22 = a22 + w * (r1 * r1);
+        }
+        x->ls_matrix[c] = a11; x->ls_matrix[nc + c] = a12; x->ls_matrix[2 * nc + c] = a22;
+        x->ls_matrix[3 * nc + c] = a11 * a22 - a12 * a12;
+    }
+    x->active = (int *)calloc((size_t)nc, sizeof(int));
+    x->hconst = (int *)calloc((size_t)nc, sizeof(int));
+    x->v = (double *)calloc((size_t)4 * nc, sizeof(double));
+    x->grad = (double *)calloc((size_t)8 * nc, sizeof(double));
+    x->vlo = (double *)calloc((size_t)4 * nc, sizeof(double));
+    x->vhi = (double *)calloc((size_t)4 * nc, sizeof(double));
+    x->trace = (double *)calloc((size_t)3 * nf, sizeof(double));
+    x->fl = (double *)calloc((size_t)3 * nf, sizeof(double));
+    x->fr = (double *)calloc((size_t)3 * nf, sizeof(double));
+    x->speed = (double *)calloc((size_t)nf, sizeof(double));
+    x->theta = (double *)calloc((size_t)nc, sizeof(double));
+    x->hydraulic = (double *)calloc((size_t)2 * (nb + 1), sizeof(double));
+    x->target_u = (double *)calloc((size_t)nf + 1, sizeof(double));
+    x->qa = (double *)calloc((size_t)3 * nc, sizeof(double));
+    x->qb = (double *)calloc((size_t)3 * nc, sizeof(double));
+    x->qc = (double *)calloc((size_t)3 * nc, sizeof(double));
+    x->la = (double *)calloc((size_t)nb + 1, sizeof(double));
+    x->lb = (double *)calloc((size_t)nb + 1, sizeof(double));
+    x->lc = (double *)calloc((size_t)nb + 1, sizeof(double));
+    return x;
+}
+
+void om_destroy(om_ctx *x)
+{
+    if (!x) return;
+    free(x->ls_weight); free(x->ls_delta); free(x->ls_matrix);
+    free(x->active); free(x->hconst); free(x->v); free(x->grad); free(x->vlo); free(x->vhi);
+    free(x->trace); free(x->fl); free(x->fr); free(x->speed); free(x->theta);
+    free(x->hydraulic); free(x->target_u); free(x->qa); free(x->qb); free(x->qc);
+    free(x->la); free(x->lb); free(x->lc);
+    free(x);
+}
+
+/* boundary_hydraulic_at: cutcell_solver.f90:9248-9269 (no section roots) */
+static void boundary_hydraulic_at(om_ctx *x, double t)
+{
+    const om_mesh *m = &x->m;
+    for (int b = 0; b < m->nb; ++b) {
+        x->hydraulic[2 * b] = m->forcing_q[b];
+        x->hydraulic[2 * b + 1] = m->forcing_stage[b];
+        if (!m->series_offset) continue;
+        int k0 = m->series_offset[b], k1 = m->series_offset[b + 1], ns = k1 - k0;
+        if (ns == 0) continue;
+        const double *tt = m->series_time + k0, *qq = m->series_q + k0, *ss = m->series_stage + k0;
+        if (t <= tt[0]) { x->hydraulic[2 * b] = qq[0]; x->hydraulic[2 * b + 1] = ss[0]; }
+        else if (t >= tt[ns - 1]) { x->hydraulic[2 * b] = qq[ns - 1]; x->hydraulic[2 * b + 1] = ss[ns - 1]; }
+        else {
+            int k = 1;   /* series_interval (:9231-9246): first knot with t <= tt[k] */
+            while (k < ns - 1 && !(t <= tt[k])) ++k;
+            double w = (t - tt[k - 1]) / (tt[k] - tt[k - 1]);
+            x->hydraulic[2 * b] = (1 - w) * qq[k - 1] + w * qq[k];
+            x->hydraulic[2 * b + 1] = (1 - w) * ss[k - 1] + w * ss[k];
+        }
+    }
+}
+
+/* face_value for scheme 1/5 (no WENO stencils on triangles): :8999-9051.
+ * out = (eta, u, v, z) at face f seen from cell c. */
+static void face_value(const om_ctx *x, int c, int f, double out[4])
+{
+    const om_mesh *m = &x->m;
+    int nc = m->nc, nf = m->nf;
+    for (int j = 0; j < 4; ++j) out[j] = x->v[j * nc + c];
+    if (m->scheme == 1) return;
+    double r0 = m->mid[f] - m->xy[c], r1 = m->mid[nf + f] - m->xy[nc + c];
+    for (int j = 0; j < 4; ++j)
+        out[j] = out[j] + x->grad[(2 * j) * nc + c] * r0 + x->grad[(2 * j + 1) * nc + c] * r1;
+    if (x->hconst[c]) {
+        for (int j = 0; j < 3; ++j) out[j] = x->v[j * nc + c];
+        if (x->v[c] - x->v[3 * nc + c] <= DRY) out[3] = x->v[3 * nc + c];
+    }
+}
+
+/* primitive (:7787-7817) and the active mask (:10795-10812). */
+static void primitive_and_active(om_ctx *x, const double *q, const double *z)
+{
+    const om_mesh *m = &x->m;
+    int nc = m->nc;
+    for (int c = 0; c < nc; ++c) {
+        x->v[c] = q[c] + z[c]; x->v[nc + c] = 0; x->v[2 * nc + c] = 0; x->v[3 * nc + c] = z[c];
+        if (q[c] > DRY) { x->v[nc + c] = q[nc + c] / q[c]; x->v[2 * nc + c] = q[2 * nc + c] / q[c]; }
+        int act = q[c] > DRY;
+        if (!act) {
+            for (int k = m->offset[c]; k < m->offset[c + 1]; ++k) {
+                int f = m->faces[k], n = m->signs[k] < 0 ? m->owner[f] : m->neighbour[f];
+                if (m->bid[f] >= 0) { act = 1; break; }
+                if (n >= 0 && q[n] > DRY) { act = 1; break; }
+            }
+        }
+        x->active[c] = act;
+    }
+}
+
+/* gradients: :8536-8623 (cpu_outlet_extension = 0 for SSPRK33) */
+static void gradients(om_ctx *x)
+{
+    const om_mesh *m = &x->m;
+    int nc = m->nc, nf = m->nf, nadj = m->nadj;
+    for (int j = 0; j < 8 * nc; ++j) x->grad[j] = 0;
+    memcpy(x->vlo, x->v, sizeof(double) * 4 * nc);
+    memcpy(x->vhi, x->v, sizeof(double) * 4 * nc);
+    if (m->scheme == 1) return;
+    for (int c = 0; c < nc; ++c) {
+        if (!x->active[c]) continue;
+        double a11 = x->ls_matrix[c], a12 = x->ls_matrix[nc + c], a22 = x->ls_matrix[2 * nc + c];
+        double b0[4] = {0, 0, 0, 0}, b1[4] = {0, 0, 0, 0}, vc[4], lo[4], hi[4];
+        for (int j = 0; j < 4; ++j) { vc[j] = x->v[j * nc + c]; lo[j] = vc[j]; hi[j] = vc[j]; }
+        for (int k = m->offset[c]; k < m->offset[c + 1]; ++k) {
+            int f = m->faces[k], n = m->signs[k] < 0 ? m->owner[f] : m->neighbour[f];
+            if (n < 0) continue;
+            double r0 = x->ls_delta[k], r1 = x->ls_delta[nadj + k], w = x->ls_weight[k];
+            double adj[4];
+            for (int j = 0; j < 4; ++j) adj[j] = x->v[j * nc + n];
+            if (vc[0] - vc[3] > DRY && adj[0] - adj[3] <= DRY && adj[3] >= vc[0]) adj[0] = vc[0];
+            for (int j = 0; j < 4; ++j) {
+                b0[j] = b0[j] + w * r0 * (adj[j] - vc[j]);
+                b1[j] = b1[j] + w * r1 * (adj[j] - vc[j]);
+                lo[j] = dmin(lo[j], adj[j]); hi[j] = dmax(hi[j], adj[j]);
+            }
+        }
+        for (int j = 0; j < 4; ++j) { x->vlo[j * nc + c] = lo[j]; x->vhi[j * nc + c] = hi[j]; }
+        double det = x->ls_matrix[3 * nc + c];
+        if (det <= 1e-12 * ((a11 + a22) * (a11 + a22))) continue;
+        double gx[4], gy[4], guard[4], theta[4] = {1, 1, 1, 1};
+        for (int j = 0; j < 4; ++j) {
+            gx[j] = (a22 * b0[j] - a12 * b1[j]) / det;
+            gy[j] = (a11 * b1[j] - a12 * b0[j]) / det;
+            guard[j] = 64 * EPS * dmax(dmax(1.0, fabs(vc[j])), dmax(fabs(lo[j]), fabs(hi[j])));
+        }
+        for (int k = m->offset[c]; k < m->offset[c + 1]; ++k) {
+            int f = m->faces[k];
+            double r0 = m->mid[f] - m->xy[c], r1 = m->mid[nf + f] - m->xy[nc + c], delta[4];
+            for (int j = 0; j < 4; ++j) delta[j] = gx[j] * r0 + gy[j] * r1;
+            if (m->neighbour[f] < 0 && m->bid[f] >= 0 && m->mode[m->bid[f]] == 2 && vc[0] - vc[3] > DRY) {
+                if (fabs(vc[1] * m->normal[f] + vc[2] * m->normal[nf + f]) < sqrt(G * (vc[0] - vc[3]))) {
+                    if (fabs(delta[3]) > 64 * EPS * dmax(1.0, fabs(vc[3]))) continue;
+                }
+            }
+            for (int j = 0; j < 4; ++j) {
+                if (fabs(delta[j]) <= guard[j]) continue;
+                if (delta[j] > 0) theta[j] = dmin(theta[j], (hi[j] - vc[j]) / delta[j]);
+                if (delta[j] < 0) theta[j] = dmin(theta[j], (lo[j] - vc[j]) / delta[j]);
+            }
+        }
+        for (int j = 0; j < 4; ++j) {
+            double t = dmax(0.0, theta[j]);
+            x->grad[(2 * j) * nc + c] = gx[j] * t;
+            x->grad[(2 * j + 1) * nc + c] = gy[j] * t;
+        }
+    }
+}
+
+/* prepare_wet_dry_traces: :9097-9133 */
+static void wet_dry_traces(om_ctx *x, const double *q)
+{
+    const om_mesh *m = &x->m;
+    int nc = m->nc, nf = m->nf;
+    for (int c = 0; c < nc; ++c) x->hconst[c] = 0;
+    if (m->scheme == 1) return;
+    for (int c = 0; c < nc; ++c) {
+        double h = q[c];
+        int flatten = h <= DRY;
+        if (!flatten) {
+            for (int k = m->offset[c]; k < m->offset[c + 1]; ++k) {
+                int f = m->faces[k];
+                double r0 = m->mid[f] - m->xy[c], r1 = m->mid[nf + f] - m->xy[nc + c];
+                double eta = x->v[c] + x->grad[c] * r0 + x->grad[nc + c] * r1;
+                double zb = x->v[3 * nc + c] + x->grad[6 * nc + c] * r0 + x->grad[7 * nc + c] * r1;
+                if (eta - zb <= DRY || (m->irregular[c] && fabs(zb - x->v[3 * nc + c]) >= h)) {
+                    flatten = 1; break;
+                }
+            }
+        }
+        x->hconst[c] = flatten;
+    }
+}
+
+/* traces for total-discharge (6) and stage (2) boundaries: :9814-10270 */
+static int traces(om_ctx *x, const double *q, const double *z, int *failure)
+{
+    const om_mesh *m = &x->m;
+    int nc = m->nc, nf = m->nf;
+    for (int j = 0; j < 3 * nf; ++j) x->trace[j] = 0;
+    for (int b = 0; b < m->nb; ++b) {
+        double desired = x->hydraulic[2 * b];
+        int k0 = m->boundary_offset[b], k1 = m->boundary_offset[b + 1];
+        if (m->mode[b] == 6) {
+            double zavg = 0, wet_length = 0, total = 0;
+            for (int k = k0; k < k1; ++k) {
+                int f = m->boundary_faces[k], o = m->owner[f];
+                if (q[o] <= DRY) continue;
+                zavg = zavg + (z[o] + q[o]) * m->length[f];
+                wet_length = wet_length + m->length[f];
+            }
+            if (desired > 0 && wet_length <= 0) { *failure = 8; return 0; }
+            if (wet_length > 0) zavg = zavg / wet_length;
+            double *target_u = x->target_u; /* indexed by position in the boundary list */
+            for (int k = k0; k < k1; ++k) {
+                int f = m->boundary_faces[k], o = m->owner[f];
+                target_u[k - k0] = 0;
+                double h = q[o];
+                if (h <= DRY) continue;
+                double weight = pow(dmax(zavg - z[o], 0.0), 1.5) * pow(dmax(h, MANNING_MIN_DEPTH), 1.0 / 6.0);
+                if (m->manning[o] > 0) weight = weight / m->manning[o];
+                target_u[k - k0] = weight * m->length[f];
+                total = total + target_u[k - k0];
+            }
+            if (desired > 0 && total <= 0) { *failure = 8; return 0; }
+            for (int k = k0; k < k1; ++k) {
+                int f = m->boundary_faces[k], o = m->owner[f];
+                double nx = m->normal[f], ny = m->normal[nf + f], h = q[o];
+                if (h <= DRY || desired == 0) {
+                    x->trace[f] = 0; x->trace[nf + f] = 0; x->trace[2 * nf + f] = 0;
+                    if (h > DRY) {
+                        double d = q[nc + o] * nx + q[2 * nc + o] * ny;
+                        double tx = q[nc + o] / q[o] - d / q[o] * nx, ty = q[2 * nc + o] / q[o] - d / q[o] * ny;
+                        x->trace[f] = h; x->trace[nf + f] = h * tx; x->trace[2 * nf + f] = h * ty;
+                    }
+                    continue;
+                }
+                double ub = desired * target_u[k - k0] / (total * h * m->length[f]);
+                double d = q[nc + o] * nx + q[2 * nc + o] * ny;
+                double tx = q[nc + o] / q[o] - d / q[o] * nx, ty = q[2 * nc + o] / q[o] - d / q[o] * ny;
+                if (sqrt(ub * ub + (tx * tx + ty * ty)) > SPEED_LIMIT) { *failure = 6; return 0; }
+                x->trace[f] = h;
+                x->trace[nf + f] = h * (-(ub * nx) + tx);
+                x->trace[2 * nf + f] = h * (-(ub * ny) + ty);
+            }
+            continue;
+        }
+        if (m->mode[b] != 2) { *failure = 99; return 0; }
+        double stage = x->hydraulic[2 * b + 1];
+        for (int k = k0; k < k1; ++k) {
+            int f = m->boundary_faces[k], o = m->owner[f];
+            double nx = m->normal[f], ny = m->normal[nf + f];
+            double es[3] = {q[o], q[nc + o], q[2 * nc + o]}, edge_bed = z[o];
+            if (q[o] > DRY) {
+                double e[4];
+                face_value(x, o, f, e);
+                if (fabs(e[3] - z[o]) > 64 * EPS * dmax(1.0, fabs(z[o])) &&
+                    fabs(q[nc + o] * nx + q[2 * nc + o] * ny) < q[o] * sqrt(G * q[o])) {
+                    edge_bed = e[3];
+                    es[0] = dmax(e[0] - edge_bed, 0.0);
+                    es[1] = es[0] * e[1]; es[2] = es[0] * e[2];
+                }
+            }
+            double h = es[0], u = 0;
+            if (h > DRY) u = -(es[1] * nx + es[2] * ny) / h;
+            double c = sqrt(G * h), invariant = u - 2 * c;
+            if (h <= DRY) continue;
+            if (u <= -c) { x->trace[f] = es[0]; x->trace[nf + f] = es[1]; x->trace[2 * nf + f] = es[2]; continue; }
+            if (u >= c) { x->trace[f] = es[0]; x->trace[nf + f] = 0; x->trace[2 * nf + f] = 0; continue; }
+            h = dmax(stage - edge_bed, 0.0);
+            if (h <= DRY || invariant <= -3 * sqrt(G * h)) {
+                double t3 = invariant / 3;
+                h = (t3 * t3) / G;
+                if (h <= DRY) { x->trace[f] = es[0]; x->trace[nf + f] = 0; x->trace[2 * nf + f] = 0; continue; }
+                double ub = -sqrt(G * h);
+                double tx = es[1] / es[0] + u * nx, ty = es[2] / es[0] + u * ny;
+                x->trace[f] = h; x->trace[nf + f] = h * (-(ub * nx) + tx); x->trace[2 * nf + f] = h * (-(ub * ny) + ty);
+                continue;
+            }
+            double ub = invariant + 2 * sqrt(G * h);
+            if (ub > 0) { x->trace[f] = es[0]; x->trace[nf + f] = 0; x->trace[2 * nf + f] = 0; continue; }
+            if (fabs(ub) >= sqrt(G * h)) { *failure = 8; return 0; }
+            double tx = es[1] / es[0] + u * nx, ty = es[2] / es[0] + u * ny;
+            x->trace[f] = h; x->trace[nf + f] = h * (-(ub * nx) + tx); x->trace[2 * nf + f] = h * (-(ub * ny) + ty);
+        }
+    }
+    return 1;
+}
+
+/* Face loop of euler (:10938-11110) for hydraulics only. */
+static void face_fluxes(om_ctx *x, const double *q, const double *z)
+{
+    const om_mesh *m = &x->m;
+    int nc = m->nc, nf = m->nf;
+    const double g = G;
+    for (int j = 0; j < 3 * nf; ++j) { x->fl[j] = 0; x->fr[j] = 0; }
+    for (int f = 0; f < nf; ++f) x->speed[f] = 0;
+    for (int f = 0; f < nf; ++f) {
+        int c = m->owner[f], n = m->neighbour[f], b = m->bid[f], closed = 0;
+        double nrm[2] = {m->normal[f], m->normal[nf + f]};
+        double fl[3] = {0, 0, 0}, fr[3] = {0, 0, 0}, speed = 0, left[4], right[4], l[3], r[3];
+        if (b < 0 && q[c] <= DRY) {
+            if (n < 0) continue;
+            if (q[n] <= DRY) continue;
+        }
+        face_value(x, c, f, left);
+        double depth = dmax(left[0] - left[3], 0.0);
+        if (q[c] <= DRY) depth = 0;
+        if (x->hconst[c] && depth <= DRY) depth = 0;
+        l[0] = depth; l[1] = depth * left[1]; l[2] = depth * left[2];
+        if (n >= 0) {
+            face_value(x, n, f, right);
+            depth = dmax(right[0] - right[3], 0.0);
+            if (q[n] <= DRY) depth = 0;
+            if (x->hconst[n] && depth <= DRY) depth = 0;
+            r[0] = depth; r[1] = depth * right[1]; r[2] = depth * right[2];
+            if (dmin(q[c], q[n]) <= DRY) {
+                double ofd = fabs((m->mid[f] - m->xy[c]) * nrm[0] + (m->mid[nf + f] - m->xy[nc + c]) * nrm[1]);
+                double nfd = fabs((m->xy[n] - m->mid[f]) * nrm[0] + (m->xy[nc + n] - m->mid[nf + f]) * nrm[1]);
+                double shared;
+                if (ofd + nfd > DBL_MIN)
+                    shared = (nfd * (z[c] + q[c]) + ofd * (z[n] + q[n])) / (ofd + nfd) - dmax(z[c], z[n]);
+                else
+                    shared = .5 * ((z[c] + q[c]) + (z[n] + q[n])) - dmax(z[c], z[n]);
+                closed = shared <= DRY;
+                if (closed) closed = !runup_opens_face(l, r, left[3], right[3], nrm);
+            }
+            if (closed) {
+                fl[0] = 0; fr[0] = 0;
+                fl[1] = .5 * g * (l[0] * l[0]) * nrm[0]; fl[2] = .5 * g * (l[0] * l[0]) * nrm[1];
+                fr[1] = .5 * g * (r[0] * r[0]) * nrm[0]; fr[2] = .5 * g * (r[0] * r[0]) * nrm[1];
+                speed = 0;
+                if (l[0] > DRY) speed = dmax(speed, fabs((l[1] * nrm[0] + l[2] * nrm[1]) / l[0]) + sqrt(g * l[0]));
+                if (r[0] > DRY) speed = dmax(speed, fabs((r[1] * nrm[0] + r[2] * nrm[1]) / r[0]) + sqrt(g * r[0]));
+            } else {
+                om_wave_flux(l, r, left[3], right[3], nrm, fl, fr, &speed, NULL);
+            }
+        } else if (b < 0) {
+            double d = l[1] * nrm[0] + l[2] * nrm[1];
+            r[0] = l[0]; r[1] = l[1] - 2 * d * nrm[0]; r[2] = l[2] - 2 * d * nrm[1];
+            om_wave_flux(l, r, left[3], left[3], nrm, fl, fr, &speed, NULL);
+            fl[0] = 0;
+            double p = fl[1] * nrm[0] + fl[2] * nrm[1];
+            fl[1] = p * nrm[0]; fl[2] = p * nrm[1];
+            fr[0] = fl[0]; fr[1] = fl[1]; fr[2] = fl[2];
+        } else {
+            l[0] = x->trace[f]; l[1] = x->trace[nf + f]; l[2] = x->trace[2 * nf + f];
+            depth = l[0];
+            fl[0] = l[1] * nrm[0] + l[2] * nrm[1];
+            if (depth > DRY) {
+                fl[1] = fl[0] * l[1] / depth + .5 * g * (depth * depth) * nrm[0];
+                fl[2] = fl[0] * l[2] / depth + .5 * g * (depth * depth) * nrm[1];
+            }
+            fr[0] = fl[0]; fr[1] = fl[1]; fr[2] = fl[2];
+            speed = sqrt(g * depth);
+            if (depth > DRY) speed = speed + fabs(fl[0]) / depth;
+        }
+        /* well-balanced bed-source completion :11063-11110 */
+        if (n >= 0 || b < 0) {
+            if (q[c] > DRY) {
+                double corr;
+                if (x->hconst[c]) corr = .5 * g * (q[c] * q[c] - l[0] * l[0]);
+                else corr = .5 * g * (q[c] + dmax(left[0] - left[3], 0.0)) * (left[3] - z[c]);
+                fl[1] = fl[1] + corr * nrm[0]; fl[2] = fl[2] + corr * nrm[1];
+            }
+            if (n >= 0) {
+                if (q[n] > DRY) {
+                    double corr;
+                    if (x->hconst[n]) corr = .5 * g * (q[n] * q[n] - r[0] * r[0]);
+                    else corr = .5 * g * (q[n] + dmax(right[0] - right[3], 0.0)) * (right[3] - z[n]);
+                    fr[1] = fr[1] + corr * nrm[0]; fr[2] = fr[2] + corr * nrm[1];
+                }
+            } else {
+                fr[0] = fl[0]; fr[1] = fl[1]; fr[2] = fl[2];
+            }
+        } else if (m->mode[b] == 2) {
+            if (q[c] > DRY) {
+                if (fabs(left[3] - z[c]) > 64 * EPS * dmax(1.0, fabs(z[c])) &&
+                    fabs(q[nc + c] * nrm[0] + q[2 * nc + c] * nrm[1]) < q[c] * sqrt(g * q[c])) {
+                    double d = dmax(left[0] - left[3], 0.0), corr;
+                    if (x->hconst[c]) corr = .5 * g * (q[c] * q[c] - d * d);
+                    else corr = .5 * g * (q[c] + d) * (left[3] - z[c]);
+                    fl[1] = fl[1] + corr * nrm[0]; fl[2] = fl[2] + corr * nrm[1];
+                }
+            }
+            fr[0] = fl[0]; fr[1] = fl[1]; fr[2] = fl[2];
+        }
+        for (int j = 0; j < 3; ++j) { x->fl[j * nf + f] = fl[j]; x->fr[j * nf + f] = fr[j]; }
+        x->speed[f] = speed;
+    }
+}
+
+static int spatial_prefix(om_ctx *x, const double *q, const double *z, double t, int *failure)
+{
+    primitive_and_active(x, q, z);
+    boundary_hydraulic_at(x, t);
+    gradients(x);
+    wet_dry_traces(x, q);
+    if (!traces(x, q, z, failure)) return 0;
+    face_fluxes(x, q, z);
+    return 1;
+}
+
+int om_euler(om_ctx *x, const double *q, const double *z, double dt, double t,
+             double *next, double *face_water, double *speed_out, double *ledger,
+             double *stage_cfl_peak, int *failure)
+{
+    const om_mesh *m = &x->m;
+    int nc = m->nc, nf = m->nf;
+    const double g = G;
+    int fail = 0;
+    memcpy(next, q, sizeof(double) * 3 * nc);
+    if (ledger) for (int b = 0; b < m->nb; ++b) ledger[b] = 0;
+    if (stage_cfl_peak) *stage_cfl_peak = 0;
+    if (!spatial_prefix(x, q, z, t, &fail)) { if (failure) *failure = fail; return 0; }
+    /* draining :11521-11557 */
+    for (int c = 0; c < nc; ++c) {
+        x->theta[c] = 1;
+        if (!x->active[c]) continue;
+        double outgoing = 0;
+        for (int k = m->offset[c]; k < m->offset[c + 1]; ++k) {
+            int f = m->faces[k];
+            outgoing = outgoing + dmax(m->signs[k] * x->fl[f], 0.0) * m->length[f];
+        }
+        if (dt * outgoing >= q[c] * m->area[c] && outgoing > 0)
+            x->theta[c] = dmin(1.0, q[c] * m->area[c] / (dt * outgoing)) * (1 - 32 * EPS);
+    }
+    for (int f = 0; f < nf; ++f) {
+        if (x->fl[f] == 0) continue;
+        int donor = x->fl[f] < 0 ? m->neighbour[f] : m->owner[f];
+        if (donor < 0) continue;
+        if (x->theta[donor] == 1) continue;
+        if (m->bid[f] >= 0) { fail = 2; continue; }
+        double corr = (x->theta[donor] - 1) * x->fl[f], vx = 0, vy = 0;
+        if (q[donor] > DRY) { vx = q[nc + donor] / q[donor]; vy = q[2 * nc + donor] / q[donor]; }
+        x->fl[f] = x->fl[f] * x->theta[donor]; x->fr[f] = x->fl[f];
+        x->fl[nf + f] = x->fl[nf + f] + corr * vx; x->fl[2 * nf + f] = x->fl[2 * nf + f] + corr * vy;
+        x->fr[nf + f] = x->fr[nf + f] + corr * vx; x->fr[2 * nf + f] = x->fr[2 * nf + f] + corr * vy;
+    }
+    if (fail) { if (failure) *failure = fail; return 0; }
+    /* gather + CFL :11564-11600 */
+    double hmax = 0;
+    for (int c = 0; c < nc; ++c) hmax = c == 0 ? q[c] : dmax(hmax, q[c]);
+    double depth_tol = 64 * EPS * dmax(hmax, 1.0), peak = 0;
+    int stage_failure = 0;
+    for (int c = 0; c < nc; ++c) {
+        if (!x->active[c]) continue;
+        double s[3] = {0, 0, 0}, cfl = 0, p = .5 * g * (q[c] * q[c]);
+        for (int k = m->offset[c]; k < m->offset[c + 1]; ++k) {
+            int f = m->faces[k];
+            double w = m->length[f] / m->area[c];
+            cfl = cfl + m->length[f] * x->speed[f];
+            if (m->signs[k] > 0) {
+                s[0] = s[0] + w * x->fl[f];
+                s[1] = s[1] + w * (x->fl[nf + f] - p * m->normal[f]);
+                s[2] = s[2] + w * (x->fl[2 * nf + f] - p * m->normal[nf + f]);
+            } else {
+                s[0] = s[0] - w * x->fr[f];
+                s[1] = s[1] - w * (x->fr[nf + f] - p * m->normal[f]);
+                s[2] = s[2] - w * (x->fr[2 * nf + f] - p * m->normal[nf + f]);
+            }
+        }
+        double ratio = dt * cfl / m->area[c];
+        peak = dmax(peak, ratio);
+        if (ratio > m->cfl_target) { if (stage_failure < 2) stage_failure = 2; continue; }
+        for (int j = 0; j < 3; ++j) next[j * nc + c] = q[j * nc + c] - dt * s[j];
+        if (next[c] < 0) {
+            if (-next[c] <= depth_tol) { next[c] = 0; next[nc + c] = 0; next[2 * nc + c] = 0; }
+            else stage_failure = 3;
+        }
+    }
+    if (stage_cfl_peak) *stage_cfl_peak = peak;
+    if (stage_failure) {
+        if (failure) *failure = stage_failure;
+        if (stage_failure == 2 && stage_cfl_peak) {
+            /* rejected-trial diagnostic: exact maximum over every cell (:11614-11625) */
+            double worst = -DBL_MAX;
+            for (int c = 0; c < nc; ++c) {
+                double cfl = 0;
+                for (int k = m->offset[c]; k < m->offset[c + 1]; ++k) cfl = cfl + m->length[m->faces[k]] * x->speed[m->faces[k]];
+                if (dt * cfl / m->area[c] > worst) worst = dt * cfl / m->area[c];
+            }
+            *stage_cfl_peak = worst;
+        }
+        return 0;
+    }
+    for (int j = 0; j < 3 * nc; ++j)
+        if (!isfinite(next[j])) { if (failure) *failure = 5; return 0; }
+    if (ledger)
+        for (int f = 0; f < nf; ++f) {
+            int b = m->bid[f];
+            if (b < 0) continue;
+            ledger[b] = ledger[b] - dt * m->length[f] * x->fl[f];
+        }
+    if (face_water) memcpy(face_water, x->fl, sizeof(double) * nf);
+    if (speed_out) memcpy(speed_out, x->speed, sizeof(double) * nf);
+    if (failure) *failure = 0;
+    return 1;
+}
+
+int om_rates(om_ctx *x, const double *q, const double *z, double dt, double t,
+             double *rate, double *water)
+{
+    const om_mesh *m = &x->m;
+    int nc = m->nc, nf = m->nf, fail = 0;
+    (void)dt;
+    if (!spatial_prefix(x, q, z, t, &fail)) return 0;
+    for (int c = 0; c < nc; ++c) {
+        double s[3] = {0, 0, 0}, p = .5 * G * (q[c] * q[c]);
+        for (int k = m->offset[c]; k < m->offset[c + 1]; ++k) {
+            int f = m->faces[k];
+            double w = m->length[f] / m->area[c];
+            if (m->signs[k] > 0) {
+                s[0] = s[0] + w * x->fl[f];
+                s[1] = s[1] + w * (x->fl[nf + f] - p * m->normal[f]);
+                s[2] = s[2] + w * (x->fl[2 * nf + f] - p * m->normal[nf + f]);
+            } else {
+                s[0] = s[0] - w * x->fr[f];
+                s[1] = s[1] - w * (x->fr[nf + f] - p * m->normal[f]);
+                s[2] = s[2] - w * (x->fr[2 * nf + f] - p * m->normal[nf + f]);
+            }
+        }
+        for (int j = 0; j < 3; ++j) rate[j * nc + c] = -s[j];
+    }
+    if (water) memcpy(water, x->fl, sizeof(double) * nf);
+    for (int j = 0; j < 3 * nc; ++j) if (!isfinite(rate[j])) return 0;
+    return 1;
+}
+
+void om_prepare_stage(om_ctx *x, double *q)
+{
+    int nc = x->m.nc;
+    for (int c = 0; c < nc; ++c) {
+        if (q[c] <= DRY) { q[nc + c] = 0; q[2 * nc + c] = 0; }
+        else {
+            double speed = sqrt(q[nc + c] * q[nc + c] + q[2 * nc + c] * q[2 * nc + c]) / q[c];
+            if (speed > SPEED_LIMIT) {
+                q[nc + c] = q[nc + c] * (SPEED_LIMIT / speed);
+                q[2 * nc + c] = q[2 * nc + c] * (SPEED_LIMIT / speed);
+            }
+        }
+    }
+}
+
+void om_manning(om_ctx *x, double *q, double dt)
+{
+    int nc = x->m.nc;
+    for (int c = 0; c < nc; ++c) {
+        double h = q[c];
+        if (h <= DRY) { q[nc + c] = 0; q[2 * nc + c] = 0; continue; }
+        double speed = sqrt(q[nc + c] * q[nc + c] + q[2 * nc + c] * q[2 * nc + c]) / h;
+        double n = x->m.manning[c];
+        double d = 1 + dt * G * (n * n) * speed / pow(h, 4.0 / 3);
+        q[nc + c] = q[nc + c] / d; q[2 * nc + c] = q[2 * nc + c] / d;
+    }
+}
+
+int om_step_ssp33(om_ctx *x, double *q, const double *z, double dt, double t,
+                  double *ledger, double *trial_cfl_peak, int *failure)
+{
+    const om_mesh *m = &x->m;
+    int nc = m->nc, nb = m->nb;
+    double pk, peak = 0;
+    int fail = 0;
+    if (failure) *failure = 0;
+    om_prepare_stage(x, q);
+    int ok = om_euler(x, q, z, dt, t, x->qa, NULL, NULL, x->la, &pk, &fail);
+    peak = dmax(peak, pk);
+    if (!ok) goto rejected;
+    om_prepare_stage(x, x->qa);
+    ok = om_euler(x, x->qa, z, dt, t + dt, x->qb, NULL, NULL, x->lb, &pk, &fail);
+    peak = dmax(peak, pk);
+    if (!ok) goto rejected;
+    for (int j = 0; j < 3 * nc; ++j) x->qb[j] = .75 * q[j] + .25 * x->qb[j];
+    om_prepare_stage(x, x->qb);
+    ok = om_euler(x, x->qb, z, dt, t + .5 * dt, x->qc, NULL, NULL, x->lc, &pk, &fail);
+    peak = dmax(peak, pk);
+    if (!ok) goto rejected;
+    for (int j = 0; j < 3 * nc; ++j) x->qc[j] = q[j] / 3.0 + 2.0 * x->qc[j] / 3.0;
+    om_prepare_stage(x, x->qc);
+    om_manning(x, x->qc, dt);
+    for (int c = 0; c < 3 * nc; ++c)
+        if (!isfinite(x->qc[c])) { fail = 5; goto rejected; }
+    for (int c = 0; c < nc; ++c)
+        if (x->qc[c] < 0) { fail = 5; goto rejected; }
+    memcpy(q, x->qc, sizeof(double) * 3 * nc);
+    if (ledger) for (int b = 0; b < nb; ++b) ledger[b] = ledger[b] + (x->la[b] + x->lb[b]) / 6.0 + 2.0 * x->lc[b] / 3.0;
+    if (trial_cfl_peak) *trial_cfl_peak = peak;
+    return 1;
+rejected:
+    if (trial_cfl_peak) *trial_cfl_peak = peak;
+    if (failure) *failure = fail;
+    return 0;
+}
+
+/* next_boundary_knot (:9270-9294) for hydrograph knots */
+static double next_knot(const om_ctx *x, double t)
+{
+    const om_mesh *m = &x->m;
+    double knot = DBL_MAX, tol = 32 * EPS * dmax(1.0, fabs(t));
+    if (!m->series_offset) return knot;
+    for (int b = 0; b < m->nb; ++b)
+        for (int k = m->series_offset[b]; k < m->series_offset[b + 1]; ++k)
+            if (m->series_time[k] > t + tol) knot = dmin(knot, m->series_time[k]);
+    return knot;
+}
+
+int om_advance(om_ctx *x, double *q, const double *z, int nsteps, double tend,
+               double *time, double *step_hint, double *ledger, int *retries)
+{
+    const om_mesh *m = &x->m;
+    int accepted = 0;
+    if (retries) *retries = 0;
+    while (accepted < nsteps && *time < tend - 1e-12) {
+        double unclipped = *step_hint, dt = dmin(unclipped, tend - *time);
+        double knot = next_knot(x, *time);
+        if (knot < DBL_MAX) dt = dmin(dt, knot - *time);
+        int clipped = dt < unclipped, ok = 0, attempt, fail = 0;
+        double peak = 0;
+        for (attempt = 1; attempt <= 35; ++attempt) {
+            if (*time + dt <= *time) return -accepted - 1;
+            ok = om_step_ssp33(x, q, z, dt, *time, ledger, &peak, &fail);
+            if (ok) break;
+            if (retries) ++*retries;
+            if (fail == 2 && peak > m->cfl_target) {
+                double s = dmax(.1, dmin(.95, .95 * m->cfl_target / peak));
+                dt = s * dt;
+            } else {
+                dt = .5 * dt;
+            }
+        }
+        if (!ok) return -accepted - 1;
+        if (attempt == 1 && clipped) *step_hint = unclipped;
+        else if (peak > 0) {
+            double s = dmax(.5, dmin(1.05, .95 * m->cfl_target / peak));
+            *step_hint = dmin(m->dtmax, s * dt);
+        } else *step_hint = dmin(m->dtmax, 1.05 * dt);
+        *time = *time + dt;
+        ++accepted;
+    }
+    return accepted;
+}
+
+double om_mass(const om_ctx *x, const double *q)
+{
+    double s = 0, comp = 0;
+    for (int c = 0; c < x->m.nc; ++c) {
+        double v = x->m.area[c] * q[c], t = s + v;
+        if (fabs(s) >= fabs(v)) comp += (s - t) + v; else comp += (v - t) + s;
+        s = t;
+    }
+    return s + comp;
+}

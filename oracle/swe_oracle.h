@@ -1,0 +1,88 @@
+/*
+ * Oracle (test infrastructure only): a plain-C FP64 restatement of the
+ * BRAIDS-HKT cut-cell hydraulic time step.  Only tests/, the smoke check and
+ * the CPU-baseline leg of bench.py may load it; it is the checker, never the
+ * product.  Every routine cites the reference lines it follows.
+ *
+ * Index conventions are zero-based.  Cell arrays are component-major SoA
+ * (q[m*nc+c]); face vectors likewise (normal[nf+f] is the y component).
+ * neighbour[f] = -1 marks an exterior face; bid[f] = -1 marks a wall.
+ */
+#ifndef BRAIDS_SWE_ORACLE_H
+#define BRAIDS_SWE_ORACLE_H
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct {
+    int nc, nf, nb, nadj;
+    int scheme;                 /* 1 first order, 5 LS-MUSCL (cutcell_solver.f90:4470) */
+    double grav, dry, cfl_target, dtmax;
+    const double *area;         /* [nc] */
+    const double *xy;           /* [2*nc] */
+    const double *manning;      /* [nc] */
+    const int *irregular;       /* [nc] 1 unless a verified rectangle (classify_rectangles) */
+    const int *owner, *neighbour, *bid;   /* [nf] */
+    const double *normal;       /* [2*nf] */
+    const double *length;       /* [nf] */
+    const double *mid;          /* [2*nf] */
+    const int *offset;          /* [nc+1] CSR, ascending face id per cell */
+    const int *faces;           /* [nadj] */
+    const int *signs;           /* [nadj] +1 owner (outward), -1 neighbour */
+    const int *mode;            /* [nb] 2 = stage outlet, 6 = total-discharge inlet */
+    const int *boundary_offset; /* [nb+1] */
+    const int *boundary_faces;  /* ascending face id per boundary */
+    const double *forcing_q;    /* [nb] constant total discharge (m^3/s) */
+    const double *forcing_stage;/* [nb] constant water level (m) */
+    const int *series_offset;   /* [nb+1] hydrograph knots, may be NULL */
+    const double *series_time, *series_q, *series_stage;
+} om_mesh;
+
+typedef struct om_ctx om_ctx;
+
+om_ctx *om_create(const om_mesh *mesh);
+void om_destroy(om_ctx *ctx);
+
+/* One GeoClaw augmented f-wave face flux: wave_flux, cutcell_solver.f90:9135-9159. */
+void om_wave_flux(const double l[3], const double r[3], double bl, double br,
+                  const double n[2], double fl[3], double fr[3], double *speed,
+                  double *rightmost);
+/* riemanntype, geoclaw_riemann_utils.f:547-658 (maxiter = 1). */
+void om_riemanntype(double hL, double hR, double uL, double uR, double *hm,
+                    double *s1m, double *s2m, int *rare1, int *rare2);
+
+/* One explicit Euler stage (euler, cutcell_solver.f90:10631-12090, hydraulic part).
+ * Returns 1 on success, 0 on rejection (failure code in *failure).
+ * next[3*nc]; face_water (fl(1,:)) [nf], speed [nf], ledger [nb] optional. */
+int om_euler(om_ctx *ctx, const double *q, const double *z, double dt, double t,
+             double *next, double *face_water, double *speed, double *ledger,
+             double *stage_cfl_peak, int *failure);
+
+/* Spatial rate without draining or CFL rejection: bend_native_rates
+ * (bend_native_bridge.f90:38-68 + bend_native_rates_tail.inc:1-23). */
+int om_rates(om_ctx *ctx, const double *q, const double *z, double dt, double t,
+             double *rate, double *water);
+
+/* prepare_hydraulic_stage, cutcell_solver.f90:8513-8534 (in place). */
+void om_prepare_stage(om_ctx *ctx, double *q);
+/* Manning part of source, cutcell_solver.f90:14580-14600 (in place). */
+void om_manning(om_ctx *ctx, double *q, double dt);
+
+/* advance_ssp33, cutcell_solver.f90:15858-15986 (hydraulic part).  q is
+ * updated in place on success.  ledger[nb] accumulates inward boundary water. */
+int om_step_ssp33(om_ctx *ctx, double *q, const double *z, double dt, double t,
+                  double *ledger, double *trial_cfl_peak, int *failure);
+
+/* Accepted-step loop with retries and step-hint control
+ * (cutcell_solver.f90:17287-17370), for nsteps accepted steps or until tend.
+ * Returns the number of accepted steps; *time and *step_hint are updated. */
+int om_advance(om_ctx *ctx, double *q, const double *z, int nsteps, double tend,
+               double *time, double *step_hint, double *ledger, int *retries);
+
+/* Mass inventory sum(A h) with compensated (Neumaier) summation. */
+double om_mass(const om_ctx *ctx, const double *q);
+
+#ifdef __cplusplus
+}
+#endif
+#endif
