@@ -1,0 +1,94 @@
+"""ctypes binding of libbraids_hip.so (the C ABI of include/braids_hip.h and
+include/braids_engine.h).  There is no fallback: if the library is missing
+the import raises, so a GPU run can never silently take another path."""
+import ctypes
+from pathlib import Path
+
+LIB_PATH = Path(__file__).resolve().parent / 'libbraids_hip.so'
+
+_D = ctypes.c_double
+_I = ctypes.c_int
+_P = ctypes.c_void_p
+
+BRAIDS_OK = 0
+BRAIDS_REJECTED = 1
+KERNEL_KINDS = ('prep', 'traces', 'face', 'drain', 'gather', 'ledger')
+
+
+class BraidsMesh(ctypes.Structure):
+    _fields_ = [('nc', _I), ('nf', _I), ('nb', _I), ('nadj', _I),
+                ('area', _P), ('xy', _P), ('manning', _P), ('irregular', _P),
+                ('owner', _P), ('neighbour', _P), ('bid', _P),
+                ('normal', _P), ('length', _P), ('mid', _P),
+                ('offset', _P), ('faces', _P), ('signs', _P),
+                ('mode', _P), ('boundary_offset', _P), ('boundary_faces', _P),
+                ('forcing_q', _P), ('forcing_stage', _P),
+                ('series_offset', _P), ('series_time', _P), ('series_q', _P), ('series_stage', _P)]
+
+
+class BraidsOptions(ctypes.Structure):
+    _fields_ = [('precision', _I), ('scheme', _I), ('grav', _D), ('dry', _D),
+                ('cfl_target', _D), ('dtmax', _D), ('block', _I), ('remap', _I)]
+
+
+class BraidsStepResult(ctypes.Structure):
+    _fields_ = [('accepted', _I), ('failure', _I), ('failed_stage', _I), ('pad', _I),
+                ('trial_cfl_peak', _D), ('stage_peak', _D * 3)]
+
+
+# (name, restype, argtypes) of every exported entry point
+SIGNATURES = {
+    'braids_last_error': (ctypes.c_char_p, []),
+    'braids_create': (_I, [ctypes.POINTER(BraidsMesh), ctypes.POINTER(BraidsOptions), _P, ctypes.POINTER(_P)]),
+    'braids_destroy': (_I, [_P]),
+    'braids_set_stream': (_I, [_P, _P]),
+    'braids_set_state': (_I, [_P, _P, _I]),
+    'braids_get_state': (_I, [_P, _P]),
+    'braids_set_bed': (_I, [_P, _P]),
+    'braids_get_ledger': (_I, [_P, _P]),
+    'braids_reset_ledger': (_I, [_P]),
+    'braids_step': (_I, [_P, _D, _D, ctypes.POINTER(BraidsStepResult)]),
+    'braids_advance': (_I, [_P, _I, _D, _P, _P, _P, _P]),
+    'braids_native_euler': (_I, [_P, _P, _P, _D, _D, _P, _P, _P]),
+    'braids_native_rates': (_I, [_P, _P, _P, _D, _D, _P, _P, _P]),
+    'river_get_face_speed': (_I, [_P, _P]),
+    'braids_mass': (_I, [_P, _P]),
+    'braids_profile': (_I, [_P, _I]),
+    'braids_profile_read': (_I, [_P, _P, _P]),
+    'braids_stream_copy': (_I, [_P, _P, ctypes.c_longlong, _P, _I, _P]),
+    'braids_engine_set_stream': (_I, [_P]),
+    'explicit_frequency': (_I, [_I, _I, _D, _D, _P, _P, _P, _P, _P, _P, _P, _P, _P, _P, _P]),
+    'midpoint_inventory': (_I, [_I, _D, _P, _P, _P, _P, _P, _P, _P, _P]),
+    'midpoint_faces': (_I, [_I, _I, _D, _P, _P, _P, _P, _P, _P, _P, _P]),
+    'midpoint_gather': (_I, [_I, _I, _P, _P, _P, _P, _P, _P, _P, _P]),
+    'river_cell_drag': (_I, [_I, _D, _D, _D, _I, _D, _I, _P, _P, _P]),
+    'river_cell_drag_spatial': (_I, [_I, _D, _D, _D, _I, _P, _I, _P, _P, _P]),
+}
+
+_lib = None
+
+
+def load():
+    """Load libbraids_hip.so (raises if it has not been built)."""
+    global _lib
+    if _lib is None:
+        if not LIB_PATH.exists():
+            raise ImportError(f'{LIB_PATH} is missing: build it with braids_hip.build.build() '
+                              '(the HIP library is required; there is no CPU path)')
+        lib = ctypes.CDLL(str(LIB_PATH))
+        for name, (res, args) in SIGNATURES.items():
+            fn = getattr(lib, name)
+            fn.restype = res
+            fn.argtypes = args
+        _lib = lib
+    return _lib
+
+
+def last_error():
+    return load().braids_last_error().decode(errors='replace')
+
+
+def check(rc, what):
+    if rc < 0:
+        raise RuntimeError(f'{what} failed ({rc}): {last_error()}')
+    return rc

@@ -1,0 +1,71 @@
+"""Build libbraids_hip.so for gfx950 with hipcc (in-tree, no JIT cache).
+
+Every translation unit is compiled with packed-FP32 instructions disabled.
+The FP64 kernels and the host-side geometry code are compiled without FMA
+contraction so that FP64 mode rounds exactly where the reference solver
+rounds; the FP32 kernels allow contraction.
+"""
+import os
+import shutil
+import subprocess
+import sys
+from concurrent.futures import ThreadPoolExecutor
+from pathlib import Path
+
+PKG = Path(__file__).resolve().parent
+ROOT = PKG.parent                      # braids-hkt_amd/
+CSRC = ROOT / 'csrc'
+INCLUDE = ROOT.parent / 'include'
+BUILD = ROOT / '_build'
+LIB = PKG / 'libbraids_hip.so'
+ARCH = os.environ.get('BRAIDS_OFFLOAD_ARCH', 'gfx950')
+
+COMMON = ['-O3', '-std=c++17', '-fPIC', f'--offload-arch={ARCH}',
+          '-Xclang', '-target-feature', '-Xclang', '-packed-fp32-ops',
+          f'-I{INCLUDE}', f'-I{CSRC}']
+UNITS = [
+    ('swe_f64.hip', ['-ffp-contract=off']),
+    ('swe_f32.hip', []),
+    ('engine.hip', ['-ffp-contract=off']),
+    ('braids_api.hip', ['-ffp-contract=off']),
+]
+
+
+def hipcc():
+    exe = shutil.which('hipcc') or '/opt/rocm/bin/hipcc'
+    if not Path(exe).exists():
+        raise RuntimeError('hipcc not found; ROCm is required to build libbraids_hip')
+    return exe
+
+
+def _compile(args):
+    src, extra = args
+    obj = BUILD / (Path(src).stem + '.o')
+    cmd = [hipcc(), *COMMON, *extra, '-c', str(CSRC / src), '-o', str(obj)]
+    p = subprocess.run(cmd, capture_output=True, text=True)
+    noise = 'packed-fp32-ops'
+    err = '\n'.join(l for l in (p.stderr or '').splitlines() if noise not in l)
+    if p.returncode != 0:
+        raise RuntimeError(f'hipcc failed for {src}:\n{err}')
+    return obj
+
+
+def build(verbose=False):
+    BUILD.mkdir(exist_ok=True)
+    units = [u for u in UNITS if (CSRC / u[0]).exists()]
+    with ThreadPoolExecutor(max_workers=min(4, len(units))) as pool:
+        objs = list(pool.map(_compile, units))
+    tmp = LIB.with_suffix('.so.tmp')
+    cmd = [hipcc(), '-shared', '-fPIC', f'--offload-arch={ARCH}', *map(str, objs), '-o', str(tmp)]
+    p = subprocess.run(cmd, capture_output=True, text=True)
+    if p.returncode != 0:
+        raise RuntimeError(f'link failed:\n{p.stderr}')
+    os.replace(tmp, LIB)
+    if verbose:
+        print(f'built {LIB}')
+    return LIB
+
+
+if __name__ == '__main__':
+    build(verbose=True)
+    sys.exit(0)

@@ -313,22 +313,24 @@ def plain_channel(nx=500, ny=25, length=2000.0, width=100.0, slope=1e-3, sigma=0
 
 
 def braided_reach(nx=1768, ny=141, length=5000.0, width=400.0, slope=1e-3, amplitude=60.0,
-                  wavelength=800.0, nbars=200, q=1500.0, seed=2, depth=3.0) -> Mesh:
+                  wavelength=800.0, nbars=200, q=1500.0, seed=2, depth=3.0,
+                  bar_height=(0.5, 2.0), bar_sigma=(15.0, 60.0), bank=4.0) -> Mesh:
     """Braided synthetic reach 5 km x 400 m: sinusoidal thalweg (amplitude
     60 m, wavelength 800 m), parabolic cross-section, 200 seeded Gaussian bars
-    (height 0.5-2 m, sigma 15-60 m) that create wet/dry fronts, four Manning
+    (height 0.5-2 m, sigma 15-60 m, centred away from the inlet and outlet
+    sections) that create wet/dry fronts, four Manning
     zones n in {0.025, 0.030, 0.035, 0.040}, inflow Q=1500 m^3/s.  The default
     2.83 m quads give about 500k triangles."""
     dx = length / nx; dy = width / ny
     rng = np.random.default_rng(seed)
-    bx = rng.uniform(0, length, nbars); by = rng.uniform(0, width, nbars)
-    bh = rng.uniform(0.5, 2.0, nbars); bs = rng.uniform(15.0, 60.0, nbars)
+    bx = rng.uniform(0.1 * length, 0.9 * length, nbars); by = rng.uniform(0, width, nbars)
+    bh = rng.uniform(*bar_height, nbars); bs = rng.uniform(*bar_sigma, nbars)
 
     def thalweg(x):
         return 0.5 * width + amplitude * np.sin(2 * np.pi * x / wavelength)
 
     def bed(x, y):
-        z = -slope * x + 4.0 * ((y - thalweg(x)) / (0.5 * width)) ** 2
+        z = -slope * x + bank * ((y - thalweg(x)) / (0.5 * width)) ** 2
         for k in range(nbars):
             z = z + bh[k] * np.exp(-((x - bx[k]) ** 2 + (y - by[k]) ** 2) / (2 * bs[k] ** 2))
         return z

@@ -1,0 +1,175 @@
+"""Python host side of the hydraulic step: a resident solver handle bound to
+one mesh, mirroring the reference's native hydraulic owner.
+
+The reference CPU path loads a deck into a Fortran library and drives it
+through ctypes (bend_native_hydraulics.py:84-138, shoreline_solver.py:86-105);
+its time loop is advance_ssp33 plus the accepted-step controller
+(cutcell_solver.f90:15858-15986, 17287-17370).  ``HydraulicSolver`` keeps the
+same contract on the GPU: state in, accepted state out, a rejected trial
+raises nothing but reports the failure so the caller retries with a smaller
+dt (the reference raises BendTrialRejected, bend_methods_v1.py:34)."""
+import ctypes
+
+import numpy as np
+
+from . import _lib
+from .mesh import Mesh
+
+
+class TrialRejected(RuntimeError):
+    """A trial step the caller must retry with a smaller dt."""
+
+
+def _p(a):
+    return ctypes.c_void_p(a.ctypes.data) if a is not None else None
+
+
+class HydraulicSolver:
+    """Resident SSPRK(3,3) augmented f-wave solver on the current HIP device.
+
+    precision: 64 (FP64 throughout) or 32 (FP32 flux/update, FP64 depth,
+    bed and free-surface differencing, ledgers)."""
+
+    def __init__(self, mesh: Mesh, precision=64, scheme=5, grav=9.81, dry=1e-8,
+                 cfl_target=0.9, dtmax=1.0, block=256, remap=1, z=None, stream=None):
+        self.L = _lib.load()
+        self.mesh = mesh
+        self.nc, self.nf, self.nb = mesh.nc, mesh.nf, mesh.nb
+        keep = {}
+
+        def a(name, x, dt):
+            x = np.ascontiguousarray(x, dtype=dt)
+            keep[name] = x
+            return x.ctypes.data
+
+        m = _lib.BraidsMesh()
+        m.nc, m.nf, m.nb, m.nadj = mesh.nc, mesh.nf, mesh.nb, mesh.nadj
+        m.area = a('area', mesh.area, np.float64)
+        m.xy = a('xy', mesh.xy, np.float64)
+        m.manning = a('manning', mesh.manning, np.float64)
+        m.irregular = a('irregular', mesh.irregular, np.int32)
+        m.owner = a('owner', mesh.owner, np.int32)
+        m.neighbour = a('neighbour', mesh.neighbour, np.int32)
+        m.bid = a('bid', mesh.bid, np.int32)
+        m.normal = a('normal', mesh.normal, np.float64)
+        m.length = a('length', mesh.length, np.float64)
+        m.mid = a('mid', mesh.mid, np.float64)
+        m.offset = a('offset', mesh.offset, np.int32)
+        m.faces = a('faces', mesh.faces_csr, np.int32)
+        m.signs = a('signs', mesh.signs, np.int32)
+        m.mode = a('mode', np.append(mesh.mode, 0), np.int32)
+        m.boundary_offset = a('boundary_offset', mesh.boundary_offset, np.int32)
+        m.boundary_faces = a('boundary_faces', np.append(mesh.boundary_faces, 0), np.int32)
+        m.forcing_q = a('forcing_q', np.append(mesh.forcing_q, 0.0), np.float64)
+        m.forcing_stage = a('forcing_stage', np.append(mesh.forcing_stage, 0.0), np.float64)
+        if mesh.series_offset is not None:
+            m.series_offset = a('series_offset', mesh.series_offset, np.int32)
+            m.series_time = a('series_time', np.append(mesh.series_time, 0.0), np.float64)
+            m.series_q = a('series_q', np.append(mesh.series_q, 0.0), np.float64)
+            m.series_stage = a('series_stage', np.append(mesh.series_stage, 0.0), np.float64)
+        o = _lib.BraidsOptions(int(precision), int(scheme), float(grav), float(dry),
+                               float(cfl_target), float(dtmax), int(block), int(remap))
+        zz = np.ascontiguousarray(mesh.z if z is None else z, dtype=np.float64)
+        handle = ctypes.c_void_p()
+        _lib.check(self.L.braids_create(ctypes.byref(m), ctypes.byref(o), _p(zz), ctypes.byref(handle)),
+                   'braids_create')
+        self.h = handle
+        self.precision, self.scheme, self.dtmax = int(precision), int(scheme), float(dtmax)
+        self.time = 0.0
+        self.step_hint = float(dtmax)
+        if stream is not None:
+            self.set_stream(stream)
+
+    def close(self):
+        if getattr(self, 'h', None):
+            self.L.braids_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def set_stream(self, stream):
+        """stream: a raw hipStream_t (int) or any object with a ``cuda_stream`` attribute."""
+        raw = getattr(stream, 'cuda_stream', stream)
+        _lib.check(self.L.braids_set_stream(self.h, ctypes.c_void_p(int(raw) if raw else 0)), 'braids_set_stream')
+
+    # -- state -------------------------------------------------------------
+    def set_state(self, q, prepare=True):
+        q = np.ascontiguousarray(q, dtype=np.float64).reshape(3, self.nc)
+        _lib.check(self.L.braids_set_state(self.h, _p(q), int(bool(prepare))), 'braids_set_state')
+
+    def state(self):
+        q = np.empty((3, self.nc))
+        _lib.check(self.L.braids_get_state(self.h, _p(q)), 'braids_get_state')
+        return q
+
+    def set_bed(self, z):
+        z = np.ascontiguousarray(z, dtype=np.float64)
+        _lib.check(self.L.braids_set_bed(self.h, _p(z)), 'braids_set_bed')
+
+    def ledger(self):
+        out = np.zeros(max(self.nb, 1))
+        _lib.check(self.L.braids_get_ledger(self.h, _p(out)), 'braids_get_ledger')
+        return out[:self.nb]
+
+    def mass(self):
+        v = ctypes.c_double()
+        _lib.check(self.L.braids_mass(self.h, ctypes.byref(v)), 'braids_mass')
+        return v.value
+
+    # -- time stepping -----------------------------------------------------
+    def step(self, dt, t=None):
+        """One SSPRK(3,3) trial; commits and returns the result, or raises
+        TrialRejected (state unchanged)."""
+        r = _lib.BraidsStepResult()
+        rc = _lib.check(self.L.braids_step(self.h, float(dt), float(self.time if t is None else t),
+                                           ctypes.byref(r)), 'braids_step')
+        info = dict(accepted=bool(r.accepted), failure=r.failure, failed_stage=r.failed_stage,
+                    cfl=r.trial_cfl_peak, stage_peak=list(r.stage_peak))
+        if rc == _lib.BRAIDS_REJECTED:
+            raise TrialRejected(f'stage {r.failed_stage} rejected (reason {r.failure}, CFL {r.trial_cfl_peak:g})')
+        return info
+
+    def advance(self, nsteps, tend=1e300):
+        """Accepted-step loop with the reference retry/step-hint rules."""
+        t = ctypes.c_double(self.time); hint = ctypes.c_double(self.step_hint)
+        acc = ctypes.c_int(); ret = ctypes.c_int()
+        rc = _lib.check(self.L.braids_advance(self.h, int(nsteps), float(tend), ctypes.byref(t),
+                                              ctypes.byref(hint), ctypes.byref(acc), ctypes.byref(ret)),
+                        'braids_advance')
+        self.time, self.step_hint = t.value, hint.value
+        if rc == _lib.BRAIDS_REJECTED:
+            raise TrialRejected(_lib.last_error())
+        return acc.value, ret.value
+
+    # -- measurement --------------------------------------------------------
+    def profile(self, enable=True):
+        _lib.check(self.L.braids_profile(self.h, int(bool(enable))), 'braids_profile')
+
+    def profile_read(self):
+        ms = (ctypes.c_double * 6)(); n = (ctypes.c_longlong * 6)()
+        _lib.check(self.L.braids_profile_read(self.h, ctypes.byref(ms), ctypes.byref(n)), 'braids_profile_read')
+        return {k: (ms[i], n[i]) for i, k in enumerate(_lib.KERNEL_KINDS)}
+
+    # -- reference bridge replacements (device pointers) --------------------
+    def native_euler(self, q_ptr, z_ptr, dt, t, qout_ptr, water_ptr):
+        code = ctypes.c_int()
+        _lib.check(self.L.braids_native_euler(self.h, ctypes.c_void_p(q_ptr), ctypes.c_void_p(z_ptr or 0),
+                                              float(dt), float(t), ctypes.c_void_p(qout_ptr),
+                                              ctypes.c_void_p(water_ptr or 0), ctypes.byref(code)),
+                   'braids_native_euler')
+        return code.value
+
+    def native_rates(self, q_ptr, z_ptr, dt, t, rate_ptr, water_ptr):
+        code = ctypes.c_int()
+        _lib.check(self.L.braids_native_rates(self.h, ctypes.c_void_p(q_ptr), ctypes.c_void_p(z_ptr or 0),
+                                              float(dt), float(t), ctypes.c_void_p(rate_ptr),
+                                              ctypes.c_void_p(water_ptr), ctypes.byref(code)),
+                   'braids_native_rates')
+        return code.value
+
+    def face_speed(self, speed_ptr):
+        _lib.check(self.L.river_get_face_speed(self.h, ctypes.c_void_p(speed_ptr)), 'river_get_face_speed')

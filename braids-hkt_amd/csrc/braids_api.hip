@@ -1,0 +1,627 @@
+// C ABI of libbraids_hip (declared in include/braids_hip.h): solver handles,
+// state transfer, the SSPRK(3,3) step and accepted-step loop, and the
+// replacements of the reference's Fortran ctypes bridge.
+#include <hip/hip_runtime.h>
+#include <cfloat>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <algorithm>
+#include <string>
+#include <vector>
+
+#include "braids_hip.h"
+#include "swe_launch.h"
+
+using namespace braids;
+
+static thread_local std::string g_error;
+static void set_error(const std::string &e) { g_error = e; }
+extern "C" const char *braids_last_error(void) { return g_error.c_str(); }
+
+#define HIP_TRY(expr)                                                                       \
+    do {                                                                                    \
+        hipError_t e_ = (expr);                                                             \
+        if (e_ != hipSuccess) {                                                             \
+            set_error(std::string(#expr) + ": " + hipGetErrorString(e_));                   \
+            return BRAIDS_EHIP;                                                             \
+        }                                                                                   \
+    } while (0)
+
+namespace {
+
+template <class R>
+__global__ void k_load_state(const double *q, StateView<R> s, int nc)
+{
+    int c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= nc) return;
+    s.h[c] = q[c]; s.hu[c] = (R)q[nc + c]; s.hv[c] = (R)q[2 * nc + c];
+}
+template <class R>
+__global__ void k_store_state(StateView<R> s, double *q, int nc)
+{
+    int c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= nc) return;
+    q[c] = s.h[c]; q[nc + c] = (double)s.hu[c]; q[2 * nc + c] = (double)s.hv[c];
+}
+// prepare_hydraulic_stage (cutcell_solver.f90:8513-8534) in place
+template <class R>
+__global__ void k_prepare(StateView<R> s, int nc, double dry)
+{
+    int c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= nc) return;
+    double h = s.h[c];
+    if (h <= dry) { s.hu[c] = R(0); s.hv[c] = R(0); return; }
+    double hu = s.hu[c], hv = s.hv[c];
+    double sp = sqrt(hu * hu + hv * hv) / h;
+    if (sp > 50.0) { s.hu[c] = (R)(hu * (50.0 / sp)); s.hv[c] = (R)(hv * (50.0 / sp)); }
+}
+__global__ void k_zero_bytes(unsigned int *p, int nwords)
+{
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < nwords) p[i] = 0u;
+}
+// fl(1,:) after the donor limiter (instantaneous_face_flux of euler)
+template <class R>
+__global__ void k_water(MeshView<R> m, Work<R> w, Control *ctl, int slot, double *water)
+{
+    int f = blockIdx.x * blockDim.x + threadIdx.x;
+    if (f >= m.nf) return;
+    R F1 = w.F[f];
+    if (ctl->drain_any[slot] && F1 != R(0)) {
+        int donor = F1 < R(0) ? m.neighbour[f] : m.owner[f];
+        if (donor >= 0 && w.theta[donor] != R(1)) F1 = F1 * w.theta[donor];
+    }
+    water[f] = (double)F1;
+}
+template <class R>
+__global__ void k_speed(MeshView<R> m, Work<R> w, double *speed)
+{
+    int f = blockIdx.x * blockDim.x + threadIdx.x;
+    if (f >= m.nf) return;
+    speed[f] = (double)w.F[5 * m.nf + f];
+}
+
+inline int blocks(int n, int b) { return n > 0 ? (n + b - 1) / b : 1; }
+
+struct SolverBase {
+    virtual ~SolverBase() {}
+    virtual int set_state(const double *q, int prepare) = 0;
+    virtual int get_state(double *q) = 0;
+    virtual int set_bed(const double *z, bool device) = 0;
+    virtual int step(double dt, double t, braids_step_result *res) = 0;
+    virtual int native_euler(const double *q, const double *z, double dt, double t, double *qout, double *water, int *code) = 0;
+    virtual int native_rates(const double *q, const double *z, double dt, double t, double *rate, double *water, int *code) = 0;
+    virtual int face_speed(double *speed) = 0;
+    hipStream_t stream = nullptr;
+    int nc = 0, nf = 0, nb = 0;
+    double cfl_target = 0.9, dtmax = 1.0;
+    std::vector<double> ledger;        // accepted inward water per boundary
+    std::vector<double> area;          // host copy for the mass inventory
+    std::vector<int> series_offset;
+    std::vector<double> series_time;
+    std::vector<double> hstate;        // scratch for the mass inventory
+    // optional HIP-event kernel timing (slot x kind x {start, stop})
+    std::vector<hipEvent_t> events;
+    double prof_ms[KK_COUNT] = {0};
+    long long prof_n[KK_COUNT] = {0};
+    virtual void set_events(hipEvent_t *ev) = 0;
+    int enable_profile(int on)
+    {
+        for (int k = 0; k < KK_COUNT; ++k) { prof_ms[k] = 0; prof_n[k] = 0; }
+        if (on && events.empty()) {
+            events.resize(4 * KK_COUNT * 2);
+            for (auto &e : events)
+                if (hipEventCreate(&e) != hipSuccess) { set_error("hipEventCreate failed"); return BRAIDS_EHIP; }
+        }
+        set_events(on ? events.data() : nullptr);
+        return BRAIDS_OK;
+    }
+    void collect(int slots)
+    {
+        if (events.empty()) return;
+        for (int slot = 0; slot < slots; ++slot)
+            for (int k = 0; k < KK_COUNT; ++k) {
+                if ((k == KK_TRACES || k == KK_LEDGER) && nb == 0) continue;
+                float ms = 0;
+                if (hipEventElapsedTime(&ms, events[(slot * KK_COUNT + k) * 2], events[(slot * KK_COUNT + k) * 2 + 1]) == hipSuccess) {
+                    prof_ms[k] += ms; prof_n[k] += 1;
+                }
+            }
+    }
+    bool profiling = false;
+};
+
+template <class R>
+struct Solver : SolverBase {
+    MeshView<R> m{};
+    std::vector<void *> allocs;
+    StateView<R> st[5];                // q, a, b, c, scratch for the native bridge
+    double *z = nullptr;
+    Work<R> w{};
+    Control *ctl = nullptr;            // device block: Control + 4*nb ledger doubles
+    double *ledger_dev = nullptr;
+    void *host_block = nullptr;        // pinned readback
+    size_t block_bytes = 0;
+    LaunchCfg cfg;
+    double *qscratch = nullptr;        // [3*nc] double for conversions
+    void set_events(hipEvent_t *ev) override { cfg.ev = ev; profiling = ev != nullptr; }
+
+    ~Solver() override
+    {
+        if (host_block) (void)hipHostFree(host_block);
+        for (void *p : allocs) (void)hipFree(p);
+        for (auto e : events) (void)hipEventDestroy(e);
+    }
+
+    template <class T> int alloc(T *&p, size_t n)
+    {
+        void *v = nullptr;
+        if (n == 0) n = 1;
+        hipError_t e = hipMalloc(&v, n * sizeof(T));
+        if (e != hipSuccess) { set_error(std::string("hipMalloc: ") + hipGetErrorString(e)); return BRAIDS_ENOMEM; }
+        allocs.push_back(v);
+        p = (T *)v;
+        return BRAIDS_OK;
+    }
+    template <class T, class S> int upload(const T *&dst, const std::vector<S> &src)
+    {
+        T *d = nullptr;
+        int rc = alloc(d, src.size());
+        if (rc) return rc;
+        std::vector<T> tmp(src.begin(), src.end());
+        if (!tmp.empty()) HIP_TRY(hipMemcpy(d, tmp.data(), tmp.size() * sizeof(T), hipMemcpyHostToDevice));
+        dst = d;
+        return BRAIDS_OK;
+    }
+
+    int init(const braids_mesh *mesh, const braids_options *opt, const double *zh)
+    {
+        nc = mesh->nc; nf = mesh->nf; nb = mesh->nb;
+        const int nadj = mesh->nadj;
+        cfl_target = opt->cfl_target; dtmax = opt->dtmax;
+        cfg.block = opt->block > 0 ? opt->block : 256;
+        cfg.remap = opt->remap;
+        m.nc = nc; m.nf = nf; m.nb = nb; m.nadj = nadj; m.scheme = opt->scheme;
+        m.grav = (R)opt->grav; m.dry = (R)opt->dry; m.cfl_target = (R)opt->cfl_target;
+        // derived geometry in FP64 (cache_sediment_geometry / cache_hydraulic_geometry,
+        // cutcell_solver.f90:8639-8691), rounded to R afterwards
+        std::vector<double> ls(4 * (size_t)nc, 0.0), wr(2 * (size_t)nadj, 0.0), dd(2 * (size_t)nadj, 0.0);
+        std::vector<int> other(nadj, -1), flags(nadj, 0);
+        for (int c = 0; c < nc; ++c) {
+            double a11 = 0, a12 = 0, a22 = 0;
+            for (int k = mesh->offset[c]; k < mesh->offset[c + 1]; ++k) {
+                const int f = mesh->faces[k];
+                const bool own = mesh->signs[k] > 0;
+                const int n = own ? mesh->neighbour[f] : mesh->owner[f];
+                other[k] = n;
+                int fl = own ? ADJ_OWNER : 0;
+                if (mesh->bid[f] >= 0 && mesh->neighbour[f] < 0) {
+                    fl |= ADJ_OPEN;
+                    if (mesh->mode[mesh->bid[f]] == 2) fl |= ADJ_OUTLET2;
+                }
+                flags[k] = fl;
+                dd[k] = mesh->mid[f] - mesh->xy[c];
+                dd[nadj + k] = mesh->mid[nf + f] - mesh->xy[nc + c];
+                if (n < 0) continue;
+                const double r0 = mesh->xy[n] - mesh->xy[c], r1 = mesh->xy[nc + n] - mesh->xy[nc + c];
+                const double wgt = mesh->length[f] / (r0 * r0 + r1 * r1);
+                wr[k] = wgt * r0; wr[nadj + k] = wgt * r1;
+                a11 = a11 + wgt * (r0 * r0); a12 = a12 + wgt * r0 * r1; a22 = a22 + wgt * (r1 * r1);
+            }
+            ls[c] = a11; ls[nc + c] = a12; ls[2 * (size_t)nc + c] = a22; ls[3 * (size_t)nc + c] = a11 * a22 - a12 * a12;
+        }
+        std::vector<double> dl(2 * (size_t)nf), dr(2 * (size_t)nf, 0.0);
+        for (int f = 0; f < nf; ++f) {
+            const int o = mesh->owner[f], n = mesh->neighbour[f];
+            dl[f] = mesh->mid[f] - mesh->xy[o]; dl[nf + f] = mesh->mid[nf + f] - mesh->xy[nc + o];
+            if (n >= 0) { dr[f] = mesh->mid[f] - mesh->xy[n]; dr[nf + f] = mesh->mid[nf + f] - mesh->xy[nc + n]; }
+        }
+        auto vec = [](const double *p, size_t n) { return std::vector<double>(p, p + n); };
+        auto ivec = [](const int *p, size_t n) { return std::vector<int>(p, p + n); };
+        int rc = 0;
+        if ((rc = upload(m.area, vec(mesh->area, nc)))) return rc;
+        if ((rc = upload(m.manning, vec(mesh->manning, nc)))) return rc;
+        if ((rc = upload(m.ls, ls))) return rc;
+        if ((rc = upload(m.offset, ivec(mesh->offset, nc + 1)))) return rc;
+        if ((rc = upload(m.irregular, ivec(mesh->irregular, nc)))) return rc;
+        if ((rc = upload(m.adj_face, ivec(mesh->faces, nadj)))) return rc;
+        if ((rc = upload(m.adj_other, other))) return rc;
+        if ((rc = upload(m.adj_flags, flags))) return rc;
+        if ((rc = upload(m.adj_wr, wr))) return rc;
+        if ((rc = upload(m.adj_d, dd))) return rc;
+        if ((rc = upload(m.owner, ivec(mesh->owner, nf)))) return rc;
+        if ((rc = upload(m.neighbour, ivec(mesh->neighbour, nf)))) return rc;
+        if ((rc = upload(m.bid, ivec(mesh->bid, nf)))) return rc;
+        if ((rc = upload(m.normal, vec(mesh->normal, 2 * (size_t)nf)))) return rc;
+        if ((rc = upload(m.length, vec(mesh->length, nf)))) return rc;
+        if ((rc = upload(m.dl, dl))) return rc;
+        if ((rc = upload(m.dr, dr))) return rc;
+        if ((rc = upload(m.mode, ivec(mesh->mode, nb)))) return rc;
+        if ((rc = upload(m.bnd_offset, ivec(mesh->boundary_offset, nb + 1)))) return rc;
+        const int nbf = mesh->boundary_offset[nb];
+        if ((rc = upload(m.bnd_faces, ivec(mesh->boundary_faces, nbf)))) return rc;
+        if ((rc = upload(m.forcing_q, vec(mesh->forcing_q, nb)))) return rc;
+        if ((rc = upload(m.forcing_stage, vec(mesh->forcing_stage, nb)))) return rc;
+        m.series_offset = nullptr; m.series_time = m.series_q = m.series_stage = nullptr;
+        if (mesh->series_offset) {
+            series_offset.assign(mesh->series_offset, mesh->series_offset + nb + 1);
+            const int ns = series_offset[nb];
+            series_time.assign(mesh->series_time, mesh->series_time + ns);
+            if ((rc = upload(m.series_offset, series_offset))) return rc;
+            if ((rc = upload(m.series_time, series_time))) return rc;
+            if ((rc = upload(m.series_q, vec(mesh->series_q, ns)))) return rc;
+            if ((rc = upload(m.series_stage, vec(mesh->series_stage, ns)))) return rc;
+        }
+        area.assign(mesh->area, mesh->area + nc);
+        for (auto &s : st) {
+            if ((rc = alloc(s.h, nc))) return rc;
+            if ((rc = alloc(s.hu, nc))) return rc;
+            if ((rc = alloc(s.hv, nc))) return rc;
+        }
+        if ((rc = alloc(z, nc))) return rc;
+        HIP_TRY(hipMemcpy(z, zh, nc * sizeof(double), hipMemcpyHostToDevice));
+        if ((rc = alloc(w.grad, 8 * (size_t)nc))) return rc;
+        if ((rc = alloc(w.flags, nc))) return rc;
+        if ((rc = alloc(w.trace, 3 * (size_t)nf))) return rc;
+        if ((rc = alloc(w.F, 6 * (size_t)nf))) return rc;
+        if ((rc = alloc(w.theta, nc))) return rc;
+        if ((rc = alloc(w.hydraulic, 2 * (size_t)nb + 2))) return rc;
+        if ((rc = alloc(w.target_u, nbf + 1))) return rc;
+        if ((rc = alloc(qscratch, 3 * (size_t)nc))) return rc;
+        HIP_TRY(hipMemset(w.grad, 0, 8 * (size_t)nc * sizeof(R)));
+        HIP_TRY(hipMemset(w.trace, 0, 3 * (size_t)nf * sizeof(R)));
+        block_bytes = sizeof(Control) + 4 * (size_t)(nb + 1) * sizeof(double);
+        block_bytes = (block_bytes + 255) & ~(size_t)255;
+        char *blk = nullptr;
+        if ((rc = alloc(blk, block_bytes))) return rc;
+        ctl = (Control *)blk;
+        ledger_dev = (double *)(blk + sizeof(Control));
+        HIP_TRY(hipHostMalloc(&host_block, block_bytes, hipHostMallocDefault));
+        ledger.assign(nb, 0.0);
+        hstate.resize(nc);
+        return BRAIDS_OK;
+    }
+
+    void zero_control()
+    {
+        int words = (int)(block_bytes / 4);
+        hipLaunchKernelGGL(k_zero_bytes, dim3(blocks(words, 256)), dim3(256), 0, stream, (unsigned int *)ctl, words);
+    }
+
+    int set_state(const double *q, int prepare) override
+    {
+        HIP_TRY(hipMemcpyAsync(qscratch, q, 3 * (size_t)nc * sizeof(double), hipMemcpyHostToDevice, stream));
+        hipLaunchKernelGGL((k_load_state<R>), dim3(blocks(nc, 256)), dim3(256), 0, stream, qscratch, st[0], nc);
+        if (prepare) hipLaunchKernelGGL((k_prepare<R>), dim3(blocks(nc, 256)), dim3(256), 0, stream, st[0], nc, (double)m.dry);
+        HIP_TRY(hipStreamSynchronize(stream));
+        return BRAIDS_OK;
+    }
+    int get_state(double *q) override
+    {
+        hipLaunchKernelGGL((k_store_state<R>), dim3(blocks(nc, 256)), dim3(256), 0, stream, st[0], qscratch, nc);
+        HIP_TRY(hipMemcpyAsync(q, qscratch, 3 * (size_t)nc * sizeof(double), hipMemcpyDeviceToHost, stream));
+        HIP_TRY(hipStreamSynchronize(stream));
+        return BRAIDS_OK;
+    }
+    int set_bed(const double *zz, bool device) override
+    {
+        HIP_TRY(hipMemcpyAsync(z, zz, nc * sizeof(double), device ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice, stream));
+        HIP_TRY(hipStreamSynchronize(stream));
+        return BRAIDS_OK;
+    }
+
+    int step(double dt, double t, braids_step_result *res) override
+    {
+        zero_control();
+        run_stage<R>(m, st[0], st[0], st[1], z, w, ctl, ledger_dev, 0, dt, t, cfg, stream);
+        run_stage<R>(m, st[1], st[0], st[2], z, w, ctl, ledger_dev + (nb + 1), 1, dt, t + dt, cfg, stream);
+        run_stage<R>(m, st[2], st[0], st[3], z, w, ctl, ledger_dev + 2 * (nb + 1), 2, dt, t + .5 * dt, cfg, stream);
+        HIP_TRY(hipGetLastError());
+        HIP_TRY(hipMemcpyAsync(host_block, ctl, block_bytes, hipMemcpyDeviceToHost, stream));
+        HIP_TRY(hipStreamSynchronize(stream));
+        if (profiling) collect(3);
+        const Control *hc = (const Control *)host_block;
+        const double *hl = (const double *)((const char *)host_block + sizeof(Control));
+        res->accepted = 0; res->failure = 0; res->failed_stage = -1; res->trial_cfl_peak = 0;
+        double peak = 0;
+        for (int k = 0; k < 3; ++k) {
+            double pk; unsigned long long b = hc->peak_bits[k];
+            memcpy(&pk, &b, 8);
+            int code = hc->failure[k];
+            // a stage that failed before its gather reports no CFL peak
+            if (code >= FAIL_DRAIN) pk = 0.0;
+            res->stage_peak[k] = pk;
+            peak = pk > peak ? pk : peak;
+            if (code) {
+                res->failure = code >= FAIL_TRACE ? code - FAIL_TRACE : (code == FAIL_DRAIN ? 2 : code);
+                res->failed_stage = k;
+                break;
+            }
+        }
+        res->trial_cfl_peak = peak;
+        if (res->failure) return BRAIDS_REJECTED;
+        std::swap(st[0], st[3]);
+        for (int b = 0; b < nb; ++b)
+            ledger[b] = ledger[b] + (hl[b] + hl[(nb + 1) + b]) / 6.0 + 2.0 * hl[2 * (nb + 1) + b] / 3.0;
+        res->accepted = 1;
+        return BRAIDS_OK;
+    }
+
+    int native_euler(const double *q, const double *zz, double dt, double t, double *qout, double *water, int *code) override
+    {
+        if (zz) HIP_TRY(hipMemcpyAsync(z, zz, nc * sizeof(double), hipMemcpyDeviceToDevice, stream));
+        zero_control();
+        hipLaunchKernelGGL((k_load_state<R>), dim3(blocks(nc, 256)), dim3(256), 0, stream, q, st[4], nc);
+        run_stage<R>(m, st[4], st[4], st[3], z, w, ctl, nullptr, 3, dt, t, cfg, stream);
+        hipLaunchKernelGGL((k_store_state<R>), dim3(blocks(nc, 256)), dim3(256), 0, stream, st[3], qout, nc);
+        if (water) hipLaunchKernelGGL((k_water<R>), dim3(blocks(nf, 256)), dim3(256), 0, stream, m, w, ctl, 3, water);
+        HIP_TRY(hipGetLastError());
+        HIP_TRY(hipMemcpyAsync(host_block, ctl, sizeof(Control), hipMemcpyDeviceToHost, stream));
+        HIP_TRY(hipStreamSynchronize(stream));
+        *code = ((const Control *)host_block)->failure[3] ? 1 : 0;
+        return BRAIDS_OK;
+    }
+    int native_rates(const double *q, const double *zz, double dt, double t, double *rate, double *water, int *code) override
+    {
+        (void)dt;
+        if (zz) HIP_TRY(hipMemcpyAsync(z, zz, nc * sizeof(double), hipMemcpyDeviceToDevice, stream));
+        zero_control();
+        hipLaunchKernelGGL((k_load_state<R>), dim3(blocks(nc, 256)), dim3(256), 0, stream, q, st[4], nc);
+        run_rates<R>(m, st[4], z, w, ctl, rate, water, t, cfg, stream);
+        HIP_TRY(hipGetLastError());
+        HIP_TRY(hipMemcpyAsync(host_block, ctl, sizeof(Control), hipMemcpyDeviceToHost, stream));
+        HIP_TRY(hipStreamSynchronize(stream));
+        *code = ((const Control *)host_block)->failure[3] ? 1 : 0;
+        return BRAIDS_OK;
+    }
+    int face_speed(double *speed) override
+    {
+        hipLaunchKernelGGL((k_speed<R>), dim3(blocks(nf, 256)), dim3(256), 0, stream, m, w, speed);
+        HIP_TRY(hipGetLastError());
+        HIP_TRY(hipStreamSynchronize(stream));
+        return BRAIDS_OK;
+    }
+};
+
+}  // namespace
+
+struct braids_solver {
+    SolverBase *impl;
+};
+
+static int check_mesh(const braids_mesh *m)
+{
+    if (!m || m->nc <= 0 || m->nf <= 0 || m->nb < 0 || m->nadj <= 0) { set_error("invalid mesh sizes"); return BRAIDS_EINVAL; }
+    if (!m->area || !m->xy || !m->manning || !m->irregular || !m->owner || !m->neighbour || !m->bid || !m->normal ||
+        !m->length || !m->mid || !m->offset || !m->faces || !m->signs || !m->boundary_offset) {
+        set_error("missing mesh array"); return BRAIDS_EINVAL;
+    }
+    if (m->nb > 0 && (!m->mode || !m->boundary_faces || !m->forcing_q || !m->forcing_stage)) { set_error("missing boundary array"); return BRAIDS_EINVAL; }
+    if (m->offset[0] != 0 || m->offset[m->nc] != m->nadj) { set_error("CSR offsets do not span nadj"); return BRAIDS_EINVAL; }
+    for (int f = 0; f < m->nf; ++f) {
+        int o = m->owner[f], n = m->neighbour[f], b = m->bid[f];
+        if (o < 0 || o >= m->nc || n < -1 || n >= m->nc || o == n || b < -1 || b >= m->nb || (n >= 0 && b >= 0) || !(m->length[f] > 0)) {
+            set_error("invalid face " + std::to_string(f)); return BRAIDS_EINVAL;
+        }
+    }
+    for (int c = 0; c < m->nc; ++c) {
+        if (m->offset[c + 1] < m->offset[c] || !(m->area[c] > 0)) { set_error("invalid cell " + std::to_string(c)); return BRAIDS_EINVAL; }
+        for (int k = m->offset[c]; k < m->offset[c + 1]; ++k) {
+            int f = m->faces[k];
+            if (f < 0 || f >= m->nf) { set_error("CSR face index out of range"); return BRAIDS_EINVAL; }
+            if ((m->signs[k] > 0 && m->owner[f] != c) || (m->signs[k] < 0 && m->neighbour[f] != c)) {
+                set_error("CSR incidence inconsistent with owner/neighbour"); return BRAIDS_EINVAL;
+            }
+        }
+    }
+    for (int b = 0; b < m->nb; ++b) {
+        if (m->mode[b] != 2 && m->mode[b] != 6) { set_error("unsupported boundary mode (2 and 6 are implemented)"); return BRAIDS_EINVAL; }
+        for (int k = m->boundary_offset[b]; k < m->boundary_offset[b + 1]; ++k) {
+            int f = m->boundary_faces[k];
+            if (f < 0 || f >= m->nf || m->bid[f] != b) { set_error("boundary face list inconsistent"); return BRAIDS_EINVAL; }
+        }
+    }
+    return BRAIDS_OK;
+}
+
+extern "C" int braids_create(const braids_mesh *mesh, const braids_options *opt, const double *z, braids_solver **out)
+{
+    if (!out || !opt || !z) { set_error("null argument"); return BRAIDS_EINVAL; }
+    *out = nullptr;
+    int rc = check_mesh(mesh);
+    if (rc) return rc;
+    if ((opt->precision != 64 && opt->precision != 32) || (opt->scheme != 1 && opt->scheme != 5) ||
+        !(opt->grav > 0) || !(opt->dry > 0) || !(opt->cfl_target > 0) || opt->cfl_target > 0.9 || !(opt->dtmax > 0) ||
+        (opt->block != 0 && (opt->block % 64 != 0 || opt->block > 1024))) {
+        set_error("invalid options"); return BRAIDS_EINVAL;
+    }
+    SolverBase *impl = nullptr;
+    if (opt->precision == 64) {
+        auto *s = new Solver<double>();
+        rc = s->init(mesh, opt, z);
+        impl = s;
+    } else {
+        auto *s = new Solver<float>();
+        rc = s->init(mesh, opt, z);
+        impl = s;
+    }
+    if (rc) { delete impl; return rc; }
+    *out = new braids_solver{impl};
+    return BRAIDS_OK;
+}
+
+extern "C" int braids_destroy(braids_solver *s)
+{
+    if (!s) return BRAIDS_OK;
+    delete s->impl;
+    delete s;
+    return BRAIDS_OK;
+}
+
+extern "C" int braids_set_stream(braids_solver *s, void *stream)
+{
+    if (!s) { set_error("null solver"); return BRAIDS_EINVAL; }
+    s->impl->stream = (hipStream_t)stream;
+    return BRAIDS_OK;
+}
+
+extern "C" int braids_set_state(braids_solver *s, const double *q, int prepare)
+{
+    if (!s || !q) { set_error("null argument"); return BRAIDS_EINVAL; }
+    return s->impl->set_state(q, prepare);
+}
+extern "C" int braids_get_state(braids_solver *s, double *q)
+{
+    if (!s || !q) { set_error("null argument"); return BRAIDS_EINVAL; }
+    return s->impl->get_state(q);
+}
+extern "C" int braids_set_bed(braids_solver *s, const double *z)
+{
+    if (!s || !z) { set_error("null argument"); return BRAIDS_EINVAL; }
+    return s->impl->set_bed(z, false);
+}
+extern "C" int braids_get_ledger(braids_solver *s, double *ledger)
+{
+    if (!s || (!ledger && s->impl->nb)) { set_error("null argument"); return BRAIDS_EINVAL; }
+    for (int b = 0; b < s->impl->nb; ++b) ledger[b] = s->impl->ledger[b];
+    return BRAIDS_OK;
+}
+extern "C" int braids_reset_ledger(braids_solver *s)
+{
+    if (!s) { set_error("null solver"); return BRAIDS_EINVAL; }
+    for (auto &v : s->impl->ledger) v = 0.0;
+    return BRAIDS_OK;
+}
+
+extern "C" int braids_step(braids_solver *s, double dt, double t, braids_step_result *res)
+{
+    if (!s || !res || !(dt > 0)) { set_error("invalid step arguments"); return BRAIDS_EINVAL; }
+    return s->impl->step(dt, t, res);
+}
+
+// next_boundary_knot (cutcell_solver.f90:9270-9294) over the hydrograph knots
+static double next_knot(const SolverBase *s, double t)
+{
+    double knot = DBL_MAX, tol = 32 * DBL_EPSILON * std::fmax(1.0, std::fabs(t));
+    if (s->series_offset.empty()) return knot;
+    for (int b = 0; b < s->nb; ++b)
+        for (int k = s->series_offset[b]; k < s->series_offset[b + 1]; ++k)
+            if (s->series_time[k] > t + tol && s->series_time[k] < knot) knot = s->series_time[k];
+    return knot;
+}
+
+extern "C" int braids_advance(braids_solver *s, int nsteps, double tend, double *time, double *step_hint,
+                              int *accepted, int *retries)
+{
+    if (!s || !time || !step_hint || !accepted || nsteps < 0) { set_error("invalid advance arguments"); return BRAIDS_EINVAL; }
+    SolverBase *x = s->impl;
+    *accepted = 0;
+    if (retries) *retries = 0;
+    while (*accepted < nsteps && *time < tend - 1e-12) {
+        double unclipped = *step_hint, dt = std::fmin(unclipped, tend - *time);
+        double knot = next_knot(x, *time);
+        if (knot < DBL_MAX) dt = std::fmin(dt, knot - *time);
+        bool clipped = dt < unclipped, ok = false;
+        int attempt;
+        braids_step_result r{};
+        for (attempt = 1; attempt <= 35; ++attempt) {
+            if (*time + dt <= *time) { set_error("timestep reached machine precision"); return BRAIDS_REJECTED; }
+            int rc = x->step(dt, *time, &r);
+            if (rc < 0) return rc;
+            if (rc == BRAIDS_OK) { ok = true; break; }
+            if (retries) ++*retries;
+            if (r.failure == 2 && r.trial_cfl_peak > x->cfl_target)
+                dt = std::fmax(.1, std::fmin(.95, .95 * x->cfl_target / r.trial_cfl_peak)) * dt;
+            else
+                dt = .5 * dt;
+        }
+        if (!ok) { set_error("step rejected 35 times"); return BRAIDS_REJECTED; }
+        if (attempt == 1 && clipped) *step_hint = unclipped;
+        else if (r.trial_cfl_peak > 0)
+            *step_hint = std::fmin(x->dtmax, std::fmax(.5, std::fmin(1.05, .95 * x->cfl_target / r.trial_cfl_peak)) * dt);
+        else *step_hint = std::fmin(x->dtmax, 1.05 * dt);
+        *time = *time + dt;
+        ++*accepted;
+    }
+    return BRAIDS_OK;
+}
+
+extern "C" int braids_native_euler(braids_solver *s, const double *q, const double *z, double dt, double t,
+                                   double *qout, double *water, int *code)
+{
+    if (!s || !q || !qout || !code) { set_error("null argument"); return BRAIDS_EINVAL; }
+    return s->impl->native_euler(q, z, dt, t, qout, water, code);
+}
+extern "C" int braids_native_rates(braids_solver *s, const double *q, const double *z, double dt, double t,
+                                   double *rate, double *water, int *code)
+{
+    if (!s || !q || !rate || !water || !code) { set_error("null argument"); return BRAIDS_EINVAL; }
+    return s->impl->native_rates(q, z, dt, t, rate, water, code);
+}
+extern "C" int river_get_face_speed(braids_solver *s, double *speed)
+{
+    if (!s || !speed) { set_error("null argument"); return BRAIDS_EINVAL; }
+    return s->impl->face_speed(speed);
+}
+
+extern "C" int braids_profile(braids_solver *s, int enable)
+{
+    if (!s) { set_error("null solver"); return BRAIDS_EINVAL; }
+    return s->impl->enable_profile(enable);
+}
+
+extern "C" int braids_profile_read(braids_solver *s, double *ms, long long *launches)
+{
+    if (!s || !ms || !launches) { set_error("null argument"); return BRAIDS_EINVAL; }
+    for (int k = 0; k < KK_COUNT; ++k) { ms[k] = s->impl->prof_ms[k]; launches[k] = s->impl->prof_n[k]; }
+    return BRAIDS_OK;
+}
+
+namespace {
+__global__ void k_stream_copy(const float4 *__restrict__ src, float4 *__restrict__ dst, long long n)
+{
+    long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    long long stride = (long long)gridDim.x * blockDim.x;
+    for (; i < n; i += stride) dst[i] = src[i];
+}
+}  // namespace
+
+extern "C" int braids_stream_copy(const void *src, void *dst, long long bytes, void *stream, int repeats, double *ms_per_copy)
+{
+    if (!src || !dst || bytes <= 0 || bytes % 16 != 0 || repeats <= 0) { set_error("invalid copy arguments"); return BRAIDS_EINVAL; }
+    hipStream_t st = (hipStream_t)stream;
+    long long n = bytes / 16;
+    int grid = (int)std::min<long long>((n + 255) / 256, 256LL * 64);
+    hipEvent_t a, b;
+    HIP_TRY(hipEventCreate(&a));
+    HIP_TRY(hipEventCreate(&b));
+    hipLaunchKernelGGL(k_stream_copy, dim3(grid), dim3(256), 0, st, (const float4 *)src, (float4 *)dst, n);
+    HIP_TRY(hipEventRecord(a, st));
+    for (int r = 0; r < repeats; ++r)
+        hipLaunchKernelGGL(k_stream_copy, dim3(grid), dim3(256), 0, st, (const float4 *)src, (float4 *)dst, n);
+    HIP_TRY(hipEventRecord(b, st));
+    HIP_TRY(hipEventSynchronize(b));
+    float ms = 0;
+    HIP_TRY(hipEventElapsedTime(&ms, a, b));
+    (void)hipEventDestroy(a); (void)hipEventDestroy(b);
+    if (ms_per_copy) *ms_per_copy = ms / repeats;
+    return BRAIDS_OK;
+}
+
+extern "C" int braids_mass(braids_solver *s, double *mass)
+{
+    if (!s || !mass) { set_error("null argument"); return BRAIDS_EINVAL; }
+    SolverBase *x = s->impl;
+    std::vector<double> q(3 * (size_t)x->nc);
+    int rc = x->get_state(q.data());
+    if (rc) return rc;
+    double sum = 0, comp = 0;
+    for (int c = 0; c < x->nc; ++c) {
+        double v = x->area[c] * q[c], t = sum + v;
+        if (std::fabs(sum) >= std::fabs(v)) comp += (sum - t) + v; else comp += (v - t) + sum;
+        sum = t;
+    }
+    *mass = sum + comp;
+    return BRAIDS_OK;
+}

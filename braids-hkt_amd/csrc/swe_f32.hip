@@ -1,0 +1,11 @@
+// FP32 mode: flux and momentum arithmetic in FP32 (contraction allowed);
+// depth inventory, bed, free-surface differencing and ledgers stay FP64.
+#include "swe_launch_impl.h"
+
+namespace braids {
+template void run_stage<float>(const MeshView<float> &, StateView<float>, StateView<float>, StateView<float>,
+                               const double *, Work<float>, Control *, double *, int, double, double,
+                               const LaunchCfg &, hipStream_t);
+template void run_rates<float>(const MeshView<float> &, StateView<float>, const double *, Work<float>, Control *,
+                               double *, double *, double, const LaunchCfg &, hipStream_t);
+}  // namespace braids

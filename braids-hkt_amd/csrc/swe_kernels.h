@@ -1,0 +1,735 @@
+// Stage kernels of the unstructured-mesh hydraulic step, templated on the
+// working precision R (double: FP64 mode; float: FP32 flux/update with FP64
+// depth inventory, bed, free-surface differencing and ledgers).
+//
+// One explicit Euler stage of the reference (euler, solver/cut_cells/
+// cutcell_solver.f90:10631-12090, hydraulic part) becomes five launches:
+//   k_prep    cells: active mask (:10795-10812), primitives (:7787-7817),
+//             LS gradient + limiter (:8536-8623), constant-surface flag
+//             (prepare_wet_dry_traces :9097-9133), max depth
+//   k_traces  one workgroup per open boundary: modes 6 and 2 (:9814-10270)
+//   k_face    faces: wet/dry aperture, run-up, f-wave flux, wall and open
+//             faces, well-balanced completion (:10938-11110)
+//   k_drain   cells: conservative draining factor (:11521-11538)
+//   k_gather  cells: donor limiting of the incident faces (:11539-11557),
+//             gather + CFL + update (:11564-11600), then the SSP stage
+//             combination, prepare_hydraulic_stage (:8513-8534) and, after the
+//             last stage, the Manning source (:14580-14600)
+// Accumulation is a per-cell gather over the CSR incidence: no atomics touch
+// the state; the only atomics are the order-independent max reductions.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+#include "swe_riemann.h"
+#include "swe_types.h"
+
+namespace braids {
+
+__device__ __forceinline__ unsigned long long dbits(double x) { return (unsigned long long)__double_as_longlong(x > 0.0 ? x : 0.0); }
+
+template <class T>
+__device__ __forceinline__ T wave_max(T v)
+{
+    for (int o = 32; o > 0; o >>= 1) {
+        T w = __shfl_xor(v, o, 64);
+        v = w > v ? w : v;
+    }
+    return v;
+}
+__device__ __forceinline__ int wave_or(int v)
+{
+    for (int o = 32; o > 0; o >>= 1) v |= __shfl_xor(v, o, 64);
+    return v;
+}
+
+// XCD-aware block remap: consecutive logical blocks (neighbouring mesh
+// ranges in Hilbert order) are kept on one XCD so they share its L2.
+__device__ __forceinline__ int logical_block(int remap)
+{
+    const int b = blockIdx.x;
+    if (!remap) return b;
+    // hardware dispatches block b to XCD b % 8; give XCD x one contiguous chunk
+    const int n = gridDim.x, q = n >> 3, r = n & 7;
+    const int x = b & 7, k = b >> 3;
+    return x * q + (x < r ? x : r) + k;
+}
+
+// ---------------------------------------------------------------------------
+template <class R, int SCHEME>
+__global__ void __launch_bounds__(256) k_prep(MeshView<R> m, StateView<R> q, const double *__restrict__ z,
+                                              Work<R> w, Control *ctl, int stage, int remap)
+{
+    const int nc = m.nc;
+    int c = logical_block(remap) * blockDim.x + threadIdx.x;
+    double hmx = 0.0;
+    if (c < nc) {
+        const R dry = m.dry;
+        const double h = q.h[c];
+        hmx = h;
+        const int k0 = m.offset[c], k1 = m.offset[c + 1];
+        bool act = h > (double)dry;
+        if (!act) {
+            for (int k = k0; k < k1; ++k) {
+                int fl = m.adj_flags[k], o = m.adj_other[k];
+                if (fl & ADJ_OPEN) { act = true; break; }
+                if (o >= 0 && q.h[o] > (double)dry) { act = true; break; }
+            }
+        }
+        int flags = act ? FLAG_ACTIVE : 0;
+        if (SCHEME == 5) {
+            R gx[4] = {R(0), R(0), R(0), R(0)}, gy[4] = {R(0), R(0), R(0), R(0)};
+            const double zc = z[c], etac = h + zc;
+            R uc = R(0), vc = R(0);
+            if (h > (double)dry) { uc = q.hu[c] / (R)h; vc = q.hv[c] / (R)h; }
+            if (act) {
+                R b0[4] = {R(0), R(0), R(0), R(0)}, b1[4] = {R(0), R(0), R(0), R(0)};
+                // lo/hi as offsets from the centre value for eta and z (FP64 differences)
+                R lo[4] = {R(0), uc, vc, R(0)}, hi[4] = {R(0), uc, vc, R(0)};
+                for (int k = k0; k < k1; ++k) {
+                    int o = m.adj_other[k];
+                    if (o < 0) continue;
+                    const R wr0 = m.adj_wr[k], wr1 = m.adj_wr[m.nadj + k];
+                    const double ho = q.h[o], zo = z[o];
+                    double etao = ho + zo;
+                    R uo = R(0), vo = R(0);
+                    if (ho > (double)dry) { uo = q.hu[o] / (R)ho; vo = q.hv[o] / (R)ho; }
+                    if (etac - zc > (double)dry && etao - zo <= (double)dry && zo >= etac) etao = etac;
+                    R de = (R)(etao - etac), du = uo - uc, dv = vo - vc, dz = (R)(zo - zc);
+                    b0[0] = b0[0] + wr0 * de; b1[0] = b1[0] + wr1 * de;
+                    b0[1] = b0[1] + wr0 * du; b1[1] = b1[1] + wr1 * du;
+                    b0[2] = b0[2] + wr0 * dv; b1[2] = b1[2] + wr1 * dv;
+                    b0[3] = b0[3] + wr0 * dz; b1[3] = b1[3] + wr1 * dz;
+                    lo[0] = vmin(lo[0], de); hi[0] = vmax(hi[0], de);
+                    lo[1] = vmin(lo[1], uo); hi[1] = vmax(hi[1], uo);
+                    lo[2] = vmin(lo[2], vo); hi[2] = vmax(hi[2], vo);
+                    lo[3] = vmin(lo[3], dz); hi[3] = vmax(hi[3], dz);
+                }
+                const R a11 = m.ls[c], a12 = m.ls[nc + c], a22 = m.ls[2 * nc + c], det = m.ls[3 * nc + c];
+                if (!(det <= R(1e-12) * ((a11 + a22) * (a11 + a22)))) {
+                    const R e64 = R(64) * Num<R>::eps();
+                    R guard[4], theta[4] = {R(1), R(1), R(1), R(1)};
+                    R cen[4] = {R(0), uc, vc, R(0)};
+                    // eta and z guards use the absolute centre values
+                    guard[0] = e64 * vmax(vmax(R(1), (R)fabs(etac)), vmax((R)fabs(etac + (double)lo[0]), (R)fabs(etac + (double)hi[0])));
+                    guard[3] = e64 * vmax(vmax(R(1), (R)fabs(zc)), vmax((R)fabs(zc + (double)lo[3]), (R)fabs(zc + (double)hi[3])));
+                    for (int j = 1; j < 3; ++j) guard[j] = e64 * vmax(vmax(R(1), (R)fabs(cen[j])), vmax((R)fabs(lo[j]), (R)fabs(hi[j])));
+                    for (int j = 0; j < 4; ++j) {
+                        gx[j] = (a22 * b0[j] - a12 * b1[j]) / det;
+                        gy[j] = (a11 * b1[j] - a12 * b0[j]) / det;
+                    }
+                    for (int k = k0; k < k1; ++k) {
+                        const R r0 = m.adj_d[k], r1 = m.adj_d[m.nadj + k];
+                        R delta[4];
+                        for (int j = 0; j < 4; ++j) delta[j] = gx[j] * r0 + gy[j] * r1;
+                        if ((m.adj_flags[k] & ADJ_OUTLET2) && etac - zc > (double)dry) {
+                            const int f = m.adj_face[k];
+                            if (fabs(uc * m.normal[f] + vc * m.normal[m.nf + f]) < Num<R>::sqrt_(m.grav * (R)(etac - zc))) {
+                                if (fabs(delta[3]) > e64 * vmax(R(1), (R)fabs(zc))) continue;
+                            }
+                        }
+                        for (int j = 0; j < 4; ++j) {
+                            if (fabs(delta[j]) <= guard[j]) continue;
+                            if (delta[j] > R(0)) theta[j] = vmin(theta[j], (hi[j] - cen[j]) / delta[j]);
+                            if (delta[j] < R(0)) theta[j] = vmin(theta[j], (lo[j] - cen[j]) / delta[j]);
+                        }
+                    }
+                    for (int j = 0; j < 4; ++j) {
+                        R t = vmax(R(0), theta[j]);
+                        gx[j] = gx[j] * t; gy[j] = gy[j] * t;
+                    }
+                }
+            }
+            // constant-surface flag (prepare_wet_dry_traces)
+            bool flatten = h <= (double)dry;
+            if (!flatten) {
+                for (int k = k0; k < k1; ++k) {
+                    const R r0 = m.adj_d[k], r1 = m.adj_d[m.nadj + k];
+                    double eta = etac + (double)(gx[0] * r0) + (double)(gy[0] * r1);
+                    double zb = zc + (double)(gx[3] * r0) + (double)(gy[3] * r1);
+                    if (eta - zb <= (double)dry || (m.irregular[c] && fabs(zb - zc) >= h)) { flatten = true; break; }
+                }
+            }
+            if (flatten) flags |= FLAG_HCONST;
+            for (int j = 0; j < 4; ++j) { w.grad[(2 * j) * nc + c] = gx[j]; w.grad[(2 * j + 1) * nc + c] = gy[j]; }
+        }
+        w.flags[c] = (uint8_t)flags;
+    }
+    double wm = wave_max(hmx);
+    if ((threadIdx.x & 63) == 0 && wm > 0.0) atomicMax(&ctl->hmax_bits[stage], dbits(wm));
+}
+
+// FP64-exact form of the LS-MUSCL preparation (same operation order as the
+// reference: primitives, then adjacent - centre differences in FP64).
+static __global__ void __launch_bounds__(256) k_prep_ls64(MeshView<double> m, StateView<double> q, const double *__restrict__ z,
+                                                        Work<double> w, Control *ctl, int stage, int remap)
+{
+    const int nc = m.nc;
+    int c = logical_block(remap) * blockDim.x + threadIdx.x;
+    double hmx = 0.0;
+    if (c < nc) {
+        const double dry = m.dry;
+        const double h = q.h[c];
+        hmx = h;
+        const int k0 = m.offset[c], k1 = m.offset[c + 1];
+        bool act = h > dry;
+        if (!act) {
+            for (int k = k0; k < k1; ++k) {
+                int fl = m.adj_flags[k], o = m.adj_other[k];
+                if (fl & ADJ_OPEN) { act = true; break; }
+                if (o >= 0 && q.h[o] > dry) { act = true; break; }
+            }
+        }
+        int flags = act ? FLAG_ACTIVE : 0;
+        double vc[4] = {h + z[c], 0.0, 0.0, z[c]};
+        if (h > dry) { vc[1] = q.hu[c] / h; vc[2] = q.hv[c] / h; }
+        double gx[4] = {0, 0, 0, 0}, gy[4] = {0, 0, 0, 0};
+        if (act) {
+            double b0[4] = {0, 0, 0, 0}, b1[4] = {0, 0, 0, 0}, lo[4], hi[4];
+            for (int j = 0; j < 4; ++j) { lo[j] = vc[j]; hi[j] = vc[j]; }
+            for (int k = k0; k < k1; ++k) {
+                int o = m.adj_other[k];
+                if (o < 0) continue;
+                const double wr0 = m.adj_wr[k], wr1 = m.adj_wr[m.nadj + k];
+                const double ho = q.h[o];
+                double adj[4] = {ho + z[o], 0.0, 0.0, z[o]};
+                if (ho > dry) { adj[1] = q.hu[o] / ho; adj[2] = q.hv[o] / ho; }
+                if (vc[0] - vc[3] > dry && adj[0] - adj[3] <= dry && adj[3] >= vc[0]) adj[0] = vc[0];
+                for (int j = 0; j < 4; ++j) {
+                    b0[j] = b0[j] + wr0 * (adj[j] - vc[j]);
+                    b1[j] = b1[j] + wr1 * (adj[j] - vc[j]);
+                    lo[j] = vmin(lo[j], adj[j]); hi[j] = vmax(hi[j], adj[j]);
+                }
+            }
+            const double a11 = m.ls[c], a12 = m.ls[nc + c], a22 = m.ls[2 * nc + c], det = m.ls[3 * nc + c];
+            if (!(det <= 1e-12 * ((a11 + a22) * (a11 + a22)))) {
+                const double e64 = 64 * 2.2204460492503131e-16;
+                double guard[4], theta[4] = {1, 1, 1, 1};
+                for (int j = 0; j < 4; ++j) {
+                    gx[j] = (a22 * b0[j] - a12 * b1[j]) / det;
+                    gy[j] = (a11 * b1[j] - a12 * b0[j]) / det;
+                    guard[j] = e64 * vmax(vmax(1.0, fabs(vc[j])), vmax(fabs(lo[j]), fabs(hi[j])));
+                }
+                for (int k = k0; k < k1; ++k) {
+                    const double r0 = m.adj_d[k], r1 = m.adj_d[m.nadj + k];
+                    double delta[4];
+                    for (int j = 0; j < 4; ++j) delta[j] = gx[j] * r0 + gy[j] * r1;
+                    if ((m.adj_flags[k] & ADJ_OUTLET2) && vc[0] - vc[3] > dry) {
+                        const int f = m.adj_face[k];
+                        if (fabs(vc[1] * m.normal[f] + vc[2] * m.normal[m.nf + f]) < sqrt(m.grav * (vc[0] - vc[3]))) {
+                            if (fabs(delta[3]) > e64 * vmax(1.0, fabs(vc[3]))) continue;
+                        }
+                    }
+                    for (int j = 0; j < 4; ++j) {
+                        if (fabs(delta[j]) <= guard[j]) continue;
+                        if (delta[j] > 0) theta[j] = vmin(theta[j], (hi[j] - vc[j]) / delta[j]);
+                        if (delta[j] < 0) theta[j] = vmin(theta[j], (lo[j] - vc[j]) / delta[j]);
+                    }
+                }
+                for (int j = 0; j < 4; ++j) {
+                    double t = vmax(0.0, theta[j]);
+                    gx[j] = gx[j] * t; gy[j] = gy[j] * t;
+                }
+            }
+        }
+        bool flatten = h <= dry;
+        if (!flatten) {
+            for (int k = k0; k < k1; ++k) {
+                const double r0 = m.adj_d[k], r1 = m.adj_d[m.nadj + k];
+                double eta = vc[0] + gx[0] * r0 + gy[0] * r1;
+                double zb = vc[3] + gx[3] * r0 + gy[3] * r1;
+                if (eta - zb <= dry || (m.irregular[c] && fabs(zb - vc[3]) >= h)) { flatten = true; break; }
+            }
+        }
+        if (flatten) flags |= FLAG_HCONST;
+        for (int j = 0; j < 4; ++j) { w.grad[(2 * j) * nc + c] = gx[j]; w.grad[(2 * j + 1) * nc + c] = gy[j]; }
+        w.flags[c] = (uint8_t)flags;
+    }
+    double wm = wave_max(hmx);
+    if ((threadIdx.x & 63) == 0 && wm > 0.0) atomicMax(&ctl->hmax_bits[stage], dbits(wm));
+}
+
+// face_value (cutcell_solver.f90:8999-9051) for schemes 1 and 5.
+// Returns eta and z in FP64, velocities in R.
+template <class R, int SCHEME>
+__device__ __forceinline__ void face_value(const MeshView<R> &m, const StateView<R> &q, const double *z,
+                                           const Work<R> &w, int c, R r0, R r1,
+                                           double &eta, R &u, R &v, double &zb)
+{
+    const double h = q.h[c];
+    const double zc = z[c], etac = h + zc;
+    R uc = R(0), vc = R(0);
+    if (h > (double)m.dry) { uc = q.hu[c] / (R)h; vc = q.hv[c] / (R)h; }
+    if (SCHEME == 1) { eta = etac; u = uc; v = vc; zb = zc; return; }
+    const int nc = m.nc;
+    const R g0x = w.grad[c], g0y = w.grad[nc + c];
+    const R g1x = w.grad[2 * nc + c], g1y = w.grad[3 * nc + c];
+    const R g2x = w.grad[4 * nc + c], g2y = w.grad[5 * nc + c];
+    const R g3x = w.grad[6 * nc + c], g3y = w.grad[7 * nc + c];
+    if (sizeof(R) == 8) {
+        eta = etac + (double)g0x * (double)r0 + (double)g0y * (double)r1;
+        zb = zc + (double)g3x * (double)r0 + (double)g3y * (double)r1;
+    } else {
+        eta = etac + ((double)(g0x * r0) + (double)(g0y * r1));
+        zb = zc + ((double)(g3x * r0) + (double)(g3y * r1));
+    }
+    u = uc + g1x * r0 + g1y * r1;
+    v = vc + g2x * r0 + g2y * r1;
+    if (w.flags[c] & FLAG_HCONST) {
+        eta = etac; u = uc; v = vc;
+        if (etac - zc <= (double)m.dry) zb = zc;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Boundary traces: one workgroup per open boundary.  The running sums of the
+// total-discharge distribution are formed by one lane in boundary face order.
+template <class R, int SCHEME>
+__global__ void __launch_bounds__(256) k_traces(MeshView<R> m, StateView<R> q, const double *__restrict__ z,
+                                                Work<R> w, Control *ctl, int stage, double t)
+{
+    const int b = blockIdx.x;
+    if (b >= m.nb) return;
+    const int k0 = m.bnd_offset[b], k1 = m.bnd_offset[b + 1];
+    const double dry = (double)m.dry, g = (double)m.grav;
+    __shared__ double s_total, s_zavg, s_hyd[2];
+    __shared__ int s_fail;
+    if (threadIdx.x == 0) {
+        // boundary_hydraulic_at (:9248-9269): hold the nearest knot outside the series
+        double Q = m.forcing_q[b], S = m.forcing_stage[b];
+        if (m.series_offset) {
+            int j0 = m.series_offset[b], j1 = m.series_offset[b + 1], ns = j1 - j0;
+            if (ns > 0) {
+                const double *tt = m.series_time + j0, *qq = m.series_q + j0, *ss = m.series_stage + j0;
+                if (t <= tt[0]) { Q = qq[0]; S = ss[0]; }
+                else if (t >= tt[ns - 1]) { Q = qq[ns - 1]; S = ss[ns - 1]; }
+                else {
+                    int k = 1;
+                    while (k < ns - 1 && !(t <= tt[k])) ++k;
+                    double wgt = (t - tt[k - 1]) / (tt[k] - tt[k - 1]);
+                    Q = (1 - wgt) * qq[k - 1] + wgt * qq[k];
+                    S = (1 - wgt) * ss[k - 1] + wgt * ss[k];
+                }
+            }
+        }
+        s_hyd[0] = Q; s_hyd[1] = S; s_fail = 0;
+        w.hydraulic[2 * b] = Q; w.hydraulic[2 * b + 1] = S;
+        if (m.mode[b] == 6) {
+            double zavg = 0, wet = 0, total = 0;
+            for (int k = k0; k < k1; ++k) {
+                int f = m.bnd_faces[k], o = m.owner[f];
+                if (q.h[o] <= dry) continue;
+                zavg = zavg + (z[o] + q.h[o]) * (double)m.length[f];
+                wet = wet + (double)m.length[f];
+            }
+            if (Q > 0 && wet <= 0) s_fail = 8;
+            if (wet > 0) zavg = zavg / wet;
+            for (int k = k0; k < k1; ++k) {
+                int f = m.bnd_faces[k], o = m.owner[f];
+                w.target_u[k] = R(0);
+                double h = q.h[o];
+                if (h <= dry) continue;
+                double weight = pow(vmax(zavg - z[o], 0.0), 1.5) * pow(vmax(h, 1e-2), 1.0 / 6.0);
+                if ((double)m.manning[o] > 0) weight = weight / (double)m.manning[o];
+                double tu = weight * (double)m.length[f];
+                w.target_u[k] = (R)tu;
+                total = total + tu;
+            }
+            if (Q > 0 && total <= 0) s_fail = 8;
+            s_total = total; s_zavg = zavg;
+        }
+    }
+    __syncthreads();
+    int fail = s_fail;
+    const double Q = s_hyd[0], S = s_hyd[1];
+    const int nf = m.nf;
+    if (!fail) {
+        for (int k = k0 + threadIdx.x; k < k1; k += blockDim.x) {
+            const int f = m.bnd_faces[k], o = m.owner[f];
+            const R nxr = m.normal[f], nyr = m.normal[nf + f];
+            const double nx = nxr, ny = nyr;
+            double tr0 = 0, tr1 = 0, tr2 = 0;
+            const double h = q.h[o];
+            if (m.mode[b] == 6) {
+                if (h <= dry || Q == 0) {
+                    if (h > dry) {
+                        double qx = q.hu[o], qy = q.hv[o];
+                        double d = qx * nx + qy * ny;
+                        tr0 = h; tr1 = h * (qx / h - d / h * nx); tr2 = h * (qy / h - d / h * ny);
+                    }
+                } else {
+                    double ub = Q * (double)w.target_u[k] / (s_total * h * (double)m.length[f]);
+                    double qx = q.hu[o], qy = q.hv[o];
+                    double d = qx * nx + qy * ny;
+                    double tx = qx / h - d / h * nx, ty = qy / h - d / h * ny;
+                    if (sqrt(ub * ub + (tx * tx + ty * ty)) > 50.0) { fail = 6; }
+                    tr0 = h; tr1 = h * (-(ub * nx) + tx); tr2 = h * (-(ub * ny) + ty);
+                }
+            } else if (m.mode[b] == 2) {
+                double es0 = h, es1 = (double)q.hu[o], es2 = (double)q.hv[o], eb = z[o];
+                if (h > dry) {
+                    double eta, zb; R u, v;
+                    face_value<R, SCHEME>(m, q, z, w, o, m.dl[f], m.dl[nf + f], eta, u, v, zb);
+                    if (fabs(zb - z[o]) > 64 * 2.2204460492503131e-16 * vmax(1.0, fabs(z[o])) &&
+                        fabs(es1 * nx + es2 * ny) < h * sqrt(g * h)) {
+                        eb = zb; es0 = vmax(eta - eb, 0.0); es1 = es0 * (double)u; es2 = es0 * (double)v;
+                    }
+                }
+                double hh = es0, u = 0;
+                if (hh > dry) u = -(es1 * nx + es2 * ny) / hh;
+                double c = sqrt(g * hh), inv = u - 2 * c;
+                if (hh <= dry) { /* dry outlet face: zero trace */ }
+                else if (u <= -c) { tr0 = es0; tr1 = es1; tr2 = es2; }
+                else if (u >= c) { tr0 = es0; }
+                else {
+                    hh = vmax(S - eb, 0.0);
+                    if (hh <= dry || inv <= -3 * sqrt(g * hh)) {
+                        double t3 = inv / 3;
+                        hh = (t3 * t3) / g;
+                        if (hh <= dry) { tr0 = es0; }
+                        else {
+                            double ub = -sqrt(g * hh);
+                            double tx = es1 / es0 + u * nx, ty = es2 / es0 + u * ny;
+                            tr0 = hh; tr1 = hh * (-(ub * nx) + tx); tr2 = hh * (-(ub * ny) + ty);
+                        }
+                    } else {
+                        double ub = inv + 2 * sqrt(g * hh);
+                        if (ub > 0) { tr0 = es0; }
+                        else if (fabs(ub) >= sqrt(g * hh)) { fail = 8; }
+                        else {
+                            double tx = es1 / es0 + u * nx, ty = es2 / es0 + u * ny;
+                            tr0 = hh; tr1 = hh * (-(ub * nx) + tx); tr2 = hh * (-(ub * ny) + ty);
+                        }
+                    }
+                }
+            } else {
+                fail = 99;
+            }
+            w.trace[f] = (R)tr0; w.trace[nf + f] = (R)tr1; w.trace[2 * nf + f] = (R)tr2;
+        }
+    }
+    if (fail) atomicMax(&ctl->failure[stage], FAIL_TRACE + fail);
+}
+
+// ---------------------------------------------------------------------------
+template <class R, int SCHEME>
+__global__ void __launch_bounds__(256) k_face(MeshView<R> m, StateView<R> q, const double *__restrict__ z,
+                                              Work<R> w, int remap)
+{
+    const int nf = m.nf;
+    const int f = logical_block(remap) * blockDim.x + threadIdx.x;
+    if (f >= nf) return;
+    const R g = m.grav, dry = m.dry;
+    const int c = m.owner[f], n = m.neighbour[f], b = m.bid[f];
+    const R nx = m.normal[f], ny = m.normal[nf + f];
+    R fl[3] = {R(0), R(0), R(0)}, fr[3] = {R(0), R(0), R(0)}, speed = R(0);
+    const double hc = q.h[c];
+    if (b < 0 && hc <= (double)dry) {
+        if (n < 0 || q.h[n] <= (double)dry) {
+            w.F[f] = R(0); w.F[nf + f] = R(0); w.F[2 * nf + f] = R(0);
+            w.F[3 * nf + f] = R(0); w.F[4 * nf + f] = R(0); w.F[5 * nf + f] = R(0);
+            return;
+        }
+    }
+    double etaL, zL; R uL, vL;
+    face_value<R, SCHEME>(m, q, z, w, c, m.dl[f], m.dl[nf + f], etaL, uL, vL, zL);
+    double depthL = vmax(etaL - zL, 0.0);
+    if (hc <= (double)dry) depthL = 0;
+    if ((w.flags[c] & FLAG_HCONST) && depthL <= (double)dry) depthL = 0;
+    R l[3] = {(R)depthL, (R)depthL * uL, (R)depthL * vL};
+    if (sizeof(R) == 8) { l[1] = (R)(depthL * (double)uL); l[2] = (R)(depthL * (double)vL); }
+    R r[3];
+    double etaR = 0, zR = 0, depthR = 0;
+    if (n >= 0) {
+        const double hn = q.h[n];
+        R uR, vR;
+        face_value<R, SCHEME>(m, q, z, w, n, m.dr[f], m.dr[nf + f], etaR, uR, vR, zR);
+        depthR = vmax(etaR - zR, 0.0);
+        if (hn <= (double)dry) depthR = 0;
+        if ((w.flags[n] & FLAG_HCONST) && depthR <= (double)dry) depthR = 0;
+        r[0] = (R)depthR; r[1] = (R)depthR * uR; r[2] = (R)depthR * vR;
+        if (sizeof(R) == 8) { r[1] = (R)(depthR * (double)uR); r[2] = (R)(depthR * (double)vR); }
+        bool closed = false;
+        if (vmin(hc, hn) <= (double)dry) {
+            double ofd, nfd;
+            if (sizeof(R) == 8) {
+                ofd = fabs((double)m.dl[f] * (double)nx + (double)m.dl[nf + f] * (double)ny);
+                nfd = fabs((double)m.dr[f] * (double)nx + (double)m.dr[nf + f] * (double)ny);
+            } else {
+                ofd = fabs((double)m.dl[f] * (double)nx + (double)m.dl[nf + f] * (double)ny);
+                nfd = fabs((double)m.dr[f] * (double)nx + (double)m.dr[nf + f] * (double)ny);
+            }
+            double shared;
+            if (ofd + nfd > 2.2250738585072014e-308)
+                shared = (nfd * (z[c] + hc) + ofd * (z[n] + hn)) / (ofd + nfd) - vmax(z[c], z[n]);
+            else
+                shared = .5 * ((z[c] + hc) + (z[n] + hn)) - vmax(z[c], z[n]);
+            closed = shared <= (double)dry;
+            if (closed) closed = !runup_opens_face<R>(l, r, zL, zR, nx, ny, g, dry);
+        }
+        if (closed) {
+            fl[0] = R(0); fr[0] = R(0);
+            fl[1] = R(.5) * g * (l[0] * l[0]) * nx; fl[2] = R(.5) * g * (l[0] * l[0]) * ny;
+            fr[1] = R(.5) * g * (r[0] * r[0]) * nx; fr[2] = R(.5) * g * (r[0] * r[0]) * ny;
+            if (l[0] > dry) speed = vmax(speed, (R)fabs((l[1] * nx + l[2] * ny) / l[0]) + Num<R>::sqrt_(g * l[0]));
+            if (r[0] > dry) speed = vmax(speed, (R)fabs((r[1] * nx + r[2] * ny) / r[0]) + Num<R>::sqrt_(g * r[0]));
+        } else {
+            wave_flux<R>(l, r, zL, zR, nx, ny, g, dry, fl, fr, speed);
+        }
+    } else if (b < 0) {
+        R d = l[1] * nx + l[2] * ny;
+        r[0] = l[0]; r[1] = l[1] - R(2) * d * nx; r[2] = l[2] - R(2) * d * ny;
+        wave_flux<R>(l, r, zL, zL, nx, ny, g, dry, fl, fr, speed);
+        fl[0] = R(0);
+        R p = fl[1] * nx + fl[2] * ny;
+        fl[1] = p * nx; fl[2] = p * ny;
+        fr[0] = fl[0]; fr[1] = fl[1]; fr[2] = fl[2];
+    } else {
+        l[0] = w.trace[f]; l[1] = w.trace[nf + f]; l[2] = w.trace[2 * nf + f];
+        R depth = l[0];
+        fl[0] = l[1] * nx + l[2] * ny;
+        if (depth > dry) {
+            fl[1] = fl[0] * l[1] / depth + R(.5) * g * (depth * depth) * nx;
+            fl[2] = fl[0] * l[2] / depth + R(.5) * g * (depth * depth) * ny;
+        }
+        fr[0] = fl[0]; fr[1] = fl[1]; fr[2] = fl[2];
+        speed = Num<R>::sqrt_(g * depth);
+        if (depth > dry) speed = speed + (R)fabs(fl[0]) / depth;
+    }
+    // well-balanced bed-source completion (:11063-11110), differences in FP64
+    const double gg = (double)g;
+    if (n >= 0 || b < 0) {
+        if (hc > (double)dry) {
+            double corr;
+            if (w.flags[c] & FLAG_HCONST) corr = sizeof(R) == 8 ? .5 * gg * (hc * hc - (double)l[0] * (double)l[0])
+                                                                 : .5 * gg * (hc * hc - depthL * depthL);
+            else corr = .5 * gg * (hc + vmax(etaL - zL, 0.0)) * (zL - z[c]);
+            if (sizeof(R) == 8) { fl[1] = fl[1] + (R)corr * nx; fl[2] = fl[2] + (R)corr * ny; }
+            else { fl[1] = (R)((double)fl[1] + corr * (double)nx); fl[2] = (R)((double)fl[2] + corr * (double)ny); }
+        }
+        if (n >= 0) {
+            const double hn = q.h[n];
+            if (hn > (double)dry) {
+                double corr;
+                if (w.flags[n] & FLAG_HCONST) corr = sizeof(R) == 8 ? .5 * gg * (hn * hn - (double)r[0] * (double)r[0])
+                                                                     : .5 * gg * (hn * hn - depthR * depthR);
+                else corr = .5 * gg * (hn + vmax(etaR - zR, 0.0)) * (zR - z[n]);
+                if (sizeof(R) == 8) { fr[1] = fr[1] + (R)corr * nx; fr[2] = fr[2] + (R)corr * ny; }
+                else { fr[1] = (R)((double)fr[1] + corr * (double)nx); fr[2] = (R)((double)fr[2] + corr * (double)ny); }
+            }
+        } else {
+            fr[0] = fl[0]; fr[1] = fl[1]; fr[2] = fl[2];
+        }
+    } else if (m.mode[b] == 2) {
+        if (hc > (double)dry) {
+            double qn = (double)q.hu[c] * (double)nx + (double)q.hv[c] * (double)ny;
+            if (fabs(zL - z[c]) > 64 * 2.2204460492503131e-16 * vmax(1.0, fabs(z[c])) && fabs(qn) < hc * sqrt(gg * hc)) {
+                double d = vmax(etaL - zL, 0.0), corr;
+                if (w.flags[c] & FLAG_HCONST) corr = .5 * gg * (hc * hc - d * d);
+                else corr = .5 * gg * (hc + d) * (zL - z[c]);
+                if (sizeof(R) == 8) { fl[1] = fl[1] + (R)corr * nx; fl[2] = fl[2] + (R)corr * ny; }
+                else { fl[1] = (R)((double)fl[1] + corr * (double)nx); fl[2] = (R)((double)fl[2] + corr * (double)ny); }
+            }
+        }
+        fr[0] = fl[0]; fr[1] = fl[1]; fr[2] = fl[2];
+    }
+    w.F[f] = fl[0]; w.F[nf + f] = fl[1]; w.F[2 * nf + f] = fl[2];
+    w.F[3 * nf + f] = fr[1]; w.F[4 * nf + f] = fr[2]; w.F[5 * nf + f] = speed;
+}
+
+// ---------------------------------------------------------------------------
+template <class R>
+__global__ void __launch_bounds__(256) k_drain(MeshView<R> m, StateView<R> q, Work<R> w, Control *ctl,
+                                               int stage, double dt, int remap)
+{
+    const int c = logical_block(remap) * blockDim.x + threadIdx.x;
+    int drains = 0, fail = 0;
+    if (c < m.nc) {
+        R theta = R(1);
+        if (w.flags[c] & FLAG_ACTIVE) {
+            const int k0 = m.offset[c], k1 = m.offset[c + 1];
+            double outgoing = 0;
+            for (int k = k0; k < k1; ++k) {
+                const int f = m.adj_face[k];
+                const R s = (m.adj_flags[k] & ADJ_OWNER) ? R(1) : R(-1);
+                outgoing = outgoing + (double)vmax(s * w.F[f], R(0)) * (double)m.length[f];
+            }
+            const double hA = q.h[c] * (double)m.area[c];
+            if (dt * outgoing >= hA && outgoing > 0) {
+                const double margin = 1.0 - 32.0 * (double)Num<R>::eps();
+                theta = (R)(vmin(1.0, hA / (dt * outgoing)) * margin);
+                drains = 1;
+                // a prescribed open-boundary outflow may not be reduced
+                for (int k = k0; k < k1; ++k)
+                    if ((m.adj_flags[k] & ADJ_OPEN) && w.F[m.adj_face[k]] > R(0)) fail = 2;
+            }
+        }
+        w.theta[c] = theta;
+    }
+    int any = wave_or(drains);
+    if ((threadIdx.x & 63) == 0 && any) atomicOr(&ctl->drain_any[stage], 1);
+    if (fail) atomicMax(&ctl->failure[stage], FAIL_DRAIN);
+}
+
+// Gather + update.  STAGE 0/1/2 select the SSPRK(3,3) combination
+// (advance_ssp33, cutcell_solver.f90:15858-15986): a = E(q); b = 3/4 q + 1/4 E(a);
+// c = 1/3 q + 2/3 E(b).  Each output is prepared (dry momentum cleared, speed
+// capped); the last one also receives the semi-implicit Manning source.
+template <class R, int STAGE>
+__global__ void __launch_bounds__(256) k_gather(MeshView<R> m, StateView<R> in, StateView<R> base,
+                                                StateView<R> out, Work<R> w, Control *ctl,
+                                                double dt, int remap)
+{
+    const int nc = m.nc, nf = m.nf;
+    const int c = logical_block(remap) * blockDim.x + threadIdx.x;
+    double ratio = 0.0;
+    int fail = 0;
+    if (c < nc) {
+        const double h = in.h[c];
+        double nh = h;
+        R nhu = in.hu[c], nhv = in.hv[c];
+        const R g = m.grav, dry = m.dry;
+        if (w.flags[c] & FLAG_ACTIVE) {
+            const int k0 = m.offset[c], k1 = m.offset[c + 1];
+            const bool drain = ctl->drain_any[STAGE] != 0;
+            const R A = m.area[c];
+            double s0 = 0.0;
+            R s1 = R(0), s2 = R(0), cfl = R(0);
+            const R p = R(.5) * g * (R)(h * h);
+            for (int k = k0; k < k1; ++k) {
+                const int f = m.adj_face[k];
+                const bool own = (m.adj_flags[k] & ADJ_OWNER) != 0;
+                R F1 = w.F[f];
+                R G2 = own ? w.F[nf + f] : w.F[3 * nf + f];
+                R G3 = own ? w.F[2 * nf + f] : w.F[4 * nf + f];
+                const R S = w.F[5 * nf + f], L = m.length[f];
+                if (drain && F1 != R(0)) {
+                    const int o = m.adj_other[k];
+                    // donor: owner for positive flux, neighbour for negative
+                    const int donor = (F1 > R(0)) == own ? c : o;
+                    if (donor >= 0) {
+                        const R th = w.theta[donor];
+                        if (th != R(1)) {
+                            const R corr = (th - R(1)) * F1;
+                            R vx = R(0), vy = R(0);
+                            const double hd = in.h[donor];
+                            if (hd > (double)dry) {
+                                if (sizeof(R) == 8) { vx = (R)((double)in.hu[donor] / hd); vy = (R)((double)in.hv[donor] / hd); }
+                                else { vx = in.hu[donor] / (R)hd; vy = in.hv[donor] / (R)hd; }
+                            }
+                            F1 = F1 * th;
+                            G2 = G2 + corr * vx; G3 = G3 + corr * vy;
+                        }
+                    }
+                }
+                const R wgt = L / A;
+                cfl = cfl + L * S;
+                const R nxf = m.normal[f], nyf = m.normal[nf + f];
+                if (own) {
+                    if (sizeof(R) == 8) s0 = s0 + (double)wgt * (double)F1;
+                    else s0 = s0 + ((double)L / (double)A) * (double)F1;
+                    s1 = s1 + wgt * (G2 - p * nxf);
+                    s2 = s2 + wgt * (G3 - p * nyf);
+                } else {
+                    if (sizeof(R) == 8) s0 = s0 - (double)wgt * (double)F1;
+                    else s0 = s0 - ((double)L / (double)A) * (double)F1;
+                    s1 = s1 - wgt * (G2 - p * nxf);
+                    s2 = s2 - wgt * (G3 - p * nyf);
+                }
+            }
+            ratio = (double)(R(dt) * cfl / A);
+            if (sizeof(R) == 8) ratio = dt * (double)cfl / (double)A;
+            if (ratio > (double)m.cfl_target) fail = 2;
+            nh = h - dt * s0;
+            nhu = in.hu[c] - (R)dt * s1;
+            nhv = in.hv[c] - (R)dt * s2;
+            if (sizeof(R) == 8) { nhu = (R)((double)in.hu[c] - dt * (double)s1); nhv = (R)((double)in.hv[c] - dt * (double)s2); }
+            if (nh < 0) {
+                const double hmax = __longlong_as_double((long long)ctl->hmax_bits[STAGE]);
+                if (-nh <= 64 * 2.2204460492503131e-16 * vmax(hmax, 1.0)) { nh = 0; nhu = R(0); nhv = R(0); }
+                else fail = fail > 3 ? fail : 3;
+            }
+            if (!isfinite(nh) || !isfinite((double)nhu) || !isfinite((double)nhv)) fail = fail > 5 ? fail : 5;
+        }
+        // SSP stage combination
+        if (STAGE == 3) { out.h[c] = nh; out.hu[c] = nhu; out.hv[c] = nhv; }
+        else if (STAGE == 1) {
+            nh = .75 * base.h[c] + .25 * nh;
+            if (sizeof(R) == 8) { nhu = (R)(.75 * (double)base.hu[c] + .25 * (double)nhu); nhv = (R)(.75 * (double)base.hv[c] + .25 * (double)nhv); }
+            else { nhu = R(.75) * base.hu[c] + R(.25) * nhu; nhv = R(.75) * base.hv[c] + R(.25) * nhv; }
+        } else if (STAGE == 2) {
+            nh = base.h[c] / 3.0 + 2.0 * nh / 3.0;
+            if (sizeof(R) == 8) { nhu = (R)((double)base.hu[c] / 3.0 + 2.0 * (double)nhu / 3.0); nhv = (R)((double)base.hv[c] / 3.0 + 2.0 * (double)nhv / 3.0); }
+            else { nhu = base.hu[c] / R(3) + R(2) * nhu / R(3); nhv = base.hv[c] / R(3) + R(2) * nhv / R(3); }
+        }
+        // prepare_hydraulic_stage
+        if (STAGE == 3) { /* plain Euler stage: output as computed */ }
+        else if (nh <= (double)dry) { nhu = R(0); nhv = R(0); }
+        else {
+            R sp = Num<R>::sqrt_(nhu * nhu + nhv * nhv) / (R)nh;
+            if (sizeof(R) == 8) sp = (R)(sqrt((double)nhu * (double)nhu + (double)nhv * (double)nhv) / nh);
+            if (sp > R(50)) { nhu = nhu * (R(50) / sp); nhv = nhv * (R(50) / sp); }
+        }
+        if (STAGE == 2) {
+            // Manning part of source(): hu /= 1 + dt g n^2 |u| / h^(4/3)
+            if (nh <= (double)dry) { nhu = R(0); nhv = R(0); }
+            else {
+                const double sp = sqrt((double)nhu * (double)nhu + (double)nhv * (double)nhv) / nh;
+                const double n = (double)m.manning[c];
+                const double d = 1 + dt * (double)g * (n * n) * sp / pow(nh, 4.0 / 3);
+                nhu = (R)((double)nhu / d); nhv = (R)((double)nhv / d);
+            }
+            if (!isfinite(nh) || !isfinite((double)nhu) || !isfinite((double)nhv) || nh < 0) fail = fail > 5 ? fail : 5;
+        }
+        if (STAGE != 3) { out.h[c] = nh; out.hu[c] = nhu; out.hv[c] = nhv; }
+    }
+    double wm = wave_max(ratio);
+    if ((threadIdx.x & 63) == 0 && wm > 0.0) atomicMax(&ctl->peak_bits[STAGE], dbits(wm));
+    if (fail) atomicMax(&ctl->failure[STAGE], fail);
+}
+
+// Spatial rate without draining or CFL (bend_native_rates_tail.inc:4-20):
+// rate = -sum_k s_k (L/A)(F - 1/2 g h^2 n); plus the shared face water flux.
+template <class R>
+__global__ void __launch_bounds__(256) k_rates(MeshView<R> m, StateView<R> in, Work<R> w,
+                                               double *rate, double *water, int remap)
+{
+    const int nc = m.nc, nf = m.nf;
+    const int c = logical_block(remap) * blockDim.x + threadIdx.x;
+    if (c < nf) water[c] = (double)w.F[c];
+    if (c < nc) {
+        const double h = in.h[c];
+        const R A = m.area[c];
+        const R p = R(.5) * m.grav * (R)(h * h);
+        double s0 = 0;
+        R s1 = R(0), s2 = R(0);
+        for (int k = m.offset[c]; k < m.offset[c + 1]; ++k) {
+            const int f = m.adj_face[k];
+            const bool own = (m.adj_flags[k] & ADJ_OWNER) != 0;
+            const R wgt = m.length[f] / A;
+            const R F1 = w.F[f];
+            const R G2 = own ? w.F[nf + f] : w.F[3 * nf + f];
+            const R G3 = own ? w.F[2 * nf + f] : w.F[4 * nf + f];
+            const R nxf = m.normal[f], nyf = m.normal[nf + f];
+            if (own) { s0 = s0 + (double)wgt * (double)F1; s1 = s1 + wgt * (G2 - p * nxf); s2 = s2 + wgt * (G3 - p * nyf); }
+            else { s0 = s0 - (double)wgt * (double)F1; s1 = s1 - wgt * (G2 - p * nxf); s2 = s2 - wgt * (G3 - p * nyf); }
+        }
+        rate[c] = -s0; rate[nc + c] = -(double)s1; rate[2 * nc + c] = -(double)s2;
+    }
+}
+
+// Boundary ledger of one stage: accepted(1,b) -= dt L fl(1,f) (:12058-12066),
+// summed by one lane per boundary in boundary face order.
+template <class R>
+__global__ void k_ledger(MeshView<R> m, Work<R> w, double *ledger_stage, double dt)
+{
+    const int b = blockIdx.x * blockDim.x + threadIdx.x;
+    if (b >= m.nb) return;
+    double s = 0;
+    for (int k = m.bnd_offset[b]; k < m.bnd_offset[b + 1]; ++k) {
+        const int f = m.bnd_faces[k];
+        s = s - dt * (double)m.length[f] * (double)w.F[f];
+    }
+    ledger_stage[b] = s;
+}
+
+}  // namespace braids

@@ -1,0 +1,61 @@
+// Device data layout of the hydraulic step (shared by the kernels and the
+// host API).  Cell arrays are component-major SoA; the incidence is CSR.
+#pragma once
+#include <stdint.h>
+
+namespace braids {
+
+enum : int { FLAG_ACTIVE = 1, FLAG_HCONST = 2 };
+enum : int { ADJ_OWNER = 1, ADJ_OPEN = 2, ADJ_OUTLET2 = 4 };
+// Failure encoding of Control::failure (max-reduced).  The reference returns
+// from euler at the first failing phase, so earlier phases take priority:
+// boundary traces (FAIL_TRACE + reason 6/8), then the boundary draining check
+// (reason 2, FAIL_DRAIN), then the gather (2 CFL, 3 negative depth, 5 non-finite).
+enum : int { FAIL_DRAIN = 50, FAIL_TRACE = 100 };
+
+template <class R> struct MeshView {
+    int nc, nf, nb, scheme;
+    R grav, dry, cfl_target;
+    // cells
+    const R *area, *manning, *ls;          // ls[4*nc]: a11, a12, a22, det
+    const int *offset, *irregular;
+    // incidence, nadj entries
+    const int *adj_face, *adj_other, *adj_flags;
+    const R *adj_wr;                      // [2*nadj] w*(x_n-x_c), w*(y_n-y_c)
+    const R *adj_d;                       // [2*nadj] mid - centroid(c)
+    int nadj;
+    // faces
+    const int *owner, *neighbour, *bid;
+    const R *normal, *length, *dl, *dr;   // dl/dr[2*nf] = mid - centroid(owner / neighbour)
+    // boundaries
+    const int *mode, *bnd_offset, *bnd_faces;
+    const double *forcing_q, *forcing_stage;
+    const int *series_offset;             // [nb+1]
+    const double *series_time, *series_q, *series_stage;
+};
+
+template <class R> struct StateView {
+    double *h;
+    R *hu, *hv;
+};
+
+template <class R> struct Work {
+    R *grad;       // [8*nc]
+    uint8_t *flags;// [nc]
+    R *trace;      // [3*nf]
+    R *F;          // [6*nf]: F1, FL2, FL3, FR2, FR3, speed
+    R *theta;      // [nc]
+    double *hydraulic; // [2*nb] current stage discharge / stage
+    R *target_u;   // [nf] mode-6 weights by boundary position
+};
+
+// Per-slot reductions; slots 0-2 are the SSPRK(3,3) stages, slot 3 a
+// single plain Euler stage.
+struct Control {
+    unsigned long long hmax_bits[4];   // max depth of the stage input (non-negative)
+    unsigned long long peak_bits[4];   // max CFL ratio over active cells
+    int failure[4];                    // encoded failure (max), see FAIL_*
+    int drain_any[4];                  // any draining cell
+};
+
+}  // namespace braids

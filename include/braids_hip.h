@@ -1,0 +1,134 @@
+/*
+ * libbraids_hip: MI355X (gfx950) hydraulic time step of BRAIDS-HKT.
+ *
+ * C ABI only: plain pointers, sizes and int return codes; no exceptions cross
+ * it and no torch/HIP C++ types appear in the signatures (a HIP stream is
+ * passed as void*).  Cell arrays are component-major SoA ([m*nc + c]), the
+ * layout of the reference ctypes bridge (qin(nc,3) in
+ * src/braids_core/sediment/bend_native_bridge.f90:38-45).  Indices are
+ * zero-based; neighbour = -1 marks the exterior, bid = -1 a wall.
+ *
+ * Return codes: BRAIDS_OK, BRAIDS_REJECTED (a trial the caller must retry
+ * with a smaller dt, like BendTrialRejected in bend_methods_v1.py:34),
+ * negative values for argument, HIP or allocation errors
+ * (braids_last_error() describes the last one).
+ */
+#ifndef BRAIDS_HIP_H
+#define BRAIDS_HIP_H
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define BRAIDS_OK 0
+#define BRAIDS_REJECTED 1
+#define BRAIDS_EINVAL (-1)
+#define BRAIDS_EHIP (-2)
+#define BRAIDS_ENOMEM (-3)
+
+/* Mesh in the cut-cell connectivity contract (CutCellMesh.incidence,
+ * src/braids_core/boundaries/cut_cells.py:157-182, and the deck arrays of
+ * load_case, src/solver/cut_cells/cutcell_solver.f90:4716-4790, 5093-5126).
+ * The CSR lists each cell's faces in ascending face order. */
+typedef struct braids_mesh {
+    int nc, nf, nb, nadj;
+    const double *area;            /* [nc] */
+    const double *xy;              /* [2*nc] centroids */
+    const double *manning;         /* [nc] Manning n */
+    const int *irregular;          /* [nc] 1 unless an axis-aligned rectangle */
+    const int *owner, *neighbour, *bid;  /* [nf] */
+    const double *normal;          /* [2*nf] unit, out of owner */
+    const double *length;          /* [nf] */
+    const double *mid;             /* [2*nf] */
+    const int *offset;             /* [nc+1] */
+    const int *faces;              /* [nadj] */
+    const int *signs;              /* [nadj] +1 owner side, -1 neighbour side */
+    const int *mode;               /* [nb] 6 total-discharge inlet, 2 stage outlet */
+    const int *boundary_offset;    /* [nb+1] */
+    const int *boundary_faces;     /* ascending face order per boundary */
+    const double *forcing_q;       /* [nb] */
+    const double *forcing_stage;   /* [nb] */
+    const int *series_offset;      /* [nb+1] hydrograph knots or NULL */
+    const double *series_time, *series_q, *series_stage;
+} braids_mesh;
+
+typedef struct braids_options {
+    int precision;      /* 64: FP64 throughout; 32: FP32 flux/update, FP64 depth/bed/ledgers */
+    int scheme;         /* 1 first order, 5 LS-MUSCL (deck scheme, cutcell_solver.f90:4470) */
+    double grav, dry, cfl_target, dtmax;
+    int block;          /* threads per workgroup (multiple of 64), 0 = default 256 */
+    int remap;          /* 1: XCD-aware workgroup order */
+} braids_options;
+
+typedef struct braids_step_result {
+    int accepted;       /* 1 if the SSPRK(3,3) step was committed */
+    int failure;        /* first failing stage's code (0, 2, 3, 5, 6, 8) */
+    int failed_stage;   /* -1 or 0..2 */
+    int pad;
+    double trial_cfl_peak;  /* max CFL ratio of the stages that ran */
+    double stage_peak[3];
+} braids_step_result;
+
+typedef struct braids_solver braids_solver;
+
+const char *braids_last_error(void);
+
+/* Build a solver: uploads the mesh (and derived least-squares geometry) and
+ * bed z [nc] to the current HIP device.  No reference counterpart: the
+ * reference binds one deck per Fortran library instance instead
+ * (bend_native_load, bend_native_bridge.f90:3-22). */
+int braids_create(const braids_mesh *mesh, const braids_options *opt, const double *z,
+                  braids_solver **out);
+int braids_destroy(braids_solver *s);
+/* Stream for every later launch of this solver (hipStream_t as void*; NULL = default). */
+int braids_set_stream(braids_solver *s, void *stream);
+/* Host state q[3*nc] (h, hu, hv).  prepare != 0 applies prepare_hydraulic_stage. */
+int braids_set_state(braids_solver *s, const double *q, int prepare);
+int braids_get_state(braids_solver *s, double *q);
+int braids_set_bed(braids_solver *s, const double *z);
+/* Accepted inward boundary water ledger [nb] (accepted(1,:), cutcell_solver.f90:12058-12066). */
+int braids_get_ledger(braids_solver *s, double *ledger);
+int braids_reset_ledger(braids_solver *s);
+
+/* One SSPRK(3,3) trial of size dt at time t (advance_ssp33,
+ * cutcell_solver.f90:15858-15986); commits on success. */
+int braids_step(braids_solver *s, double dt, double t, braids_step_result *res);
+/* Accepted-step loop (cutcell_solver.f90:17287-17370): up to nsteps accepted
+ * steps or until tend; retries with the reference's dt reduction rules. */
+int braids_advance(braids_solver *s, int nsteps, double tend, double *time, double *step_hint,
+                   int *accepted, int *retries);
+
+/* Replaces shoreline_native_euler (shoreline_native_bridge.inc:3-30): one
+ * explicit Euler stage with draining and CFL rejection.  Device pointers:
+ * q[3*nc], z[nc] (may be NULL to keep the solver's bed), qout[3*nc], water[nf].
+ * *code = 0 ok, 1 rejected. */
+int braids_native_euler(braids_solver *s, const double *q, const double *z, double dt, double t,
+                        double *qout, double *water, int *code);
+/* Replaces bend_native_rates (bend_native_bridge.f90:38-68): spatial rate
+ * without draining or CFL.  Device pointers rate[3*nc], water[nf]. */
+int braids_native_rates(braids_solver *s, const double *q, const double *z, double dt, double t,
+                        double *rate, double *water, int *code);
+/* Replaces river_get_face_speed (shoreline_solver.py:53-57): face wave
+ * speeds of the last spatial evaluation; device pointer speed[nf]. */
+int river_get_face_speed(braids_solver *s, double *speed);
+
+/* Mass inventory sum(A h) of the resident state (compensated FP64 sum). */
+int braids_mass(braids_solver *s, double *mass);
+
+/* Measurement hooks (no reference counterpart; the reference only has the
+ * BRAIDS_PROFILE wall-clock phases, cutcell_solver.f90:16584-16646).
+ * braids_profile(s, 1) brackets every kernel launch of later steps with HIP
+ * events on the solver's stream; braids_profile_read returns accumulated
+ * milliseconds and launch counts for the kernel kinds
+ * {prep, traces, face, drain, gather, ledger}. */
+int braids_profile(braids_solver *s, int enable);
+int braids_profile_read(braids_solver *s, double *ms, long long *launches);
+/* Plain device copy of `bytes` (multiple of 16) repeated `repeats` times on
+ * `stream`; reports the mean milliseconds per copy (STREAM-copy bandwidth and
+ * the byte-counter calibration of the profiler). */
+int braids_stream_copy(const void *src, void *dst, long long bytes, void *stream, int repeats,
+                       double *ms_per_copy);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,79 @@
+"""Write the golden fixtures of the hydraulic step from the reference solver
+built from its own sources (oracle/_ref, compiled by `make -C oracle ref`).
+
+Run in a container that holds /root/reference:
+    python tests/golden/make_golden.py
+Outputs (committed; data only):
+    wave_flux_ref.npz      wave_flux inputs/outputs over wet, dry, wall,
+                           supercritical and bed-step states
+    trajectories_ref.npz   accepted states of xbraids_cutcell runs on decks
+                           written for small meshes (schemes 1 and 5)
+"""
+import sys
+from pathlib import Path
+
+import numpy as np
+
+ROOT = Path(__file__).resolve().parents[2]
+sys.path.insert(0, str(ROOT))
+sys.path.insert(0, str(ROOT / 'braids-hkt_amd'))
+
+from oracle import swe, deck  # noqa: E402
+from braids_hip import mesh as M  # noqa: E402
+
+OUT = Path(__file__).resolve().parent
+
+
+def wave_flux_cases(n=1200, seed=11):
+    rng = np.random.default_rng(seed)
+    rows = []
+    for i in range(n):
+        kind = i % 10
+        hl, hr = rng.uniform(0, 5, 2)
+        ul, ur = rng.normal(0, 3, 2)
+        vl, vr = rng.normal(0, 1, 2)
+        bl = rng.normal(0, 1); br = bl + rng.normal(0, 0.5)
+        if kind == 1: hr = 0.0
+        if kind == 2: hl = 0.0
+        if kind == 3: hl = rng.uniform(0, 2e-8)
+        if kind == 4: hl = hr = 0.0
+        if kind == 5: ul, ur = rng.normal(0, 20, 2)          # supercritical
+        if kind == 6: br = bl                                # flat bed
+        if kind == 7: ul, ur = 0.0, 0.0; br = bl + (hl - hr)  # still water over a step
+        if kind == 8: hr = 0.0; br = bl + rng.uniform(0, 3)   # dry step: wall mirroring
+        if kind == 9: ur = -ul; hr = hl; vr = vl             # symmetric collision
+        th = rng.uniform(0, 2 * np.pi)
+        rows.append([hl, hl * ul, hl * vl, hr, hr * ur, hr * vr, bl, br, np.cos(th), np.sin(th)])
+    return np.array(rows)
+
+
+def main():
+    ref = swe.Reference()
+    inputs = wave_flux_cases()
+    outputs = []
+    for r in inputs:
+        fl, fr, s, rm = ref.wave_flux(r[0:3], r[3:6], r[6], r[7], r[8:10])
+        outputs.append(np.concatenate([fl, fr, [s, rm]]))
+    np.savez_compressed(OUT / 'wave_flux_ref.npz', inputs=inputs, outputs=np.array(outputs))
+
+    runs = {}
+    cases = [
+        ('channel_s1', dict(nx=50, ny=6, length=200.0, width=24.0), 1, 2.0),
+        ('channel_s5', dict(nx=50, ny=6, length=200.0, width=24.0), 5, 2.0),
+        ('braided_s5', dict(nx=60, ny=12, length=300.0, width=60.0, amplitude=8.0, wavelength=150.0,
+                            nbars=12, q=30.0, depth=1.0, bar_height=(0.3, 1.0), bar_sigma=(3.0, 10.0),
+                            bank=2.0), 5, 3.0),
+    ]
+    tmp = ROOT / '_work' / 'golden_runs'
+    for name, kw, scheme, tend in cases:
+        m = M.braided_reach(**kw) if name.startswith('braided') else M.plain_channel(**kw)
+        frames, _ = deck.run_reference(m, tmp / name, scheme=scheme, endtime=tend, output_dt=tend, dtmax=1.0)
+        t, steps, q, led = frames[-1]
+        runs[name] = dict(q=q[:3], ledger=led[0], time=t, steps=steps)
+    np.savez_compressed(OUT / 'trajectories_ref.npz',
+                        **{f'{k}_{f}': np.asarray(v[f]) for k, v in runs.items() for f in ('q', 'ledger', 'time', 'steps')})
+    print('wrote', OUT / 'wave_flux_ref.npz', OUT / 'trajectories_ref.npz')
+
+
+if __name__ == '__main__':
+    main()

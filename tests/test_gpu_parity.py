@@ -1,0 +1,147 @@
+"""GPU parity: libbraids_hip (HIP, gfx950) against the oracle on the same
+seeded inputs.  The oracle itself reproduces the reference executable bit for
+bit (tests/test_oracle.py), so these are parity tests against the reference.
+
+Tolerances (per field h, hu, hv; relative L2 over cells):
+  FP64 mode  <= 1e-10   (only libm pow/ulp differences remain)
+  FP32 mode  <= 1e-6    against the FP64 oracle (FP32 flux/update, FP64 depth,
+                         bed and free-surface differencing)
+Mass: |sum(A h) - sum(A h0) - boundary ledger| / sum(A h0) <= 1e-12 in both modes.
+"""
+import numpy as np
+import pytest
+
+from braids_hip import mesh as M
+
+pytestmark = pytest.mark.gpu
+
+
+def rel_l2(a, b):
+    out = []
+    for j in range(3):
+        den = np.linalg.norm(b[j])
+        out.append(np.linalg.norm(a[j] - b[j]) / (den if den > 0 else 1.0))
+    return np.array(out)
+
+
+def small_channel():
+    return M.plain_channel(nx=50, ny=6, length=200.0, width=24.0)
+
+
+def small_braided():
+    return M.braided_reach(nx=60, ny=12, length=300.0, width=60.0, amplitude=8.0, wavelength=150.0,
+                           nbars=12, q=30.0, depth=1.0, bar_height=(0.3, 1.0), bar_sigma=(3.0, 10.0), bank=2.0)
+
+
+CASES = {
+    'channel_s1': (small_channel, 1, 4.0),
+    'channel_s5': (small_channel, 5, 4.0),
+    'braided_s5': (small_braided, 5, 6.0),
+}
+
+
+def run_both(mesh, scheme, tend, precision):
+    from oracle import swe
+    from braids_hip.solver import HydraulicSolver
+    print(f'[parity] {mesh.name} nc={mesh.nc} scheme={scheme} precision={precision} tend={tend}', flush=True)
+    o = swe.Oracle(mesh, scheme=scheme, cfl_target=0.9, dtmax=1.0)
+    qo, to, hint, no, ro, lo = o.advance(mesh.q0, mesh.z, 10 ** 9, tend=tend)
+    s = HydraulicSolver(mesh, precision=precision, scheme=scheme, cfl_target=0.9, dtmax=1.0)
+    s.set_state(mesh.q0)
+    ng, rg = s.advance(10 ** 9, tend=tend)
+    qg = s.state()
+    return dict(qo=qo, qg=qg, no=no, ng=ng, to=to, tg=s.time, lo=lo, lg=s.ledger(),
+                mass0=o.mass(mesh.q0), mass_o=o.mass(qo), mass_g=s.mass(), solver=s)
+
+
+@pytest.mark.parametrize('case', list(CASES))
+def test_fp64_trajectory_matches_oracle(case):
+    make, scheme, tend = CASES[case]
+    r = run_both(make(), scheme, tend, 64)
+    err = rel_l2(r['qg'], r['qo'])
+    print(f'[parity] {case} fp64 steps gpu={r["ng"]} oracle={r["no"]} rel_l2={err}', flush=True)
+    assert r['ng'] == r['no']
+    assert abs(r['tg'] - r['to']) <= 1e-12 * r['to']
+    assert (err <= 1e-10).all(), err
+    assert np.allclose(r['lg'], r['lo'], rtol=1e-10, atol=1e-10)
+
+
+@pytest.mark.parametrize('case', list(CASES))
+def test_fp32_trajectory_within_stated_error(case):
+    make, scheme, tend = CASES[case]
+    r = run_both(make(), scheme, tend, 32)
+    err = rel_l2(r['qg'], r['qo'])
+    print(f'[parity] {case} fp32 steps gpu={r["ng"]} oracle={r["no"]} rel_l2={err}', flush=True)
+    assert (err <= 1e-6).all(), err
+
+
+@pytest.mark.parametrize('precision', [64, 32])
+def test_mass_conservation(precision):
+    make, scheme, tend = CASES['braided_s5']
+    r = run_both(make(), scheme, tend, precision)
+    drift = abs(r['mass_g'] - r['mass0'] - r['lg'].sum()) / r['mass0']
+    print(f'[parity] mass drift precision={precision}: {drift:.3e}', flush=True)
+    assert drift <= 1e-12
+    assert r['qg'][0].min() >= 0.0
+
+
+@pytest.mark.parametrize('precision,tol', [(64, 1e-10), (32, 1e-5)])
+def test_lake_at_rest(precision, tol):
+    from braids_hip.solver import HydraulicSolver
+    m = M.lake_at_rest()
+    s = HydraulicSolver(m, precision=precision, scheme=5)
+    s.set_state(m.q0)
+    n, _ = s.advance(40)
+    q = s.state()
+    print(f'[parity] lake at rest precision={precision} max change {np.abs(q - m.q0).max():.3e}', flush=True)
+    assert n == 40
+    assert np.abs(q - m.q0).max() < tol
+
+
+def test_renumbered_mesh_maps_back_to_oracle():
+    """Hilbert renumbering is a pure permutation: the GPU run on the
+    renumbered mesh, scattered back, matches the oracle on the original."""
+    from oracle import swe
+    from braids_hip.solver import HydraulicSolver
+    m = small_braided()
+    r = m.renumbered('hilbert')
+    o = swe.Oracle(m, scheme=5)
+    qo, *_ = o.advance(m.q0, m.z, 10 ** 9, tend=3.0)
+    s = HydraulicSolver(r, precision=64, scheme=5)
+    s.set_state(r.q0)
+    s.advance(10 ** 9, tend=3.0)
+    qg = r.to_original_cells(s.state())
+    err = rel_l2(qg, qo)
+    print(f'[parity] renumbered rel_l2={err}', flush=True)
+    assert (err <= 1e-10).all()
+
+
+def test_native_bridge_matches_oracle():
+    """braids_native_euler / braids_native_rates / river_get_face_speed replace
+    shoreline_native_euler, bend_native_rates and river_get_face_speed."""
+    import torch
+    from oracle import swe
+    from braids_hip.solver import HydraulicSolver
+    m = small_braided()
+    o = swe.Oracle(m, scheme=5)
+    q, *_ = o.advance(m.q0, m.z, 30)           # a developed state with wet/dry fronts
+    dt = 0.05
+    ref = o.euler(q, m.z, dt, 1.0)
+    ok_r, rate_o, water_r = o.rates(q, m.z, dt, 1.0)
+    s = HydraulicSolver(m, precision=64, scheme=5)
+    dev = torch.device('cuda:0')
+    qd = torch.tensor(q, dtype=torch.float64, device=dev).contiguous()
+    zd = torch.tensor(m.z, dtype=torch.float64, device=dev)
+    qout = torch.empty_like(qd); water = torch.empty(m.nf, dtype=torch.float64, device=dev)
+    code = s.native_euler(qd.data_ptr(), zd.data_ptr(), dt, 1.0, qout.data_ptr(), water.data_ptr())
+    assert code == (0 if ref['ok'] else 1)
+    assert (rel_l2(qout.cpu().numpy(), ref['next']) <= 1e-12).all()
+    assert np.allclose(water.cpu().numpy(), ref['water'], rtol=1e-12, atol=1e-14)
+    speed = torch.empty(m.nf, dtype=torch.float64, device=dev)
+    s.face_speed(speed.data_ptr())
+    assert np.allclose(speed.cpu().numpy(), ref['speed'], rtol=1e-12, atol=1e-14)
+    rate = torch.empty_like(qd)
+    code = s.native_rates(qd.data_ptr(), zd.data_ptr(), dt, 1.0, rate.data_ptr(), water.data_ptr())
+    assert code == 0 and ok_r
+    assert np.allclose(rate.cpu().numpy(), rate_o, rtol=1e-11, atol=1e-12)
+    assert np.allclose(water.cpu().numpy(), water_r, rtol=1e-12, atol=1e-14)
