@@ -72,6 +72,13 @@ def load():
     """Load libbraids_hip.so (raises if it has not been built)."""
     global _lib
     if _lib is None:
+        # PyTorch-ROCm ships its own HIP runtime with the same soname as the
+        # ROCm one this library links: load torch's first so that both bind
+        # to one runtime instance in the process.
+        try:
+            import torch  # noqa: F401
+        except ImportError:
+            pass
         if not LIB_PATH.exists():
             raise ImportError(f'{LIB_PATH} is missing: build it with braids_hip.build.build() '
                               '(the HIP library is required; there is no CPU path)')
