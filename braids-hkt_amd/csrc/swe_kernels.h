@@ -536,6 +536,116 @@ __global__ void __launch_bounds__(256) k_face(MeshView<R> m, StateView<R> q, con
     w.F[3 * nf + f] = fr[1]; w.F[4 * nf + f] = fr[2]; w.F[5 * nf + f] = speed;
 }
 
+// FP32 face kernel.  Same face classification as k_face, but the stored
+// momentum fluxes are net of the adjacent cell-centre hydrostatic pressure
+// (G = F - 1/2 g h_c^2 n), and every pressure term is summed in FP64:
+//   G = F_dyn + [1/2 g (h_f^2 - h_c^2) + well-balanced correction] n,
+// which for the wet-column correction equals 1/2 g (h_c + h_f)(eta_f - eta_c),
+// a free-surface difference.  A lake at rest therefore stays at rest in FP32.
+template <int SCHEME>
+__global__ void __launch_bounds__(256) k_face_f32(MeshView<float> m, StateView<float> q, const double *__restrict__ z,
+                                                  Work<float> w, int remap)
+{
+    typedef float R;
+    const int nf = m.nf;
+    const int f = logical_block(remap) * blockDim.x + threadIdx.x;
+    if (f >= nf) return;
+    const R g = m.grav, dry = m.dry;
+    const double gg = (double)g, dd = (double)dry;
+    const int c = m.owner[f], n = m.neighbour[f], b = m.bid[f];
+    const R nx = m.normal[f], ny = m.normal[nf + f];
+    const double nxd = nx, nyd = ny;
+    const double hc = q.h[c];
+    if (b < 0 && hc <= dd) {
+        if (n < 0 || q.h[n] <= dd) {
+            for (int j = 0; j < 6; ++j) w.F[j * nf + f] = 0.f;
+            return;
+        }
+    }
+    double etaL, zL; R uL, vL;
+    face_value<R, SCHEME>(m, q, z, w, c, m.dl[f], m.dl[nf + f], etaL, uL, vL, zL);
+    const double rawL = vmax(etaL - zL, 0.0);
+    double depthL = rawL;
+    if (hc <= dd) depthL = 0;
+    if ((w.flags[c] & FLAG_HCONST) && depthL <= dd) depthL = 0;
+    R l[3] = {(R)depthL, (R)depthL * uL, (R)depthL * vL};
+    R fl[3] = {0.f, 0.f, 0.f}, fr[3] = {0.f, 0.f, 0.f}, speed = 0.f;
+    // pressure brackets (FP64), added along n to each side
+    double PL = 0.0, PR = 0.0;
+    auto wb_bracket = [&](double hcell, double dface, double raw, double zf, double zcell, bool hconst) {
+        // 1/2 g (h_f^2 - h_c^2) + correction of euler :11063-11084
+        if (hcell > dd) {
+            if (hconst) return 0.0;                                     // 1/2 g (h_c^2 - h_f^2) cancels it
+            return .5 * gg * (dface * dface - hcell * hcell) + .5 * gg * (hcell + raw) * (zf - zcell);
+        }
+        return .5 * gg * (dface * dface - hcell * hcell);
+    };
+    if (n >= 0) {
+        const double hn = q.h[n];
+        double etaR, zR; R uR, vR;
+        face_value<R, SCHEME>(m, q, z, w, n, m.dr[f], m.dr[nf + f], etaR, uR, vR, zR);
+        const double rawR = vmax(etaR - zR, 0.0);
+        double depthR = rawR;
+        if (hn <= dd) depthR = 0;
+        if ((w.flags[n] & FLAG_HCONST) && depthR <= dd) depthR = 0;
+        R r[3] = {(R)depthR, (R)depthR * uR, (R)depthR * vR};
+        bool closed = false;
+        if (vmin(hc, hn) <= dd) {
+            const double ofd = fabs((double)m.dl[f] * nxd + (double)m.dl[nf + f] * nyd);
+            const double nfd = fabs((double)m.dr[f] * nxd + (double)m.dr[nf + f] * nyd);
+            double shared;
+            if (ofd + nfd > 2.2250738585072014e-308)
+                shared = (nfd * (z[c] + hc) + ofd * (z[n] + hn)) / (ofd + nfd) - vmax(z[c], z[n]);
+            else
+                shared = .5 * ((z[c] + hc) + (z[n] + hn)) - vmax(z[c], z[n]);
+            closed = shared <= dd;
+            if (closed) closed = !runup_opens_face<R>(l, r, zL, zR, nx, ny, g, dry);
+        }
+        if (closed) {
+            if (l[0] > dry) speed = vmax(speed, fabsf((l[1] * nx + l[2] * ny) / l[0]) + __builtin_sqrtf(g * l[0]));
+            if (r[0] > dry) speed = vmax(speed, fabsf((r[1] * nx + r[2] * ny) / r[0]) + __builtin_sqrtf(g * r[0]));
+        } else {
+            wave_flux_f32(l, r, depthL, depthR, zL, zR, nx, ny, g, dry, fl, fr, speed);
+        }
+        PL = wb_bracket(hc, depthL, rawL, zL, z[c], (w.flags[c] & FLAG_HCONST) != 0);
+        PR = wb_bracket(hn, depthR, rawR, zR, z[n], (w.flags[n] & FLAG_HCONST) != 0);
+    } else if (b < 0) {
+        R d = l[1] * nx + l[2] * ny;
+        R r[3] = {l[0], l[1] - 2.f * d * nx, l[2] - 2.f * d * ny};
+        wave_flux_f32(l, r, depthL, depthL, zL, zL, nx, ny, g, dry, fl, fr, speed);
+        fl[0] = 0.f;
+        R p = fl[1] * nx + fl[2] * ny;
+        fl[1] = p * nx; fl[2] = p * ny;
+        fr[0] = fl[0]; fr[1] = fl[1]; fr[2] = fl[2];
+        PL = wb_bracket(hc, depthL, rawL, zL, z[c], (w.flags[c] & FLAG_HCONST) != 0);
+        PR = PL;
+    } else {
+        const R t0 = w.trace[f], t1 = w.trace[nf + f], t2 = w.trace[2 * nf + f];
+        const R depth = t0;
+        fl[0] = t1 * nx + t2 * ny;
+        if (depth > dry) { fl[1] = fl[0] * t1 / depth; fl[2] = fl[0] * t2 / depth; }
+        fr[0] = fl[0]; fr[1] = fl[1]; fr[2] = fl[2];
+        speed = __builtin_sqrtf(g * depth);
+        if (depth > dry) speed = speed + fabsf(fl[0]) / depth;
+        const double dt0 = (double)depth;
+        PL = depth > dry ? .5 * gg * (dt0 * dt0 - hc * hc) : -.5 * gg * hc * hc;
+        if (m.mode[b] == 2 && hc > dd) {
+            const double qn = (double)q.hu[c] * nxd + (double)q.hv[c] * nyd;
+            if (fabs(zL - z[c]) > 64 * 2.2204460492503131e-16 * vmax(1.0, fabs(z[c])) && fabs(qn) < hc * sqrt(gg * hc)) {
+                const double d = rawL;
+                PL += (w.flags[c] & FLAG_HCONST) ? .5 * gg * (hc * hc - d * d) : .5 * gg * (hc + d) * (zL - z[c]);
+            }
+        }
+        PR = PL;
+    }
+    w.F[f] = fl[0];
+    w.F[nf + f] = (R)((double)fl[1] + PL * nxd);
+    w.F[2 * nf + f] = (R)((double)fl[2] + PL * nyd);
+    w.F[3 * nf + f] = (R)((double)fr[1] + PR * nxd);
+    w.F[4 * nf + f] = (R)((double)fr[2] + PR * nyd);
+    w.F[5 * nf + f] = speed;
+}
+
 // ---------------------------------------------------------------------------
 template <class R>
 __global__ void __launch_bounds__(256) k_drain(MeshView<R> m, StateView<R> q, Work<R> w, Control *ctl,
@@ -623,17 +733,22 @@ __global__ void __launch_bounds__(256) k_gather(MeshView<R> m, StateView<R> in, 
                 }
                 const R wgt = L / A;
                 cfl = cfl + L * S;
-                const R nxf = m.normal[f], nyf = m.normal[nf + f];
-                if (own) {
-                    if (sizeof(R) == 8) s0 = s0 + (double)wgt * (double)F1;
-                    else s0 = s0 + ((double)L / (double)A) * (double)F1;
-                    s1 = s1 + wgt * (G2 - p * nxf);
-                    s2 = s2 + wgt * (G3 - p * nyf);
+                if (sizeof(R) == 8) {
+                    const R nxf = m.normal[f], nyf = m.normal[nf + f];
+                    if (own) {
+                        s0 = s0 + (double)wgt * (double)F1;
+                        s1 = s1 + wgt * (G2 - p * nxf);
+                        s2 = s2 + wgt * (G3 - p * nyf);
+                    } else {
+                        s0 = s0 - (double)wgt * (double)F1;
+                        s1 = s1 - wgt * (G2 - p * nxf);
+                        s2 = s2 - wgt * (G3 - p * nyf);
+                    }
                 } else {
-                    if (sizeof(R) == 8) s0 = s0 - (double)wgt * (double)F1;
-                    else s0 = s0 - ((double)L / (double)A) * (double)F1;
-                    s1 = s1 - wgt * (G2 - p * nxf);
-                    s2 = s2 - wgt * (G3 - p * nyf);
+                    // FP32 face fluxes are stored net of this cell's pressure
+                    const double wd = (double)L / (double)A;
+                    if (own) { s0 = s0 + wd * (double)F1; s1 = s1 + wgt * G2; s2 = s2 + wgt * G3; }
+                    else { s0 = s0 - wd * (double)F1; s1 = s1 - wgt * G2; s2 = s2 - wgt * G3; }
                 }
             }
             ratio = (double)(R(dt) * cfl / A);
@@ -710,8 +825,9 @@ __global__ void __launch_bounds__(256) k_rates(MeshView<R> m, StateView<R> in, W
             const R G2 = own ? w.F[nf + f] : w.F[3 * nf + f];
             const R G3 = own ? w.F[2 * nf + f] : w.F[4 * nf + f];
             const R nxf = m.normal[f], nyf = m.normal[nf + f];
-            if (own) { s0 = s0 + (double)wgt * (double)F1; s1 = s1 + wgt * (G2 - p * nxf); s2 = s2 + wgt * (G3 - p * nyf); }
-            else { s0 = s0 - (double)wgt * (double)F1; s1 = s1 - wgt * (G2 - p * nxf); s2 = s2 - wgt * (G3 - p * nyf); }
+            const R pp = sizeof(R) == 8 ? p : R(0);      // FP32 fluxes are net of the cell pressure
+            if (own) { s0 = s0 + (double)wgt * (double)F1; s1 = s1 + wgt * (G2 - pp * nxf); s2 = s2 + wgt * (G3 - pp * nyf); }
+            else { s0 = s0 - (double)wgt * (double)F1; s1 = s1 - wgt * (G2 - pp * nxf); s2 = s2 - wgt * (G3 - pp * nyf); }
         }
         rate[c] = -s0; rate[nc + c] = -(double)s1; rate[2 * nc + c] = -(double)s2;
     }

@@ -33,6 +33,17 @@ static void launch_prep(const MeshView<R> &m, StateView<R> in, const double *z, 
 }
 
 template <class R, int SCHEME>
+static void launch_face(const MeshView<R> &m, StateView<R> in, const double *z, Work<R> w,
+                        const LaunchCfg &cfg, hipStream_t s)
+{
+    const int B = cfg.block;
+    if constexpr (std::is_same<R, float>::value)
+        hipLaunchKernelGGL((k_face_f32<SCHEME>), dim3(grid_for(m.nf, B)), dim3(B), 0, s, m, in, z, w, cfg.remap);
+    else
+        hipLaunchKernelGGL((k_face<R, SCHEME>), dim3(grid_for(m.nf, B)), dim3(B), 0, s, m, in, z, w, cfg.remap);
+}
+
+template <class R, int SCHEME>
 static void stage_kernels(const MeshView<R> &m, StateView<R> in, StateView<R> base, StateView<R> out,
                           const double *z, Work<R> w, Control *ctl, double *ledger_stage, int mode,
                           double dt, double t, const LaunchCfg &cfg, hipStream_t s)
@@ -43,8 +54,7 @@ static void stage_kernels(const MeshView<R> &m, StateView<R> in, StateView<R> ba
         EvScope e(cfg, s, mode, KK_TRACES);
         hipLaunchKernelGGL((k_traces<R, SCHEME>), dim3(m.nb), dim3(256), 0, s, m, in, z, w, ctl, mode, t);
     }
-    { EvScope e(cfg, s, mode, KK_FACE);
-      hipLaunchKernelGGL((k_face<R, SCHEME>), dim3(grid_for(m.nf, B)), dim3(B), 0, s, m, in, z, w, cfg.remap); }
+    { EvScope e(cfg, s, mode, KK_FACE); launch_face<R, SCHEME>(m, in, z, w, cfg, s); }
     { EvScope e(cfg, s, mode, KK_DRAIN);
       hipLaunchKernelGGL((k_drain<R>), dim3(grid_for(m.nc, B)), dim3(B), 0, s, m, in, w, ctl, mode, dt, cfg.remap); }
     {
@@ -79,7 +89,7 @@ static void rate_kernels(const MeshView<R> &m, StateView<R> in, const double *z,
     launch_prep<R, SCHEME>(m, in, z, w, ctl, 3, cfg, s);
     if (m.nb > 0)
         hipLaunchKernelGGL((k_traces<R, SCHEME>), dim3(m.nb), dim3(256), 0, s, m, in, z, w, ctl, 3, t);
-    hipLaunchKernelGGL((k_face<R, SCHEME>), dim3(grid_for(m.nf, B)), dim3(B), 0, s, m, in, z, w, cfg.remap);
+    launch_face<R, SCHEME>(m, in, z, w, cfg, s);
     const int n = m.nc > m.nf ? m.nc : m.nf;
     hipLaunchKernelGGL((k_rates<R>), dim3(grid_for(n, B)), dim3(B), 0, s, m, in, w, rate, water, 0);
 }

@@ -255,3 +255,153 @@ __device__ __forceinline__ bool runup_opens_face(const R l[3], const R r[3], dou
 }
 
 }  // namespace braids
+
+namespace braids {
+
+// FP32 form of wave_flux that keeps a lake at rest exactly balanced.
+// Differences from the FP64 form (same waves otherwise):
+//  * face depths dl/dr and bed elevations arrive in FP64; after the dry-step
+//    wall tests the free-surface jump deta = (hR + bR) - (hL + bL) is formed
+//    in FP64, and the first and third components of the decomposed jump are
+//    written as deta-based terms plus a remainder that is exactly zero when
+//    the velocities vanish:
+//      del1 = deta - (deldelh + delb),  del3 = dyn + g hbar deta - (g hbar delb + deldelphi);
+//  * the physical fluxes returned exclude the face hydrostatic pressure
+//    1/2 g h^2; the caller adds every pressure term in FP64.
+__device__ __forceinline__ void wave_flux_f32(const float l_in[3], const float r_in[3], double dl, double dr,
+                                              double bl, double br, float nx, float ny, float g, float drytol,
+                                              float fl[3], float fr[3], float &speed)
+{
+    typedef float R;
+    const R tx = -ny, ty = nx;
+    R lq[3] = {l_in[0], l_in[1] * nx + l_in[2] * ny, l_in[1] * tx + l_in[2] * ty};
+    R rq[3] = {r_in[0], r_in[1] * nx + r_in[2] * ny, r_in[1] * tx + r_in[2] * ty};
+    R s[3] = {0.f, 0.f, 0.f};
+    R fwave[3][3] = {{0.f, 0.f, 0.f}, {0.f, 0.f, 0.f}, {0.f, 0.f, 0.f}};
+    if (!(lq[0] <= drytol && rq[0] <= drytol)) {
+        R hL = lq[0], hR = rq[0], huL = lq[1], huR = rq[1], hvL = lq[2], hvR = rq[2];
+        double hLd = dl, hRd = dr, bL = bl, bR = br;
+        R uL, uR, vL, vR;
+        R wall0 = 1.f, wall1 = 1.f, wall2 = 1.f;
+        if (hR > drytol) { uR = huR / hR; vR = hvR / hR; }
+        else { hR = 0.f; hRd = 0.0; huR = 0.f; hvR = 0.f; uR = 0.f; vR = 0.f; }
+        if (hL > drytol) { uL = huL / hL; vL = hvL / hL; }
+        else { hL = 0.f; hLd = 0.0; huL = 0.f; hvL = 0.f; uL = 0.f; vL = 0.f; }
+        if (hR <= drytol) {
+            RType<R> t = riemanntype<R>(hL, hL, uL, -uL, g, drytol);
+            R hstartest = vmax(hL, t.hm);
+            if ((double)hstartest + bL < bR) {
+                wall1 = 0.f; wall2 = 0.f;
+                hR = hL; hRd = hLd; huR = -huL; bR = bL; uR = -uL; vR = vL;
+            } else if (hLd + bL < bR) {
+                bR = hLd + bL;
+            }
+        } else if (hL <= drytol) {
+            RType<R> t = riemanntype<R>(hR, hR, -uR, uR, g, drytol);
+            R hstartest = vmax(hR, t.hm);
+            if ((double)hstartest + bR < bL) {
+                wall0 = 0.f; wall1 = 0.f;
+                hL = hR; hLd = hRd; huL = -huR; bL = bR; uL = -uR; vL = vR;
+            } else if (hRd + bR < bL) {
+                bL = hRd + bR;
+            }
+        }
+        const R deta = (R)((hRd + bR) - (hLd + bL));
+        const R delb = (R)(bR - bL);
+        R sL = uL - __builtin_sqrtf(g * hL);
+        R sR = uR + __builtin_sqrtf(g * hR);
+        R uhat = (__builtin_sqrtf(g * hL) * uL + __builtin_sqrtf(g * hR) * uR) / (__builtin_sqrtf(g * hR) + __builtin_sqrtf(g * hL));
+        R chat = __builtin_sqrtf(g * 0.5f * (hR + hL));
+        R sE1 = vmin(sL, uhat - chat), sE2 = vmax(sR, uhat + chat);
+        R delh = hR - hL, delhu = huR - huL;
+        R dynphi = huR * uR - huL * uL;                 // (hu^2/h)_R - (hu^2/h)_L
+        RType<R> rt = riemanntype<R>(hL, hR, uL, uR, g, drytol);
+        R lam0 = vmin(sE1, rt.s2m), lam2 = vmax(sE2, rt.s1m);
+        sE1 = lam0; sE2 = lam2;
+        R hstarHLL = vmax((huL - huR + sE2 * hR - sE1 * hL) / (sE2 - sE1), 0.f);
+        R lam1 = 0.5f * (lam0 + lam2);
+        R criticaltol = vmax(drytol * g, 1e-6f);
+        R criticaltol_2 = __builtin_sqrtf(criticaltol);
+        R hbar = vmax(0.5f * (hL + hR), 0.f);
+        R su = uL + uR;
+        R s1s2bar = 0.25f * (su * su) - g * hbar;
+        R s1s2tilde = vmax(0.f, uL * uR) - g * hbar;
+        bool sonic = false;
+        if (fabsf(s1s2bar) <= criticaltol) sonic = true;
+        else if (s1s2bar * s1s2tilde <= criticaltol * criticaltol) sonic = true;
+        else if (s1s2bar * sE1 * sE2 <= criticaltol * criticaltol) sonic = true;
+        else if (vmin(fabsf(sE1), fabsf(sE2)) < criticaltol_2) sonic = true;
+        else if (sE1 < criticaltol_2 && rt.s1m > -criticaltol_2) sonic = true;
+        else if (sE2 > -criticaltol_2 && rt.s2m < criticaltol_2) sonic = true;
+        else if ((uL + __builtin_sqrtf(g * hL)) * (uR + __builtin_sqrtf(g * hR)) < 0.f) sonic = true;
+        else if ((uL - __builtin_sqrtf(g * hL)) * (uR - __builtin_sqrtf(g * hR)) < 0.f) sonic = true;
+        // deldelh + delb, exactly zero at rest
+        R ddh_bal = sonic ? 0.f : delb * (0.25f * (su * su)) / s1s2bar;
+        R deldelh = ddh_bal - delb;
+        bool clipped_h = false;
+        {
+            R d0 = deldelh;
+            if (sE1 < -criticaltol && sE2 > criticaltol) {
+                deldelh = vmin(deldelh, hstarHLL * (sE2 - sE1) / sE2);
+                deldelh = vmax(deldelh, hstarHLL * (sE2 - sE1) / sE1);
+            } else if (sE1 >= criticaltol) {
+                deldelh = vmin(deldelh, hstarHLL * (sE2 - sE1) / sE1);
+                deldelh = vmax(deldelh, -hL);
+            } else if (sE2 <= -criticaltol) {
+                deldelh = vmin(deldelh, hR);
+                deldelh = vmax(deldelh, hstarHLL * (sE2 - sE1) / sE2);
+            }
+            clipped_h = deldelh != d0;
+        }
+        // g hbar delb + deldelphi, exactly zero at rest
+        R ddp_bal = sonic ? 0.f : g * hbar * delb * (s1s2bar - s1s2tilde) / s1s2bar;
+        R deldelphi = ddp_bal - g * hbar * delb;
+        bool clipped_p = false;
+        {
+            R d0 = deldelphi;
+            deldelphi = vmin(deldelphi, g * vmax(-hL * delb, -hR * delb));
+            deldelphi = vmax(deldelphi, g * vmin(-hL * delb, -hR * delb));
+            clipped_p = deldelphi != d0;
+        }
+        R d0 = clipped_h ? delh - deldelh : deta - ddh_bal;
+        R d1 = delhu;
+        R d2 = clipped_p ? (0.5f * g * (hR * hR) - 0.5f * g * (hL * hL) + dynphi) - deldelphi
+                         : dynphi + g * hbar * deta - ddp_bal;
+        // eigenvectors (1, lam0, lam0^2), (0, 0, 1), (1, lam2, lam2^2): solve directly
+        R det = lam2 - lam0;
+        R b0 = (lam2 * d0 - d1) / det;
+        R b2 = (d1 - lam0 * d0) / det;
+        R b1 = d2 - b0 * lam0 * lam0 - b2 * lam2 * lam2;
+        R fw[3][3];
+        fw[0][0] = b0 * lam0; fw[1][0] = b0 * lam0 * lam0; fw[2][0] = b0 * lam0 * vL;
+        fw[0][1] = 0.f;       fw[1][1] = b1;               fw[2][1] = 0.f;
+        fw[0][2] = b2 * lam2; fw[1][2] = b2 * lam2 * lam2; fw[2][2] = b2 * lam2 * vR;
+        R hustar_interface = huL + fw[0][0];
+        R trans = huR * vR - huL * vL - fw[2][0] - fw[2][2];
+        if (hustar_interface <= 0.f) fw[2][0] += trans; else fw[2][2] += trans;
+        R lam[3] = {lam0, lam1, lam2}, wall[3] = {wall0, wall1, wall2};
+        for (int mw = 0; mw < 3; ++mw) {
+            s[mw] = lam[mw] * wall[mw];
+            fwave[0][mw] = fw[0][mw] * wall[mw];
+            fwave[1][mw] = fw[1][mw] * wall[mw];
+            fwave[2][mw] = fw[2][mw] * wall[mw];
+        }
+    }
+    R am[3] = {0.f, 0.f, 0.f}, ap[3] = {0.f, 0.f, 0.f};
+    for (int mw = 0; mw < 3; ++mw) {
+        if (s[mw] < -1e-14f) { for (int j = 0; j < 3; ++j) am[j] += fwave[j][mw]; }
+        else if (s[mw] > 1e-14f) { for (int j = 0; j < 3; ++j) ap[j] += fwave[j][mw]; }
+        else { for (int j = 0; j < 3; ++j) { am[j] += 0.5f * fwave[j][mw]; ap[j] += 0.5f * fwave[j][mw]; } }
+    }
+    R pl[3] = {0.f, 0.f, 0.f}, pr[3] = {0.f, 0.f, 0.f};
+    if (lq[0] > drytol) { pl[0] = lq[1]; pl[1] = (lq[1] * lq[1]) / lq[0]; pl[2] = lq[1] * lq[2] / lq[0]; }
+    if (rq[0] > drytol) { pr[0] = rq[1]; pr[1] = (rq[1] * rq[1]) / rq[0]; pr[2] = rq[1] * rq[2] / rq[0]; }
+    R fluxl[3], fluxr[3];
+    for (int j = 0; j < 3; ++j) { fluxl[j] = pl[j] + am[j]; fluxr[j] = pr[j] - ap[j]; }
+    fl[0] = fluxl[0]; fl[1] = fluxl[1] * nx + fluxl[2] * tx; fl[2] = fluxl[1] * ny + fluxl[2] * ty;
+    fr[0] = fluxr[0]; fr[1] = fluxr[1] * nx + fluxr[2] * tx; fr[2] = fluxr[1] * ny + fluxr[2] * ty;
+    fl[0] = 0.5f * (fl[0] + fr[0]); fr[0] = fl[0];
+    speed = vmax(vmax(fabsf(s[0]), fabsf(s[1])), fabsf(s[2]));
+}
+
+}  // namespace braids
