@@ -32,7 +32,7 @@ class BraidsOptions(ctypes.Structure):
 
 
 class BraidsStepResult(ctypes.Structure):
-    _fields_ = [('accepted', _I), ('failure', _I), ('failed_stage', _I), ('pad', _I),
+    _fields_ = [('accepted', _I), ('failure', _I), ('failed_stage', _I), ('fail_cell', _I),
                 ('trial_cfl_peak', _D), ('stage_peak', _D * 3)]
 
 

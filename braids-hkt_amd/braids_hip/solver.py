@@ -128,9 +128,12 @@ class HydraulicSolver:
         rc = _lib.check(self.L.braids_step(self.h, float(dt), float(self.time if t is None else t),
                                            ctypes.byref(r)), 'braids_step')
         info = dict(accepted=bool(r.accepted), failure=r.failure, failed_stage=r.failed_stage,
-                    cfl=r.trial_cfl_peak, stage_peak=list(r.stage_peak))
+                    fail_cell=r.fail_cell, cfl=r.trial_cfl_peak, stage_peak=list(r.stage_peak))
         if rc == _lib.BRAIDS_REJECTED:
-            raise TrialRejected(f'stage {r.failed_stage} rejected (reason {r.failure}, CFL {r.trial_cfl_peak:g})')
+            err = TrialRejected(f'stage {r.failed_stage} rejected (reason {r.failure}, cell {r.fail_cell}, '
+                                f'CFL {r.trial_cfl_peak:g})')
+            err.info = info
+            raise err
         return info
 
     def advance(self, nsteps, tend=1e300):

@@ -323,7 +323,7 @@ struct Solver : SolverBase {
         if (profiling) collect(3);
         const Control *hc = (const Control *)host_block;
         const double *hl = (const double *)((const char *)host_block + sizeof(Control));
-        res->accepted = 0; res->failure = 0; res->failed_stage = -1; res->trial_cfl_peak = 0;
+        res->accepted = 0; res->failure = 0; res->failed_stage = -1; res->fail_cell = -1; res->trial_cfl_peak = 0;
         double peak = 0;
         for (int k = 0; k < 3; ++k) {
             double pk; unsigned long long b = hc->peak_bits[k];
@@ -336,6 +336,7 @@ struct Solver : SolverBase {
             if (code) {
                 res->failure = code >= FAIL_TRACE ? code - FAIL_TRACE : (code == FAIL_DRAIN ? 2 : code);
                 res->failed_stage = k;
+                res->fail_cell = hc->fail_cell[k] - 1;
                 break;
             }
         }

@@ -281,18 +281,36 @@ __device__ __forceinline__ void face_value(const MeshView<R> &m, const StateView
 }
 
 // ---------------------------------------------------------------------------
-// Boundary traces: one workgroup per open boundary.  The running sums of the
-// total-discharge distribution are formed by one lane in boundary face order.
+// Deterministic block sum (fixed tree order) over BLOCK threads.
+template <int BLOCK>
+__device__ __forceinline__ double block_sum(double v, double *sh)
+{
+    sh[threadIdx.x] = v;
+    __syncthreads();
+#pragma unroll
+    for (int s = BLOCK / 2; s > 0; s >>= 1) {
+        if ((int)threadIdx.x < s) sh[threadIdx.x] += sh[threadIdx.x + s];
+        __syncthreads();
+    }
+    double r = sh[0];
+    __syncthreads();
+    return r;
+}
+
+constexpr int TRACE_BLOCK = 256;
+
+// Boundary traces: one workgroup per open boundary.  The sums of the
+// total-discharge distribution are strided per lane, then tree-reduced.
 template <class R, int SCHEME>
-__global__ void __launch_bounds__(256) k_traces(MeshView<R> m, StateView<R> q, const double *__restrict__ z,
-                                                Work<R> w, Control *ctl, int stage, double t)
+__global__ void __launch_bounds__(TRACE_BLOCK) k_traces(MeshView<R> m, StateView<R> q, const double *__restrict__ z,
+                                                        Work<R> w, Control *ctl, int stage, double t)
 {
     const int b = blockIdx.x;
     if (b >= m.nb) return;
     const int k0 = m.bnd_offset[b], k1 = m.bnd_offset[b + 1];
     const double dry = (double)m.dry, g = (double)m.grav;
-    __shared__ double s_total, s_zavg, s_hyd[2];
-    __shared__ int s_fail;
+    __shared__ double sh[TRACE_BLOCK];
+    __shared__ double s_hyd[2];
     if (threadIdx.x == 0) {
         // boundary_hydraulic_at (:9248-9269): hold the nearest knot outside the series
         double Q = m.forcing_q[b], S = m.forcing_stage[b];
@@ -311,39 +329,44 @@ __global__ void __launch_bounds__(256) k_traces(MeshView<R> m, StateView<R> q, c
                 }
             }
         }
-        s_hyd[0] = Q; s_hyd[1] = S; s_fail = 0;
+        s_hyd[0] = Q; s_hyd[1] = S;
         w.hydraulic[2 * b] = Q; w.hydraulic[2 * b + 1] = S;
-        if (m.mode[b] == 6) {
-            double zavg = 0, wet = 0, total = 0;
-            for (int k = k0; k < k1; ++k) {
-                int f = m.bnd_faces[k], o = m.owner[f];
-                if (q.h[o] <= dry) continue;
-                zavg = zavg + (z[o] + q.h[o]) * (double)m.length[f];
-                wet = wet + (double)m.length[f];
-            }
-            if (Q > 0 && wet <= 0) s_fail = 8;
-            if (wet > 0) zavg = zavg / wet;
-            for (int k = k0; k < k1; ++k) {
-                int f = m.bnd_faces[k], o = m.owner[f];
-                w.target_u[k] = R(0);
-                double h = q.h[o];
-                if (h <= dry) continue;
-                double weight = pow(vmax(zavg - z[o], 0.0), 1.5) * pow(vmax(h, 1e-2), 1.0 / 6.0);
-                if ((double)m.manning[o] > 0) weight = weight / (double)m.manning[o];
-                double tu = weight * (double)m.length[f];
-                w.target_u[k] = (R)tu;
-                total = total + tu;
-            }
-            if (Q > 0 && total <= 0) s_fail = 8;
-            s_total = total; s_zavg = zavg;
-        }
     }
     __syncthreads();
-    int fail = s_fail;
     const double Q = s_hyd[0], S = s_hyd[1];
+    int fail = 0;
+    double total = 0.0;
+    if (m.mode[b] == 6) {
+        double zs = 0.0, wl = 0.0;
+        for (int k = k0 + threadIdx.x; k < k1; k += TRACE_BLOCK) {
+            const int f = m.bnd_faces[k], o = m.owner[f];
+            if (q.h[o] <= dry) continue;
+            zs = zs + (z[o] + q.h[o]) * (double)m.length[f];
+            wl = wl + (double)m.length[f];
+        }
+        double zavg = block_sum<TRACE_BLOCK>(zs, sh);
+        const double wet = block_sum<TRACE_BLOCK>(wl, sh);
+        if (Q > 0 && wet <= 0) fail = 8;
+        if (wet > 0) zavg = zavg / wet;
+        double ts = 0.0;
+        for (int k = k0 + threadIdx.x; k < k1; k += TRACE_BLOCK) {
+            const int f = m.bnd_faces[k], o = m.owner[f];
+            const double h = q.h[o];
+            double tu = 0.0;
+            if (h > dry) {
+                double weight = pow(vmax(zavg - z[o], 0.0), 1.5) * pow(vmax(h, 1e-2), 1.0 / 6.0);
+                if ((double)m.manning[o] > 0) weight = weight / (double)m.manning[o];
+                tu = weight * (double)m.length[f];
+            }
+            w.target_u[k] = (R)tu;
+            ts = ts + tu;
+        }
+        total = block_sum<TRACE_BLOCK>(ts, sh);
+        if (Q > 0 && total <= 0) fail = 8;
+    }
     const int nf = m.nf;
     if (!fail) {
-        for (int k = k0 + threadIdx.x; k < k1; k += blockDim.x) {
+        for (int k = k0 + threadIdx.x; k < k1; k += TRACE_BLOCK) {
             const int f = m.bnd_faces[k], o = m.owner[f];
             const R nxr = m.normal[f], nyr = m.normal[nf + f];
             const double nx = nxr, ny = nyr;
@@ -357,7 +380,7 @@ __global__ void __launch_bounds__(256) k_traces(MeshView<R> m, StateView<R> q, c
                         tr0 = h; tr1 = h * (qx / h - d / h * nx); tr2 = h * (qy / h - d / h * ny);
                     }
                 } else {
-                    double ub = Q * (double)w.target_u[k] / (s_total * h * (double)m.length[f]);
+                    double ub = Q * (double)w.target_u[k] / (total * h * (double)m.length[f]);
                     double qx = q.hu[o], qy = q.hv[o];
                     double d = qx * nx + qy * ny;
                     double tx = qx / h - d / h * nx, ty = qy / h - d / h * ny;
@@ -799,7 +822,7 @@ __global__ void __launch_bounds__(256) k_gather(MeshView<R> m, StateView<R> in, 
     }
     double wm = wave_max(ratio);
     if ((threadIdx.x & 63) == 0 && wm > 0.0) atomicMax(&ctl->peak_bits[STAGE], dbits(wm));
-    if (fail) atomicMax(&ctl->failure[STAGE], fail);
+    if (fail) { atomicMax(&ctl->failure[STAGE], fail); atomicMax(&ctl->fail_cell[STAGE], c + 1); }
 }
 
 // Spatial rate without draining or CFL (bend_native_rates_tail.inc:4-20):
@@ -834,18 +857,20 @@ __global__ void __launch_bounds__(256) k_rates(MeshView<R> m, StateView<R> in, W
 }
 
 // Boundary ledger of one stage: accepted(1,b) -= dt L fl(1,f) (:12058-12066),
-// summed by one lane per boundary in boundary face order.
+// one workgroup per boundary (strided partial sums, fixed-order tree).
 template <class R>
-__global__ void k_ledger(MeshView<R> m, Work<R> w, double *ledger_stage, double dt)
+__global__ void __launch_bounds__(TRACE_BLOCK) k_ledger(MeshView<R> m, Work<R> w, double *ledger_stage, double dt)
 {
-    const int b = blockIdx.x * blockDim.x + threadIdx.x;
+    const int b = blockIdx.x;
     if (b >= m.nb) return;
+    __shared__ double sh[TRACE_BLOCK];
     double s = 0;
-    for (int k = m.bnd_offset[b]; k < m.bnd_offset[b + 1]; ++k) {
+    for (int k = m.bnd_offset[b] + threadIdx.x; k < m.bnd_offset[b + 1]; k += TRACE_BLOCK) {
         const int f = m.bnd_faces[k];
         s = s - dt * (double)m.length[f] * (double)w.F[f];
     }
-    ledger_stage[b] = s;
+    s = block_sum<TRACE_BLOCK>(s, sh);
+    if (threadIdx.x == 0) ledger_stage[b] = s;
 }
 
 }  // namespace braids

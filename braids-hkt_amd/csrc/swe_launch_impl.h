@@ -52,7 +52,7 @@ static void stage_kernels(const MeshView<R> &m, StateView<R> in, StateView<R> ba
     { EvScope e(cfg, s, mode, KK_PREP); launch_prep<R, SCHEME>(m, in, z, w, ctl, mode, cfg, s); }
     if (m.nb > 0) {
         EvScope e(cfg, s, mode, KK_TRACES);
-        hipLaunchKernelGGL((k_traces<R, SCHEME>), dim3(m.nb), dim3(256), 0, s, m, in, z, w, ctl, mode, t);
+        hipLaunchKernelGGL((k_traces<R, SCHEME>), dim3(m.nb), dim3(TRACE_BLOCK), 0, s, m, in, z, w, ctl, mode, t);
     }
     { EvScope e(cfg, s, mode, KK_FACE); launch_face<R, SCHEME>(m, in, z, w, cfg, s); }
     { EvScope e(cfg, s, mode, KK_DRAIN);
@@ -68,7 +68,7 @@ static void stage_kernels(const MeshView<R> &m, StateView<R> in, StateView<R> ba
     }
     if (m.nb > 0 && ledger_stage) {
         EvScope e(cfg, s, mode, KK_LEDGER);
-        hipLaunchKernelGGL((k_ledger<R>), dim3(1), dim3(64), 0, s, m, w, ledger_stage, dt);
+        hipLaunchKernelGGL((k_ledger<R>), dim3(m.nb), dim3(TRACE_BLOCK), 0, s, m, w, ledger_stage, dt);
     }
 }
 
@@ -88,7 +88,7 @@ static void rate_kernels(const MeshView<R> &m, StateView<R> in, const double *z,
     const int B = cfg.block;
     launch_prep<R, SCHEME>(m, in, z, w, ctl, 3, cfg, s);
     if (m.nb > 0)
-        hipLaunchKernelGGL((k_traces<R, SCHEME>), dim3(m.nb), dim3(256), 0, s, m, in, z, w, ctl, 3, t);
+        hipLaunchKernelGGL((k_traces<R, SCHEME>), dim3(m.nb), dim3(TRACE_BLOCK), 0, s, m, in, z, w, ctl, 3, t);
     launch_face<R, SCHEME>(m, in, z, w, cfg, s);
     const int n = m.nc > m.nf ? m.nc : m.nf;
     hipLaunchKernelGGL((k_rates<R>), dim3(grid_for(n, B)), dim3(B), 0, s, m, in, w, rate, water, 0);

@@ -56,6 +56,7 @@ struct Control {
     unsigned long long peak_bits[4];   // max CFL ratio over active cells
     int failure[4];                    // encoded failure (max), see FAIL_*
     int drain_any[4];                  // any draining cell
+    int fail_cell[4];                  // 1 + largest cell index that failed in the gather
 };
 
 }  // namespace braids

@@ -63,7 +63,7 @@ typedef struct braids_step_result {
     int accepted;       /* 1 if the SSPRK(3,3) step was committed */
     int failure;        /* first failing stage's code (0, 2, 3, 5, 6, 8) */
     int failed_stage;   /* -1 or 0..2 */
-    int pad;
+    int fail_cell;      /* a cell that failed the gather checks (largest index), or -1 */
     double trial_cfl_peak;  /* max CFL ratio of the stages that ran */
     double stage_peak[3];
 } braids_step_result;
