@@ -314,7 +314,8 @@ def plain_channel(nx=500, ny=25, length=2000.0, width=100.0, slope=1e-3, sigma=0
 
 def braided_reach(nx=1768, ny=141, length=5000.0, width=400.0, slope=1e-3, amplitude=60.0,
                   wavelength=800.0, nbars=200, q=1500.0, seed=2, depth=3.0,
-                  bar_height=(0.5, 2.0), bar_sigma=(15.0, 60.0), bank=4.0) -> Mesh:
+                  bar_height=(0.5, 2.0), bar_sigma=(15.0, 60.0), bank=4.0,
+                  open_min_depth=None) -> Mesh:
     """Braided synthetic reach 5 km x 400 m: sinusoidal thalweg (amplitude
     60 m, wavelength 800 m), parabolic cross-section, 200 seeded Gaussian bars
     (height 0.5-2 m, sigma 15-60 m, centred away from the inlet and outlet
@@ -342,8 +343,16 @@ def braided_reach(nx=1768, ny=141, length=5000.0, width=400.0, slope=1e-3, ampli
     def depth0(x, y, z):
         return (-slope * x + depth) - z
 
-    return structured_triangles(nx, ny, dx, dy, bed=bed, manning=manning, depth0=depth0,
-                                inflow_q=q, outlet_stage=-slope * length + depth, name='braided_reach')
+    m = structured_triangles(nx, ny, dx, dy, bed=bed, manning=manning, depth0=depth0,
+                             inflow_q=q, outlet_stage=-slope * length + depth, name='braided_reach')
+    # The inlet and outlet sections span the active channel only: end faces
+    # whose owner starts shallower than open_min_depth (the bank margins)
+    # are walls, so no open face ever feeds a nearly dry cell.
+    thr = 0.25 * depth if open_min_depth is None else open_min_depth
+    shallow = (m.bid >= 0) & (m.q0[0][m.owner] < thr)
+    m.bid[shallow] = -1
+    m.build_incidence()
+    return m
 
 
 def lake_at_rest(nx=20, ny=6, dx=4.0, dy=4.0, stage=1.0, seed=3, bumps=True) -> Mesh:

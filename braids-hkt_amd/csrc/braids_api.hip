@@ -35,14 +35,14 @@ __global__ void k_load_state(const double *q, StateView<R> s, int nc)
 {
     int c = blockIdx.x * blockDim.x + threadIdx.x;
     if (c >= nc) return;
-    s.h[c] = q[c]; s.hu[c] = (R)q[nc + c]; s.hv[c] = (R)q[2 * nc + c];
+    s.h[c] = q[c]; s.hu[c] = q[nc + c]; s.hv[c] = q[2 * nc + c];
 }
 template <class R>
 __global__ void k_store_state(StateView<R> s, double *q, int nc)
 {
     int c = blockIdx.x * blockDim.x + threadIdx.x;
     if (c >= nc) return;
-    q[c] = s.h[c]; q[nc + c] = (double)s.hu[c]; q[2 * nc + c] = (double)s.hv[c];
+    q[c] = s.h[c]; q[nc + c] = s.hu[c]; q[2 * nc + c] = s.hv[c];
 }
 // prepare_hydraulic_stage (cutcell_solver.f90:8513-8534) in place
 template <class R>
@@ -51,10 +51,10 @@ __global__ void k_prepare(StateView<R> s, int nc, double dry)
     int c = blockIdx.x * blockDim.x + threadIdx.x;
     if (c >= nc) return;
     double h = s.h[c];
-    if (h <= dry) { s.hu[c] = R(0); s.hv[c] = R(0); return; }
+    if (h <= dry) { s.hu[c] = 0.0; s.hv[c] = 0.0; return; }
     double hu = s.hu[c], hv = s.hv[c];
     double sp = sqrt(hu * hu + hv * hv) / h;
-    if (sp > 50.0) { s.hu[c] = (R)(hu * (50.0 / sp)); s.hv[c] = (R)(hv * (50.0 / sp)); }
+    if (sp > 50.0) { s.hu[c] = hu * (50.0 / sp); s.hv[c] = hv * (50.0 / sp); }
 }
 __global__ void k_zero_bytes(unsigned int *p, int nwords)
 {

@@ -80,7 +80,7 @@ __global__ void __launch_bounds__(256) k_prep(MeshView<R> m, StateView<R> q, con
             R gx[4] = {R(0), R(0), R(0), R(0)}, gy[4] = {R(0), R(0), R(0), R(0)};
             const double zc = z[c], etac = h + zc;
             R uc = R(0), vc = R(0);
-            if (h > (double)dry) { uc = q.hu[c] / (R)h; vc = q.hv[c] / (R)h; }
+            if (h > (double)dry) { uc = (R)(q.hu[c] / h); vc = (R)(q.hv[c] / h); }
             if (act) {
                 R b0[4] = {R(0), R(0), R(0), R(0)}, b1[4] = {R(0), R(0), R(0), R(0)};
                 // lo/hi as offsets from the centre value for eta and z (FP64 differences)
@@ -92,7 +92,7 @@ __global__ void __launch_bounds__(256) k_prep(MeshView<R> m, StateView<R> q, con
                     const double ho = q.h[o], zo = z[o];
                     double etao = ho + zo;
                     R uo = R(0), vo = R(0);
-                    if (ho > (double)dry) { uo = q.hu[o] / (R)ho; vo = q.hv[o] / (R)ho; }
+                    if (ho > (double)dry) { uo = (R)(q.hu[o] / ho); vo = (R)(q.hv[o] / ho); }
                     if (etac - zc > (double)dry && etao - zo <= (double)dry && zo >= etac) etao = etac;
                     R de = (R)(etao - etac), du = uo - uc, dv = vo - vc, dz = (R)(zo - zc);
                     b0[0] = b0[0] + wr0 * de; b1[0] = b1[0] + wr1 * de;
@@ -258,7 +258,7 @@ __device__ __forceinline__ void face_value(const MeshView<R> &m, const StateView
     const double h = q.h[c];
     const double zc = z[c], etac = h + zc;
     R uc = R(0), vc = R(0);
-    if (h > (double)m.dry) { uc = q.hu[c] / (R)h; vc = q.hv[c] / (R)h; }
+    if (h > (double)m.dry) { uc = (R)(q.hu[c] / h); vc = (R)(q.hv[c] / h); }
     if (SCHEME == 1) { eta = etac; u = uc; v = vc; zb = zc; return; }
     const int nc = m.nc;
     const R g0x = w.grad[c], g0y = w.grad[nc + c];
@@ -718,16 +718,17 @@ __global__ void __launch_bounds__(256) k_gather(MeshView<R> m, StateView<R> in, 
     int fail = 0;
     if (c < nc) {
         const double h = in.h[c];
-        double nh = h;
-        R nhu = in.hu[c], nhv = in.hv[c];
+        double nh = h, nhu = in.hu[c], nhv = in.hv[c];
         const R g = m.grav, dry = m.dry;
         if (w.flags[c] & FLAG_ACTIVE) {
             const int k0 = m.offset[c], k1 = m.offset[c + 1];
             const bool drain = ctl->drain_any[STAGE] != 0;
             const R A = m.area[c];
-            double s0 = 0.0;
-            R s1 = R(0), s2 = R(0), cfl = R(0);
-            const R p = R(.5) * g * (R)(h * h);
+            // FP64 mode: every sum in the reference order.  FP32 mode: face
+            // products in FP32, accumulated in FP64.
+            double s0 = 0.0, s1 = 0.0, s2 = 0.0;
+            R cfl = R(0);
+            const double p = .5 * (double)g * (h * h);
             for (int k = k0; k < k1; ++k) {
                 const int f = m.adj_face[k];
                 const bool own = (m.adj_flags[k] & ADJ_OWNER) != 0;
@@ -745,10 +746,7 @@ __global__ void __launch_bounds__(256) k_gather(MeshView<R> m, StateView<R> in, 
                             const R corr = (th - R(1)) * F1;
                             R vx = R(0), vy = R(0);
                             const double hd = in.h[donor];
-                            if (hd > (double)dry) {
-                                if (sizeof(R) == 8) { vx = (R)((double)in.hu[donor] / hd); vy = (R)((double)in.hv[donor] / hd); }
-                                else { vx = in.hu[donor] / (R)hd; vy = in.hv[donor] / (R)hd; }
-                            }
+                            if (hd > (double)dry) { vx = (R)(in.hu[donor] / hd); vy = (R)(in.hv[donor] / hd); }
                             F1 = F1 * th;
                             G2 = G2 + corr * vx; G3 = G3 + corr * vy;
                         }
@@ -757,66 +755,64 @@ __global__ void __launch_bounds__(256) k_gather(MeshView<R> m, StateView<R> in, 
                 const R wgt = L / A;
                 cfl = cfl + L * S;
                 if (sizeof(R) == 8) {
-                    const R nxf = m.normal[f], nyf = m.normal[nf + f];
+                    const double nxf = m.normal[f], nyf = m.normal[nf + f];
                     if (own) {
                         s0 = s0 + (double)wgt * (double)F1;
-                        s1 = s1 + wgt * (G2 - p * nxf);
-                        s2 = s2 + wgt * (G3 - p * nyf);
+                        s1 = s1 + (double)wgt * ((double)G2 - p * nxf);
+                        s2 = s2 + (double)wgt * ((double)G3 - p * nyf);
                     } else {
                         s0 = s0 - (double)wgt * (double)F1;
-                        s1 = s1 - wgt * (G2 - p * nxf);
-                        s2 = s2 - wgt * (G3 - p * nyf);
+                        s1 = s1 - (double)wgt * ((double)G2 - p * nxf);
+                        s2 = s2 - (double)wgt * ((double)G3 - p * nyf);
                     }
                 } else {
                     // FP32 face fluxes are stored net of this cell's pressure
                     const double wd = (double)L / (double)A;
-                    if (own) { s0 = s0 + wd * (double)F1; s1 = s1 + wgt * G2; s2 = s2 + wgt * G3; }
-                    else { s0 = s0 - wd * (double)F1; s1 = s1 - wgt * G2; s2 = s2 - wgt * G3; }
+                    if (own) { s0 = s0 + wd * (double)F1; s1 = s1 + (double)(wgt * G2); s2 = s2 + (double)(wgt * G3); }
+                    else { s0 = s0 - wd * (double)F1; s1 = s1 - (double)(wgt * G2); s2 = s2 - (double)(wgt * G3); }
                 }
             }
             ratio = (double)(R(dt) * cfl / A);
             if (sizeof(R) == 8) ratio = dt * (double)cfl / (double)A;
             if (ratio > (double)m.cfl_target) fail = 2;
             nh = h - dt * s0;
-            nhu = in.hu[c] - (R)dt * s1;
-            nhv = in.hv[c] - (R)dt * s2;
-            if (sizeof(R) == 8) { nhu = (R)((double)in.hu[c] - dt * (double)s1); nhv = (R)((double)in.hv[c] - dt * (double)s2); }
+            nhu = in.hu[c] - dt * s1;
+            nhv = in.hv[c] - dt * s2;
             if (nh < 0) {
                 const double hmax = __longlong_as_double((long long)ctl->hmax_bits[STAGE]);
-                if (-nh <= 64 * 2.2204460492503131e-16 * vmax(hmax, 1.0)) { nh = 0; nhu = R(0); nhv = R(0); }
+                if (-nh <= 64 * 2.2204460492503131e-16 * vmax(hmax, 1.0)) { nh = 0; nhu = 0; nhv = 0; }
                 else fail = fail > 3 ? fail : 3;
             }
-            if (!isfinite(nh) || !isfinite((double)nhu) || !isfinite((double)nhv)) fail = fail > 5 ? fail : 5;
+            if (!isfinite(nh) || !isfinite(nhu) || !isfinite(nhv)) fail = fail > 5 ? fail : 5;
         }
         // SSP stage combination
         if (STAGE == 3) { out.h[c] = nh; out.hu[c] = nhu; out.hv[c] = nhv; }
         else if (STAGE == 1) {
             nh = .75 * base.h[c] + .25 * nh;
-            if (sizeof(R) == 8) { nhu = (R)(.75 * (double)base.hu[c] + .25 * (double)nhu); nhv = (R)(.75 * (double)base.hv[c] + .25 * (double)nhv); }
-            else { nhu = R(.75) * base.hu[c] + R(.25) * nhu; nhv = R(.75) * base.hv[c] + R(.25) * nhv; }
+            nhu = .75 * base.hu[c] + .25 * nhu;
+            nhv = .75 * base.hv[c] + .25 * nhv;
         } else if (STAGE == 2) {
             nh = base.h[c] / 3.0 + 2.0 * nh / 3.0;
-            if (sizeof(R) == 8) { nhu = (R)((double)base.hu[c] / 3.0 + 2.0 * (double)nhu / 3.0); nhv = (R)((double)base.hv[c] / 3.0 + 2.0 * (double)nhv / 3.0); }
-            else { nhu = base.hu[c] / R(3) + R(2) * nhu / R(3); nhv = base.hv[c] / R(3) + R(2) * nhv / R(3); }
+            nhu = base.hu[c] / 3.0 + 2.0 * nhu / 3.0;
+            nhv = base.hv[c] / 3.0 + 2.0 * nhv / 3.0;
         }
         // prepare_hydraulic_stage
         if (STAGE == 3) { /* plain Euler stage: output as computed */ }
-        else if (nh <= (double)dry) { nhu = R(0); nhv = R(0); }
+        else if (nh <= (double)dry) { nhu = 0; nhv = 0; }
         else {
-            R sp = Num<R>::sqrt_(nhu * nhu + nhv * nhv) / (R)nh;
-            if (sizeof(R) == 8) sp = (R)(sqrt((double)nhu * (double)nhu + (double)nhv * (double)nhv) / nh);
-            if (sp > R(50)) { nhu = nhu * (R(50) / sp); nhv = nhv * (R(50) / sp); }
+            const double sp = sqrt(nhu * nhu + nhv * nhv) / nh;
+            if (sp > 50.0) { nhu = nhu * (50.0 / sp); nhv = nhv * (50.0 / sp); }
         }
         if (STAGE == 2) {
             // Manning part of source(): hu /= 1 + dt g n^2 |u| / h^(4/3)
-            if (nh <= (double)dry) { nhu = R(0); nhv = R(0); }
+            if (nh <= (double)dry) { nhu = 0; nhv = 0; }
             else {
-                const double sp = sqrt((double)nhu * (double)nhu + (double)nhv * (double)nhv) / nh;
+                const double sp = sqrt(nhu * nhu + nhv * nhv) / nh;
                 const double n = (double)m.manning[c];
                 const double d = 1 + dt * (double)g * (n * n) * sp / pow(nh, 4.0 / 3);
-                nhu = (R)((double)nhu / d); nhv = (R)((double)nhv / d);
+                nhu = nhu / d; nhv = nhv / d;
             }
-            if (!isfinite(nh) || !isfinite((double)nhu) || !isfinite((double)nhv) || nh < 0) fail = fail > 5 ? fail : 5;
+            if (!isfinite(nh) || !isfinite(nhu) || !isfinite(nhv) || nh < 0) fail = fail > 5 ? fail : 5;
         }
         if (STAGE != 3) { out.h[c] = nh; out.hu[c] = nhu; out.hv[c] = nhv; }
     }
@@ -837,9 +833,8 @@ __global__ void __launch_bounds__(256) k_rates(MeshView<R> m, StateView<R> in, W
     if (c < nc) {
         const double h = in.h[c];
         const R A = m.area[c];
-        const R p = R(.5) * m.grav * (R)(h * h);
-        double s0 = 0;
-        R s1 = R(0), s2 = R(0);
+        const double p = .5 * (double)m.grav * (h * h);
+        double s0 = 0, s1 = 0, s2 = 0;
         for (int k = m.offset[c]; k < m.offset[c + 1]; ++k) {
             const int f = m.adj_face[k];
             const bool own = (m.adj_flags[k] & ADJ_OWNER) != 0;
@@ -847,12 +842,12 @@ __global__ void __launch_bounds__(256) k_rates(MeshView<R> m, StateView<R> in, W
             const R F1 = w.F[f];
             const R G2 = own ? w.F[nf + f] : w.F[3 * nf + f];
             const R G3 = own ? w.F[2 * nf + f] : w.F[4 * nf + f];
-            const R nxf = m.normal[f], nyf = m.normal[nf + f];
-            const R pp = sizeof(R) == 8 ? p : R(0);      // FP32 fluxes are net of the cell pressure
-            if (own) { s0 = s0 + (double)wgt * (double)F1; s1 = s1 + wgt * (G2 - pp * nxf); s2 = s2 + wgt * (G3 - pp * nyf); }
-            else { s0 = s0 - (double)wgt * (double)F1; s1 = s1 - wgt * (G2 - pp * nxf); s2 = s2 - wgt * (G3 - pp * nyf); }
+            const double nxf = m.normal[f], nyf = m.normal[nf + f];
+            const double pp = sizeof(R) == 8 ? p : 0.0;      // FP32 fluxes are net of the cell pressure
+            if (own) { s0 = s0 + (double)wgt * (double)F1; s1 = s1 + (double)wgt * ((double)G2 - pp * nxf); s2 = s2 + (double)wgt * ((double)G3 - pp * nyf); }
+            else { s0 = s0 - (double)wgt * (double)F1; s1 = s1 - (double)wgt * ((double)G2 - pp * nxf); s2 = s2 - (double)wgt * ((double)G3 - pp * nyf); }
         }
-        rate[c] = -s0; rate[nc + c] = -(double)s1; rate[2 * nc + c] = -(double)s2;
+        rate[c] = -s0; rate[nc + c] = -s1; rate[2 * nc + c] = -s2;
     }
 }
 

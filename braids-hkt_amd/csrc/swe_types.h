@@ -34,9 +34,11 @@ template <class R> struct MeshView {
     const double *series_time, *series_q, *series_stage;
 };
 
+// The conserved state is held in FP64 in both modes (it is the mass and
+// momentum inventory); R is the arithmetic type of reconstruction and fluxes.
 template <class R> struct StateView {
     double *h;
-    R *hu, *hv;
+    double *hu, *hv;
 };
 
 template <class R> struct Work {
