@@ -149,8 +149,15 @@ class HydraulicSolver:
         return acc.value, ret.value
 
     # -- measurement --------------------------------------------------------
-    def profile(self, enable=True):
-        _lib.check(self.L.braids_profile(self.h, int(bool(enable))), 'braids_profile')
+    def profile(self, kinds=True):
+        """kinds: True (all kernel kinds), False (off) or an iterable of kind names."""
+        if kinds is True:
+            mask = -1
+        elif not kinds:
+            mask = 0
+        else:
+            mask = sum(1 << _lib.KERNEL_KINDS.index(k) for k in kinds)
+        _lib.check(self.L.braids_profile(self.h, int(mask)), 'braids_profile')
 
     def profile_read(self):
         ms = (ctypes.c_double * 6)(); n = (ctypes.c_longlong * 6)()

@@ -105,16 +105,18 @@ struct SolverBase {
     std::vector<hipEvent_t> events;
     double prof_ms[KK_COUNT] = {0};
     long long prof_n[KK_COUNT] = {0};
-    virtual void set_events(hipEvent_t *ev) = 0;
+    virtual void set_events(hipEvent_t *ev, int mask) = 0;
+    int prof_mask = 0;
     int enable_profile(int on)
     {
+        prof_mask = on == 0 ? 0 : (on < 0 ? (1 << KK_COUNT) - 1 : on);
         for (int k = 0; k < KK_COUNT; ++k) { prof_ms[k] = 0; prof_n[k] = 0; }
         if (on && events.empty()) {
             events.resize(4 * KK_COUNT * 2);
             for (auto &e : events)
                 if (hipEventCreate(&e) != hipSuccess) { set_error("hipEventCreate failed"); return BRAIDS_EHIP; }
         }
-        set_events(on ? events.data() : nullptr);
+        set_events(on ? events.data() : nullptr, prof_mask);
         return BRAIDS_OK;
     }
     void collect(int slots)
@@ -123,6 +125,7 @@ struct SolverBase {
         for (int slot = 0; slot < slots; ++slot)
             for (int k = 0; k < KK_COUNT; ++k) {
                 if ((k == KK_TRACES || k == KK_LEDGER) && nb == 0) continue;
+                if (!((prof_mask >> k) & 1)) continue;
                 float ms = 0;
                 if (hipEventElapsedTime(&ms, events[(slot * KK_COUNT + k) * 2], events[(slot * KK_COUNT + k) * 2 + 1]) == hipSuccess) {
                     prof_ms[k] += ms; prof_n[k] += 1;
@@ -145,7 +148,7 @@ struct Solver : SolverBase {
     size_t block_bytes = 0;
     LaunchCfg cfg;
     double *qscratch = nullptr;        // [3*nc] double for conversions
-    void set_events(hipEvent_t *ev) override { cfg.ev = ev; profiling = ev != nullptr; }
+    void set_events(hipEvent_t *ev, int mask) override { cfg.ev = ev; cfg.ev_mask = mask; profiling = ev != nullptr; }
 
     ~Solver() override
     {

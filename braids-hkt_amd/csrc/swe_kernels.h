@@ -42,6 +42,37 @@ __device__ __forceinline__ int wave_or(int v)
     return v;
 }
 
+// Block-level max of a non-negative double, then one device atomic per
+// workgroup (every thread of the block must call it).
+__device__ __forceinline__ void block_max_to(double v, unsigned long long *target)
+{
+    __shared__ double part[16];
+    v = wave_max(v);
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    if (lane == 0) part[wave] = v;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double m = 0.0;
+        const int nw = (blockDim.x + 63) >> 6;
+        for (int i = 0; i < nw; ++i) m = part[i] > m ? part[i] : m;
+        if (m > 0.0) atomicMax(target, (unsigned long long)__double_as_longlong(m));
+    }
+}
+__device__ __forceinline__ void block_or_to(int v, int *target)
+{
+    __shared__ int part[16];
+    v = wave_or(v);
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    if (lane == 0) part[wave] = v;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        int m = 0;
+        const int nw = (blockDim.x + 63) >> 6;
+        for (int i = 0; i < nw; ++i) m |= part[i];
+        if (m) atomicOr(target, m);
+    }
+}
+
 // XCD-aware block remap: consecutive logical blocks (neighbouring mesh
 // ranges in Hilbert order) are kept on one XCD so they share its L2.
 __device__ __forceinline__ int logical_block(int remap)
@@ -106,13 +137,16 @@ __global__ void __launch_bounds__(256) k_prep(MeshView<R> m, StateView<R> q, con
                 }
                 const R a11 = m.ls[c], a12 = m.ls[nc + c], a22 = m.ls[2 * nc + c], det = m.ls[3 * nc + c];
                 if (!(det <= R(1e-12) * ((a11 + a22) * (a11 + a22)))) {
-                    const R e64 = R(64) * Num<R>::eps();
-                    R guard[4], theta[4] = {R(1), R(1), R(1), R(1)};
+                    // The roundoff guard of the reference limiter (64 eps of FP64):
+                    // the eta and z differences are formed in FP64, so an increment
+                    // above this guard is a real slope in either precision.
+                    const double e64d = 64 * 2.2204460492503131e-16;
+                    double guard[4];
+                    R theta[4] = {R(1), R(1), R(1), R(1)};
                     R cen[4] = {R(0), uc, vc, R(0)};
-                    // eta and z guards use the absolute centre values
-                    guard[0] = e64 * vmax(vmax(R(1), (R)fabs(etac)), vmax((R)fabs(etac + (double)lo[0]), (R)fabs(etac + (double)hi[0])));
-                    guard[3] = e64 * vmax(vmax(R(1), (R)fabs(zc)), vmax((R)fabs(zc + (double)lo[3]), (R)fabs(zc + (double)hi[3])));
-                    for (int j = 1; j < 3; ++j) guard[j] = e64 * vmax(vmax(R(1), (R)fabs(cen[j])), vmax((R)fabs(lo[j]), (R)fabs(hi[j])));
+                    guard[0] = e64d * vmax(vmax(1.0, fabs(etac)), vmax(fabs(etac + (double)lo[0]), fabs(etac + (double)hi[0])));
+                    guard[3] = e64d * vmax(vmax(1.0, fabs(zc)), vmax(fabs(zc + (double)lo[3]), fabs(zc + (double)hi[3])));
+                    for (int j = 1; j < 3; ++j) guard[j] = e64d * vmax(vmax(1.0, (double)fabs(cen[j])), vmax((double)fabs(lo[j]), (double)fabs(hi[j])));
                     for (int j = 0; j < 4; ++j) {
                         gx[j] = (a22 * b0[j] - a12 * b1[j]) / det;
                         gy[j] = (a11 * b1[j] - a12 * b0[j]) / det;
@@ -124,11 +158,11 @@ __global__ void __launch_bounds__(256) k_prep(MeshView<R> m, StateView<R> q, con
                         if ((m.adj_flags[k] & ADJ_OUTLET2) && etac - zc > (double)dry) {
                             const int f = m.adj_face[k];
                             if (fabs(uc * m.normal[f] + vc * m.normal[m.nf + f]) < Num<R>::sqrt_(m.grav * (R)(etac - zc))) {
-                                if (fabs(delta[3]) > e64 * vmax(R(1), (R)fabs(zc))) continue;
+                                if ((double)fabs(delta[3]) > e64d * vmax(1.0, fabs(zc))) continue;
                             }
                         }
                         for (int j = 0; j < 4; ++j) {
-                            if (fabs(delta[j]) <= guard[j]) continue;
+                            if ((double)fabs(delta[j]) <= guard[j]) continue;
                             if (delta[j] > R(0)) theta[j] = vmin(theta[j], (hi[j] - cen[j]) / delta[j]);
                             if (delta[j] < R(0)) theta[j] = vmin(theta[j], (lo[j] - cen[j]) / delta[j]);
                         }
@@ -154,8 +188,7 @@ __global__ void __launch_bounds__(256) k_prep(MeshView<R> m, StateView<R> q, con
         }
         w.flags[c] = (uint8_t)flags;
     }
-    double wm = wave_max(hmx);
-    if ((threadIdx.x & 63) == 0 && wm > 0.0) atomicMax(&ctl->hmax_bits[stage], dbits(wm));
+    block_max_to(hmx > 0.0 ? hmx : 0.0, &ctl->hmax_bits[stage]);
 }
 
 // FP64-exact form of the LS-MUSCL preparation (same operation order as the
@@ -244,8 +277,7 @@ static __global__ void __launch_bounds__(256) k_prep_ls64(MeshView<double> m, St
         for (int j = 0; j < 4; ++j) { w.grad[(2 * j) * nc + c] = gx[j]; w.grad[(2 * j + 1) * nc + c] = gy[j]; }
         w.flags[c] = (uint8_t)flags;
     }
-    double wm = wave_max(hmx);
-    if ((threadIdx.x & 63) == 0 && wm > 0.0) atomicMax(&ctl->hmax_bits[stage], dbits(wm));
+    block_max_to(hmx > 0.0 ? hmx : 0.0, &ctl->hmax_bits[stage]);
 }
 
 // face_value (cutcell_solver.f90:8999-9051) for schemes 1 and 5.
@@ -698,8 +730,7 @@ __global__ void __launch_bounds__(256) k_drain(MeshView<R> m, StateView<R> q, Wo
         }
         w.theta[c] = theta;
     }
-    int any = wave_or(drains);
-    if ((threadIdx.x & 63) == 0 && any) atomicOr(&ctl->drain_any[stage], 1);
+    block_or_to(drains, &ctl->drain_any[stage]);
     if (fail) atomicMax(&ctl->failure[stage], FAIL_DRAIN);
 }
 
@@ -816,8 +847,7 @@ __global__ void __launch_bounds__(256) k_gather(MeshView<R> m, StateView<R> in, 
         }
         if (STAGE != 3) { out.h[c] = nh; out.hu[c] = nhu; out.hv[c] = nhv; }
     }
-    double wm = wave_max(ratio);
-    if ((threadIdx.x & 63) == 0 && wm > 0.0) atomicMax(&ctl->peak_bits[STAGE], dbits(wm));
+    block_max_to(ratio > 0.0 ? ratio : 0.0, &ctl->peak_bits[STAGE]);
     if (fail) { atomicMax(&ctl->failure[STAGE], fail); atomicMax(&ctl->fail_cell[STAGE], c + 1); }
 }
 

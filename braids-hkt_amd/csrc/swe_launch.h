@@ -13,8 +13,10 @@ enum : int { KK_PREP = 0, KK_TRACES, KK_FACE, KK_DRAIN, KK_GATHER, KK_LEDGER, KK
 struct LaunchCfg {
     int block = 256;    // threads per workgroup (k_prep/k_face/k_drain/k_gather)
     int remap = 1;      // XCD-aware block remap
-    // When non-null: ev[(slot*KK_COUNT + kind)*2 + {0,1}] bracket each launch.
+    // When non-null: ev[(slot*KK_COUNT + kind)*2 + {0,1}] bracket each launch
+    // whose kind bit is set in ev_mask.
     hipEvent_t *ev = nullptr;
+    int ev_mask = 0;
 };
 
 // mode: 0,1,2 = SSPRK(3,3) stage index (combination + prepare [+ Manning]);

@@ -11,13 +11,15 @@ static inline int grid_for(int n, int block) { return n > 0 ? (n + block - 1) / 
 
 struct EvScope {
     const LaunchCfg &cfg; hipStream_t s; int idx;
-    EvScope(const LaunchCfg &c, hipStream_t st, int slot, int kind) : cfg(c), s(st), idx((slot * KK_COUNT + kind) * 2)
+    bool on;
+    EvScope(const LaunchCfg &c, hipStream_t st, int slot, int kind)
+        : cfg(c), s(st), idx((slot * KK_COUNT + kind) * 2), on(c.ev && ((c.ev_mask >> kind) & 1))
     {
-        if (cfg.ev) (void)hipEventRecord(cfg.ev[idx], s);
+        if (on) (void)hipEventRecord(cfg.ev[idx], s);
     }
     ~EvScope()
     {
-        if (cfg.ev) (void)hipEventRecord(cfg.ev[idx + 1], s);
+        if (on) (void)hipEventRecord(cfg.ev[idx + 1], s);
     }
 };
 

@@ -116,10 +116,10 @@ int braids_mass(braids_solver *s, double *mass);
 
 /* Measurement hooks (no reference counterpart; the reference only has the
  * BRAIDS_PROFILE wall-clock phases, cutcell_solver.f90:16584-16646).
- * braids_profile(s, 1) brackets every kernel launch of later steps with HIP
- * events on the solver's stream; braids_profile_read returns accumulated
- * milliseconds and launch counts for the kernel kinds
- * {prep, traces, face, drain, gather, ledger}. */
+ * braids_profile(s, mask) brackets the launches of later steps whose kind bit
+ * is set in mask (bit k of {prep, traces, face, drain, gather, ledger}; -1 =
+ * all, 0 = off) with HIP events on the solver's stream; braids_profile_read
+ * returns accumulated milliseconds and launch counts per kind. */
 int braids_profile(braids_solver *s, int enable);
 int braids_profile_read(braids_solver *s, double *ms, long long *launches);
 /* Plain device copy of `bytes` (multiple of 16) repeated `repeats` times on
