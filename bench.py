@@ -52,6 +52,28 @@ def algorithmic_bytes(kind, m, precision, scheme, stage_mix=(1, 1, 1)):
     return 0
 
 
+def stage_bytes(m, precision, scheme):
+    """Compulsory HBM bytes of one whole Euler stage, each array touched once:
+    nc*(6 s_q + 8 + 2 s_g + 2 s_g o2) + nf*(8 + 3 s_g + 2 s_g o2), with s_q the
+    conserved-state width (FP64 in both forms here), s_g the geometry width,
+    the bed in FP64 and o2 = 1 for LS-MUSCL."""
+    s_q = 8
+    s_g = 4 if precision == 32 else 8
+    o2 = 1 if scheme == 5 else 0
+    return m.nc * (6 * s_q + 8 + 2 * s_g + 2 * s_g * o2) + m.nf * (8 + 3 * s_g + 2 * s_g * o2)
+
+
+def measured_traffic(workload, precision, kind):
+    """HBM bytes per launch of `kind` from the committed rocprofv3 summary of
+    this workload (FETCH_SIZE and WRITE_SIZE runs calibrated on a copy kernel),
+    or None when no summary exists."""
+    from braids_hip import rocprof
+    path = ROOT / 'profiles' / f'rocprof_{workload}_f{precision}.md'
+    if not path.exists():
+        return None
+    return rocprof.parse_summary(path).get(kind)
+
+
 def build_workload(name):
     from braids_hip import mesh as M
     if name == 'braided_reach_500k':
@@ -147,6 +169,9 @@ def main():
     stream_gbs = 2 * nbytes / (ms.value / 1e3) / 1e9
     del src, dst
 
+    stage_ms = sum(v[0] / v[1] for k, v in warm.items() if k in ('prep', 'traces', 'face', 'drain', 'gather', 'ledger'))
+    sbytes = stage_bytes(m, args.precision, args.scheme)
+    traffic = measured_traffic(args.workload, args.precision, dom)
     line = {
         'metric': METRIC,
         'value': m.nc * acc / el,
@@ -167,10 +192,12 @@ def main():
                    'turbines': 0, 'renumbering': 'hilbert', 'block': args.block, 'xcd_remap': bool(args.remap),
                    'retries': retries, 'state_finite_nonnegative': finite},
         'roofline': {'bound': 'hbm', 'kernel': dom, 'achieved': achieved, 'peak': HBM_PEAK_GBS, 'unit': 'GB/s',
-                     'frac': achieved / HBM_PEAK_GBS, 'traffic': None,
+                     'frac': achieved / HBM_PEAK_GBS, 'traffic': traffic,
                      'algorithmic_bytes_per_launch': abytes, 'avg_launch_us': avg_s * 1e6,
                      'stream_copy_GBs': stream_gbs,
-                     'warmup_kernels_ms_per_launch': {k: v[0] / v[1] for k, v in kinds.items()}},
+                     'warmup_kernels_ms_per_launch': {k: v[0] / v[1] for k, v in kinds.items()},
+                     'stage': {'algorithmic_bytes': sbytes, 'kernels_us': stage_ms * 1e3,
+                               'achieved': sbytes / (stage_ms / 1e3) / 1e9}},
     }
     if not args.no_cpu_baseline:
         print('[bench] cpu baseline (oracle on one host core)', file=sys.stderr, flush=True)

@@ -3,25 +3,27 @@
 #pragma once
 #include "swe_launch.h"
 #include "swe_kernels.h"
+#include <hip/hip_ext.h>
 #include <type_traits>
 
 namespace braids {
 
 static inline int grid_for(int n, int block) { return n > 0 ? (n + block - 1) / block : 1; }
 
-struct EvScope {
-    const LaunchCfg &cfg; hipStream_t s; int idx;
-    bool on;
-    EvScope(const LaunchCfg &c, hipStream_t st, int slot, int kind)
-        : cfg(c), s(st), idx((slot * KK_COUNT + kind) * 2), on(c.ev && ((c.ev_mask >> kind) & 1))
-    {
-        if (on) (void)hipEventRecord(cfg.ev[idx], s);
+// Launch with the kernel's own dispatch timestamps bound to the kind's event
+// pair when that kind is being timed (hipExtLaunchKernel stamps the start and
+// stop events at the kernel's begin and end, with no marker packets between).
+template <class K, class... A>
+static inline void timed_launch(const LaunchCfg &cfg, int slot, int kind, K kernel, dim3 grid, dim3 block,
+                                hipStream_t s, A... args)
+{
+    hipEvent_t a = nullptr, b = nullptr;
+    if (cfg.ev && ((cfg.ev_mask >> kind) & 1)) {
+        const int i = (slot * KK_COUNT + kind) * 2;
+        a = cfg.ev[i]; b = cfg.ev[i + 1];
     }
-    ~EvScope()
-    {
-        if (on) (void)hipEventRecord(cfg.ev[idx + 1], s);
-    }
-};
+    hipExtLaunchKernelGGL(kernel, grid, block, 0, s, a, b, 0, args...);
+}
 
 template <class R, int SCHEME>
 static void launch_prep(const MeshView<R> &m, StateView<R> in, const double *z, Work<R> w, Control *ctl,
@@ -29,20 +31,20 @@ static void launch_prep(const MeshView<R> &m, StateView<R> in, const double *z, 
 {
     const int B = cfg.block;
     if constexpr (std::is_same<R, double>::value && SCHEME == 5)
-        hipLaunchKernelGGL(k_prep_ls64, dim3(grid_for(m.nc, B)), dim3(B), 0, s, m, in, z, w, ctl, slot, cfg.remap);
+        timed_launch(cfg, slot, KK_PREP, k_prep_ls64, dim3(grid_for(m.nc, B)), dim3(B), s, m, in, z, w, ctl, slot, cfg.remap);
     else
-        hipLaunchKernelGGL((k_prep<R, SCHEME>), dim3(grid_for(m.nc, B)), dim3(B), 0, s, m, in, z, w, ctl, slot, cfg.remap);
+        timed_launch(cfg, slot, KK_PREP, k_prep<R, SCHEME>, dim3(grid_for(m.nc, B)), dim3(B), s, m, in, z, w, ctl, slot, cfg.remap);
 }
 
 template <class R, int SCHEME>
 static void launch_face(const MeshView<R> &m, StateView<R> in, const double *z, Work<R> w,
-                        const LaunchCfg &cfg, hipStream_t s)
+                        int slot, const LaunchCfg &cfg, hipStream_t s)
 {
     const int B = cfg.block;
     if constexpr (std::is_same<R, float>::value)
-        hipLaunchKernelGGL((k_face_f32<SCHEME>), dim3(grid_for(m.nf, B)), dim3(B), 0, s, m, in, z, w, cfg.remap);
+        timed_launch(cfg, slot, KK_FACE, k_face_f32<SCHEME>, dim3(grid_for(m.nf, B)), dim3(B), s, m, in, z, w, cfg.remap);
     else
-        hipLaunchKernelGGL((k_face<R, SCHEME>), dim3(grid_for(m.nf, B)), dim3(B), 0, s, m, in, z, w, cfg.remap);
+        timed_launch(cfg, slot, KK_FACE, k_face<R, SCHEME>, dim3(grid_for(m.nf, B)), dim3(B), s, m, in, z, w, cfg.remap);
 }
 
 template <class R, int SCHEME>
@@ -51,27 +53,19 @@ static void stage_kernels(const MeshView<R> &m, StateView<R> in, StateView<R> ba
                           double dt, double t, const LaunchCfg &cfg, hipStream_t s)
 {
     const int B = cfg.block;
-    { EvScope e(cfg, s, mode, KK_PREP); launch_prep<R, SCHEME>(m, in, z, w, ctl, mode, cfg, s); }
-    if (m.nb > 0) {
-        EvScope e(cfg, s, mode, KK_TRACES);
-        hipLaunchKernelGGL((k_traces<R, SCHEME>), dim3(m.nb), dim3(TRACE_BLOCK), 0, s, m, in, z, w, ctl, mode, t);
-    }
-    { EvScope e(cfg, s, mode, KK_FACE); launch_face<R, SCHEME>(m, in, z, w, cfg, s); }
-    { EvScope e(cfg, s, mode, KK_DRAIN);
-      hipLaunchKernelGGL((k_drain<R>), dim3(grid_for(m.nc, B)), dim3(B), 0, s, m, in, w, ctl, mode, dt, cfg.remap); }
-    {
-    EvScope eg(cfg, s, mode, KK_GATHER);
+    const dim3 gc(grid_for(m.nc, B)), blk(B), gb(m.nb), tb(TRACE_BLOCK);
+    launch_prep<R, SCHEME>(m, in, z, w, ctl, mode, cfg, s);
+    if (m.nb > 0) timed_launch(cfg, mode, KK_TRACES, k_traces<R, SCHEME>, gb, tb, s, m, in, z, w, ctl, mode, t);
+    launch_face<R, SCHEME>(m, in, z, w, mode, cfg, s);
+    timed_launch(cfg, mode, KK_DRAIN, k_drain<R>, gc, blk, s, m, in, w, ctl, mode, dt, cfg.remap);
     switch (mode) {
-    case 0: hipLaunchKernelGGL((k_gather<R, 0>), dim3(grid_for(m.nc, B)), dim3(B), 0, s, m, in, base, out, w, ctl, dt, cfg.remap); break;
-    case 1: hipLaunchKernelGGL((k_gather<R, 1>), dim3(grid_for(m.nc, B)), dim3(B), 0, s, m, in, base, out, w, ctl, dt, cfg.remap); break;
-    case 2: hipLaunchKernelGGL((k_gather<R, 2>), dim3(grid_for(m.nc, B)), dim3(B), 0, s, m, in, base, out, w, ctl, dt, cfg.remap); break;
-    default: hipLaunchKernelGGL((k_gather<R, 3>), dim3(grid_for(m.nc, B)), dim3(B), 0, s, m, in, base, out, w, ctl, dt, cfg.remap); break;
+    case 0: timed_launch(cfg, mode, KK_GATHER, k_gather<R, 0>, gc, blk, s, m, in, base, out, w, ctl, dt, cfg.remap); break;
+    case 1: timed_launch(cfg, mode, KK_GATHER, k_gather<R, 1>, gc, blk, s, m, in, base, out, w, ctl, dt, cfg.remap); break;
+    case 2: timed_launch(cfg, mode, KK_GATHER, k_gather<R, 2>, gc, blk, s, m, in, base, out, w, ctl, dt, cfg.remap); break;
+    default: timed_launch(cfg, mode, KK_GATHER, k_gather<R, 3>, gc, blk, s, m, in, base, out, w, ctl, dt, cfg.remap); break;
     }
-    }
-    if (m.nb > 0 && ledger_stage) {
-        EvScope e(cfg, s, mode, KK_LEDGER);
-        hipLaunchKernelGGL((k_ledger<R>), dim3(m.nb), dim3(TRACE_BLOCK), 0, s, m, w, ledger_stage, dt);
-    }
+    if (m.nb > 0 && ledger_stage)
+        timed_launch(cfg, mode, KK_LEDGER, k_ledger<R>, gb, tb, s, m, w, ledger_stage, dt);
 }
 
 template <class R>
@@ -91,7 +85,7 @@ static void rate_kernels(const MeshView<R> &m, StateView<R> in, const double *z,
     launch_prep<R, SCHEME>(m, in, z, w, ctl, 3, cfg, s);
     if (m.nb > 0)
         hipLaunchKernelGGL((k_traces<R, SCHEME>), dim3(m.nb), dim3(TRACE_BLOCK), 0, s, m, in, z, w, ctl, 3, t);
-    launch_face<R, SCHEME>(m, in, z, w, cfg, s);
+    launch_face<R, SCHEME>(m, in, z, w, 3, cfg, s);
     const int n = m.nc > m.nf ? m.nc : m.nf;
     hipLaunchKernelGGL((k_rates<R>), dim3(grid_for(n, B)), dim3(B), 0, s, m, in, w, rate, water, 0);
 }
