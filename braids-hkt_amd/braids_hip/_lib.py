@@ -12,7 +12,10 @@ _P = ctypes.c_void_p
 
 BRAIDS_OK = 0
 BRAIDS_REJECTED = 1
-KERNEL_KINDS = ('prep', 'traces', 'face', 'drain', 'gather', 'ledger')
+KERNEL_KINDS = ('prep', 'traces', 'face', 'drain', 'gather', 'ledger',
+                'rdrag', 'rtheta', 'turbine', 'rupdate')
+TURBINE_DIAG = 16
+RIVER_REASONS = {0: None, 1: 'spatial', 2: 'cfl', 3: 'boundary', 4: 'nonfinite', 5: 'negative', 6: 'turbine'}
 
 
 class BraidsMesh(ctypes.Structure):
@@ -36,6 +39,20 @@ class BraidsStepResult(ctypes.Structure):
                 ('trial_cfl_peak', _D), ('stage_peak', _D * 3)]
 
 
+class BraidsRiverResult(ctypes.Structure):
+    _fields_ = [('accepted', _I), ('reason', _I), ('failed_stage', _I), ('fail_cell', _I),
+                ('frequency_peak', _D)]
+
+
+class BraidsTurbines(ctypes.Structure):
+    _fields_ = [('nrot', _I),
+                ('D', _P), ('top', _P), ('bottom', _P), ('tol', _P), ('min_surface', _P), ('min_bed', _P),
+                ('control', _P), ('ct_const', _P), ('cp_const', _P),
+                ('table_off', _P), ('table_speed', _P), ('table_ct', _P), ('table_cp', _P),
+                ('foot_off', _P), ('foot_cell', _P), ('foot_norm', _P), ('foot_density', _P),
+                ('plane_off', _P), ('plane_cell', _P), ('plane_dx', _P), ('plane_dy', _P), ('plane_inv', _P)]
+
+
 # (name, restype, argtypes) of every exported entry point
 SIGNATURES = {
     'braids_last_error': (ctypes.c_char_p, []),
@@ -53,6 +70,12 @@ SIGNATURES = {
     'braids_native_rates': (_I, [_P, _P, _P, _D, _D, _P, _P, _P]),
     'river_get_face_speed': (_I, [_P, _P]),
     'braids_mass': (_I, [_P, _P]),
+    'braids_river_configure': (_I, [_P, _D, _I, _P]),
+    'braids_river_step': (_I, [_P, _D, _D, ctypes.POINTER(BraidsRiverResult)]),
+    'braids_river_advance': (_I, [_P, _D, _D, _P, _P, _P, _P]),
+    'braids_turbines_set': (_I, [_P, ctypes.POINTER(BraidsTurbines)]),
+    'braids_turbines_read': (_I, [_P, _P, _P]),
+    'braids_turbine_footprint': (_I, [_I, _P, _P, _P, _P, _P, _P, _P]),
     'braids_profile': (_I, [_P, _I]),
     'braids_profile_read': (_I, [_P, _P, _P]),
     'braids_stream_copy': (_I, [_P, _P, ctypes.c_longlong, _P, _I, _P]),

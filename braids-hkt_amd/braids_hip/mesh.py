@@ -68,6 +68,7 @@ class Mesh:
     signs: np.ndarray = None
     boundary_offset: np.ndarray = None
     boundary_faces: np.ndarray = None
+    verts: Optional[np.ndarray] = None       # [6, nc] triangle vertices x0,y0,x1,y1,x2,y2 (CCW)
     cell_perm: Optional[np.ndarray] = None   # new index -> original cell
     face_perm: Optional[np.ndarray] = None   # new index -> original face
 
@@ -138,7 +139,8 @@ class Mesh:
             irregular=self.irregular[cperm].copy(), ii=self.ii[cperm].copy(), jj=self.jj[cperm].copy(),
             nx=self.nx, ny=self.ny, dx=self.dx, dy=self.dy,
             series_offset=self.series_offset, series_time=self.series_time,
-            series_q=self.series_q, series_stage=self.series_stage, name=self.name)
+            series_q=self.series_q, series_stage=self.series_stage, name=self.name,
+            verts=None if self.verts is None else self.verts[:, cperm].copy())
         base_c = self.cell_perm if self.cell_perm is not None else np.arange(self.nc)
         base_f = self.face_perm if self.face_perm is not None else np.arange(self.nf)
         m.cell_perm = base_c[cperm]
@@ -217,6 +219,14 @@ def structured_triangles(nx: int, ny: int, dx: float, dy: float, *,
     cy[1::2] = np.where(slash, y0 + 2 * dy / 3, y0 + 2 * dy / 3)
     nc = 2 * nq
     area = np.full(nc, 0.5 * dx * dy)
+    x1 = x0 + dx; y1 = y0 + dy
+    verts = np.empty((6, nc))
+    # T0: "/" (x0,y0),(x1,y0),(x1,y1)   "\" (x0,y0),(x1,y0),(x0,y1)
+    verts[0, 0::2] = x0; verts[1, 0::2] = y0; verts[2, 0::2] = x1; verts[3, 0::2] = y0
+    verts[4, 0::2] = np.where(slash, x1, x0); verts[5, 0::2] = y1
+    # T1: "/" (x0,y0),(x1,y1),(x0,y1)   "\" (x1,y0),(x1,y1),(x0,y1)
+    verts[0, 1::2] = np.where(slash, x0, x1); verts[1, 1::2] = y0; verts[2, 1::2] = x1; verts[3, 1::2] = y1
+    verts[4, 1::2] = x0; verts[5, 1::2] = y1
     tri0 = 2 * (J * nx + I); tri1 = tri0 + 1
     # which triangle touches each side of the quad
     bottom = tri0
@@ -285,7 +295,7 @@ def structured_triangles(nx: int, ny: int, dx: float, dy: float, *,
                 forcing_q=np.array([float(inflow_q), 0.0]),
                 forcing_stage=np.array([0.0, float(outlet_stage)]),
                 irregular=np.ones(nc, np.int32), ii=ii, jj=jj, nx=nx, ny=ny, dx=dx, dy=dy,
-                series_offset=so, series_time=st, series_q=sq, series_stage=ss, name=name)
+                series_offset=so, series_time=st, series_q=sq, series_stage=ss, name=name, verts=verts)
 
 
 # ---------------------------------------------------------------------------

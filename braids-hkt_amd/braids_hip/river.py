@@ -1,0 +1,60 @@
+"""Python host side of the river SSPRK(2,2) hydraulic integration, mirroring
+the reference's RiverExplicit owner (src/braids_core/sediment/river_explicit.py).
+
+The reference's controller drives one spatial rate per stage through the
+native library and four WaveEngine operators, with a host round-trip per
+stage (the CFL frequency reduction and the limiter status).  Here a substep is
+one sequence of device launches followed by one readback of the rejection
+flags; the substep and hint logic of RiverExplicit.advance runs in
+braids_river_advance."""
+import numpy as np
+
+from .solver import HydraulicSolver, TrialRejected
+
+METHOD = 'explicit_fwave_ssprk22_river_v1'
+CFL_SAFETY = .98
+MAX_CFL = 1.
+
+
+class RiverExplicit:
+    """RiverExplicit(solver, cfl=.5): SSPRK(2,2) with Strang drag around each
+    substep, the conservative inventory limiter and the turbine sink."""
+
+    def __init__(self, solver: HydraulicSolver, cfl=.5, drag_mode=2, roughness=None):
+        if isinstance(cfl, bool) or not np.isfinite(cfl) or not 0 < cfl <= MAX_CFL:
+            raise ValueError('Explicit hydraulic face-sum CFL must be in (0, 1]')
+        self.solver = solver
+        self.cfl = float(cfl)
+        solver.river_configure(self.cfl, drag_mode, roughness)
+        self.step_hint = None
+        self.identity = dict(method=METHOD, cfl=self.cfl,
+                             friction='Strang drag around each hydraulic substep; trial-local',
+                             cfl_definition='dt * max_cell(sum_face(length * wave_speed) / actual_area)',
+                             clock_policy=dict(predictor_safety=CFL_SAFETY, maximum_cfl=MAX_CFL,
+                                               accepted_growth_limit=2., rejected_growth_limit=.5),
+                             explicit_fallback=False)
+
+    def advance(self, dt, time):
+        """Advance the resident state over [time, time + dt] in substeps;
+        returns the reference's substep report (the state stays on the device)."""
+        s = self.solver
+        s.river_hint = self.step_hint
+        n, rejections, cfl_max = s.river_advance(dt, time)
+        self.step_hint = s.river_hint
+        return dict(integrator=METHOD, hydraulic_substeps=n, hydraulic_rejections=rejections,
+                    accepted_cfl_max=cfl_max, hydraulic_next_substep_hint_s=self.step_hint, explicit_fallback=False)
+
+    def single(self, dt, time):
+        """One substep (RiverExplicit._single); raises TrialRejected."""
+        return self.solver.river_step(dt, time)
+
+
+def install(solver: HydraulicSolver, cfl=.5, **kw):
+    """river_explicit.install: a controller on a fresh solver."""
+    if solver.time:
+        raise ValueError('Install explicit controller on a fresh owner')
+    solver.midpoint = RiverExplicit(solver, cfl, **kw)
+    return solver
+
+
+__all__ = ['RiverExplicit', 'install', 'TrialRejected']

@@ -148,6 +148,42 @@ class HydraulicSolver:
             raise TrialRejected(_lib.last_error())
         return acc.value, ret.value
 
+    # -- river SSPRK(2,2) (RiverExplicit) -------------------------------------
+    def river_configure(self, cfl=.5, drag_mode=2, roughness=None):
+        """Face-sum CFL limit and the Strang drag (mode as river_roughness.selected;
+        roughness [nc] per cell, default the mesh Manning n)."""
+        r = None if roughness is None else np.ascontiguousarray(roughness, dtype=np.float64).reshape(self.nc)
+        _lib.check(self.L.braids_river_configure(self.h, float(cfl), int(drag_mode), _p(r)), 'braids_river_configure')
+        self.river_cfl = float(cfl)
+
+    def river_step(self, dt, t=None):
+        """One SSPRK(2,2) substep; raises TrialRejected (state unchanged)."""
+        r = _lib.BraidsRiverResult()
+        rc = _lib.check(self.L.braids_river_step(self.h, float(dt), float(self.time if t is None else t),
+                                                 ctypes.byref(r)), 'braids_river_step')
+        info = dict(accepted=bool(r.accepted), reason=_lib.RIVER_REASONS[r.reason], failed_stage=r.failed_stage,
+                    fail_cell=r.fail_cell, frequency=r.frequency_peak)
+        if rc == _lib.BRAIDS_REJECTED:
+            err = TrialRejected(f'river stage {r.failed_stage} rejected ({info["reason"]}, cell {r.fail_cell}, '
+                                f'CFL {dt * r.frequency_peak:g})')
+            err.info = info
+            raise err
+        return info
+
+    def river_advance(self, dt, time=None):
+        """RiverExplicit.advance over the outer interval dt; returns
+        (substeps, rejections, accepted CFL max)."""
+        t0 = self.time if time is None else float(time)
+        hint = ctypes.c_double(self.river_hint if getattr(self, 'river_hint', None) else 0.0)
+        n = ctypes.c_int(); rej = ctypes.c_int(); cmax = ctypes.c_double()
+        rc = _lib.check(self.L.braids_river_advance(self.h, float(dt), t0, ctypes.byref(hint), ctypes.byref(n),
+                                                    ctypes.byref(rej), ctypes.byref(cmax)), 'braids_river_advance')
+        self.river_hint = hint.value
+        if rc == _lib.BRAIDS_REJECTED:
+            raise TrialRejected(_lib.last_error())
+        self.time = t0 + float(dt)
+        return n.value, rej.value, cmax.value
+
     # -- measurement --------------------------------------------------------
     def profile(self, kinds=True):
         """kinds: True (all kernel kinds), False (off) or an iterable of kind names."""
@@ -160,7 +196,8 @@ class HydraulicSolver:
         _lib.check(self.L.braids_profile(self.h, int(mask)), 'braids_profile')
 
     def profile_read(self):
-        ms = (ctypes.c_double * 6)(); n = (ctypes.c_longlong * 6)()
+        nk = len(_lib.KERNEL_KINDS)
+        ms = (ctypes.c_double * nk)(); n = (ctypes.c_longlong * nk)()
         _lib.check(self.L.braids_profile_read(self.h, ctypes.byref(ms), ctypes.byref(n)), 'braids_profile_read')
         return {k: (ms[i], n[i]) for i, k in enumerate(_lib.KERNEL_KINDS)}
 

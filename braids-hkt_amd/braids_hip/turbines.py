@@ -1,0 +1,302 @@
+"""Host side of the Thetis turbine sink: rotor validation, footprint weights
+(computed on the GPU), rotor-plane sample rows and the device upload.
+
+Mirrors the construction of the reference rotor array:
+  validate_control                river_turbine_thetis.py:95-163 (constant_ct, thetis_speed_table)
+  CoveredRotor.__init__           river_turbine_flow.py:117-135 (heights, clearances,
+                                  tolerance, plane rows, unresolved-footprint check)
+  polygon_integrals               river_turbine_geometry.py:42-71 (footprint weights and
+                                  the coverage / bump-integral checks)
+  ThetisRotor.__init__            river_turbine_thetis.py:244-261 (neutral axis-aligned
+                                  D x D bump, density = weights / ((D/2)^2 1.45661))
+  prepared_geometry inverse       bend_methods_v1.py:114-124 (pinv of the full LS metric)
+The cell polygons are the mesh triangles; the clipped polygon is fan
+triangulated (the reference uses a constrained Delaunay triangulation of the
+same polygon), so individual weights agree with the reference's to the
+quadrature convergence tolerance (2e-12 D^2) and the cell assignment
+(weight > 0) is exact.
+"""
+import ctypes
+import math
+from dataclasses import dataclass
+
+import numpy as np
+
+from . import _lib
+
+CONTROL_MODES = {'constant_ct': 0, 'thetis_speed_table': 1}
+ORDERS = (16, 32, 64, 128)
+BUMP_NORM = 1.45661
+
+
+def validate_control(control):
+    """The constant and speed-table controls of river_turbine_thetis.py:95-163."""
+    if not isinstance(control, dict):
+        raise ValueError('Choose Thetis constant coefficients or a speed / Ct / Cp table.')
+    c = dict(control)
+    mode = c.get('mode')
+    if mode == 'constant_ct':
+        if set(c) != {'mode', 'thrust_coefficient', 'power_coefficient'}:
+            raise ValueError('Constant input requires Ct and Cp (null or zero selects Thetis ideal-disk Cp).')
+        pairs = [(c['thrust_coefficient'], c['power_coefficient'])]
+    elif mode == 'thetis_speed_table':
+        if set(c) != {'mode', 'model_name', 'performance'} or not isinstance(c['model_name'], str) \
+                or not c['model_name'].strip():
+            raise ValueError('Name the turbine speed table and provide speed, Ct and Cp rows.')
+        rows = c['performance']
+        if not isinstance(rows, list) or len(rows) < 2:
+            raise ValueError('Thetis needs at least two speed / Ct / Cp rows.')
+        speeds = []
+        for r in rows:
+            if not isinstance(r, dict) or set(r) != {'speed_m_s', 'thrust_coefficient', 'power_coefficient'}:
+                raise ValueError('Each Thetis table row requires speed_m_s, thrust_coefficient and power_coefficient.')
+            u = r['speed_m_s']
+            if type(u) not in (int, float) or not math.isfinite(u) or u < 0:
+                raise ValueError('Table speeds must be finite and nonnegative.')
+            speeds.append(u)
+        if any(b <= a for a, b in zip(speeds, speeds[1:])):
+            raise ValueError('Table speeds must be strictly increasing; remove duplicates and sort the rows.')
+        pairs = [(r['thrust_coefficient'], r['power_coefficient']) for r in rows]
+    else:
+        raise ValueError(f'Unsupported turbine control mode {mode!r} (constant_ct and thetis_speed_table are built)')
+    for ct, cp in pairs:
+        if type(ct) not in (int, float) or not math.isfinite(ct) or not 0 <= ct < 1:
+            raise ValueError('Ct must be finite, with 0 <= Ct < 1.')
+        if cp is None and mode == 'constant_ct':
+            continue
+        if type(cp) not in (int, float) or not math.isfinite(cp) or not 0 <= cp <= 1:
+            raise ValueError('Cp must be finite and between 0 and 1.')
+    return c
+
+
+def quadrature_rules():
+    """Gauss-Legendre rules on [0, 1] (the reference's _quadrature,
+    river_turbine_geometry.py:24-27), concatenated in ORDERS order."""
+    from numpy.polynomial.legendre import leggauss
+    s_all, w_all = [], []
+    for order in ORDERS:
+        x, w = leggauss(order)
+        s_all.append((x + 1) / 2)
+        w_all.append(w / 2)
+    return np.concatenate(s_all), np.concatenate(w_all)
+
+
+def bump_integral(D):
+    """Independent bump integral (integral of exp(1 - 1/(1-s^2)) over (-1, 1)
+    times D/2) squared, river_turbine_geometry.py:67-69."""
+    try:
+        from scipy.integrate import quad
+        one = quad(lambda s: math.exp(1 - 1 / (1 - s * s)), -1., 1., epsabs=1e-13, epsrel=1e-13)[0] * D / 2
+    except ImportError:        # pragma: no cover - scipy is part of the image
+        from numpy.polynomial.legendre import leggauss
+        x, w = leggauss(400)
+        one = float(np.sum(w * np.exp(1 - 1 / (1 - x * x)))) * D / 2
+    return one * one
+
+
+def footprint(mesh, X, Y, D):
+    """Footprint weights of one rotor on the GPU: (cells, weights, covered
+    area), cells ascending with weight > 0; raises like polygon_integrals."""
+    v = mesh.verts
+    xmin = np.minimum(np.minimum(v[0], v[2]), v[4]); xmax = np.maximum(np.maximum(v[0], v[2]), v[4])
+    ymin = np.minimum(np.minimum(v[1], v[3]), v[5]); ymax = np.maximum(np.maximum(v[1], v[3]), v[5])
+    h = D / 2
+    cand = np.flatnonzero((xmax >= X - h) & (xmin <= X + h) & (ymax >= Y - h) & (ymin <= Y + h))
+    tri = np.ascontiguousarray((v[:, cand] - np.array([X, Y, X, Y, X, Y])[:, None]).T)
+    n = len(cand)
+    Dp = np.full(n, float(D))
+    nodes, weights = quadrature_rules()
+    value = np.zeros(n); clipped = np.zeros(n); order = np.zeros(n, np.int32)
+    L = _lib.load()
+    P = lambda a: ctypes.c_void_p(a.ctypes.data)
+    _lib.check(L.braids_turbine_footprint(n, P(tri), P(Dp), P(nodes), P(weights), P(value), P(clipped), P(order)),
+               'braids_turbine_footprint')
+    if np.any(order < 0):
+        raise ValueError('Polygon bump quadrature did not converge')
+    area = float(np.sum(clipped[clipped > 0]))
+    if not math.isclose(area, D * D, rel_tol=1e-10, abs_tol=1e-13):
+        raise ValueError('Turbine footprint intersects a wall or is not covered exactly')
+    expected = bump_integral(D)
+    if abs(value.sum() - expected) > 1e-9 * expected:
+        raise ValueError('Polygon partition changes the physical bump integral')
+    keep = value > 0
+    return cand[keep], value[keep], area
+
+
+def plane_rows(mesh, X, Y, D):
+    """Rotor-plane samples (CoveredRotor.__init__, river_turbine_flow.py:126-133):
+    the segment through the centre along the tangent of the neutral normal
+    (1, 0), i.e. x = X, y in [Y - D/2, Y + D/2], intersected with each cell;
+    both end points of every piece longer than 1e-12 give a row."""
+    v = mesh.verts
+    y0, y1 = Y - D / 2, Y + D / 2
+    xmin = np.minimum(np.minimum(v[0], v[2]), v[4]); xmax = np.maximum(np.maximum(v[0], v[2]), v[4])
+    ymin = np.minimum(np.minimum(v[1], v[3]), v[5]); ymax = np.maximum(np.maximum(v[1], v[3]), v[5])
+    cand = np.flatnonzero((xmin <= X) & (xmax >= X) & (ymax >= y0) & (ymin <= y1))
+    rows, dx, dy = [], [], []
+    for i in cand:
+        P = v[:, i].reshape(3, 2)
+        lo, hi = 0.0, 1.0                      # parameter along (X, y0) -> (X, y1)
+        ok = True
+        for e in range(3):
+            a, b = P[e], P[(e + 1) % 3]
+            # inside of a CCW edge: cross(b - a, p - a) >= 0
+            c0 = (b[0] - a[0]) * (y0 - a[1]) - (b[1] - a[1]) * (X - a[0])
+            c1 = (b[0] - a[0]) * (y1 - a[1]) - (b[1] - a[1]) * (X - a[0])
+            if c0 < 0 and c1 < 0:
+                ok = False; break
+            if c0 < 0 or c1 < 0:
+                t = c0 / (c0 - c1)
+                if c0 < 0: lo = max(lo, t)
+                else: hi = min(hi, t)
+        if not ok or hi - lo <= 0:
+            continue
+        ya, yb = y0 + lo * (y1 - y0), y0 + hi * (y1 - y0)
+        if yb - ya <= 1e-12:
+            continue
+        for yy in (ya, yb):
+            rows.append(int(i)); dx.append(X - mesh.xy[0, i]); dy.append(yy - mesh.xy[1, i])
+    if not rows:
+        raise ValueError('Rotor plane is not represented')
+    return np.array(rows, np.int32), np.array(dx), np.array(dy)
+
+
+def ls_inverse(mesh, i):
+    """pinv of the full LS metric of cell i (prepared_geometry,
+    bend_methods_v1.py:114-124): sum over neighbours of outer(d, d)/|d|^2."""
+    a = np.zeros((2, 2))
+    for k in range(mesh.offset[i], mesh.offset[i + 1]):
+        f = mesh.faces_csr[k]
+        j = mesh.neighbour[f] if mesh.owner[f] == i else mesh.owner[f]
+        if j < 0:
+            continue
+        d = mesh.xy[:, j] - mesh.xy[:, i]
+        rr = d @ d
+        if rr <= 0:
+            raise ValueError('Coincident cell centres')
+        a += np.outer(d, d) / rr
+    p = np.linalg.pinv(a)
+    return p[0, 0], p[0, 1], p[1, 1]
+
+
+@dataclass
+class Rotor:
+    name: str
+    X: float
+    Y: float
+    D: float
+    top: float
+    bottom: float
+    tol: float
+    min_surface: float
+    min_bed: float
+    control: dict
+    cells: np.ndarray
+    weights: np.ndarray
+    plane_cell: np.ndarray
+    plane_dx: np.ndarray
+    plane_dy: np.ndarray
+    plane_inv: np.ndarray
+
+
+def make_rotor(mesh, c):
+    """One Thetis rotor from a turbine row (keys: name, X, Y, D,
+    hub_elevation_m, minimum_bed_clearance_m, minimum_surface_clearance_m,
+    control)."""
+    if 'hub_elevation_m' not in c or not math.isfinite(c['hub_elevation_m']):
+        raise ValueError('Explicit fixed hub elevation required')
+    for name in ['minimum_bed_clearance_m', 'minimum_surface_clearance_m']:
+        if name not in c or isinstance(c[name], bool) or not math.isfinite(c[name]) or c[name] < 0:
+            raise ValueError('Explicit nonnegative turbine clearance limits required')
+    control = validate_control(c['control'])
+    D = float(c['D'])
+    top = c['hub_elevation_m'] + D / 2
+    bottom = c['hub_elevation_m'] - D / 2
+    cells, weights, _ = footprint(mesh, c['X'], c['Y'], D)
+    v = mesh.verts[:, cells]
+    ys = v[[1, 3, 5]]
+    if np.any(np.ptp(ys, axis=0) > D * (1 + 1e-9)):
+        raise ValueError('Merged turbine footprint is unresolved across the rotor; reduce the smallest mesh size '
+                         'or move the turbine away from nearby constraints.')
+    pc, pdx, pdy = plane_rows(mesh, c['X'], c['Y'], D)
+    inv = np.array([ls_inverse(mesh, i) for i in pc]).reshape(-1)
+    tol = max(1e-9, 64 * np.finfo(float).eps * max(1., abs(top), abs(bottom)))
+    return Rotor(c.get('name', ''), float(c['X']), float(c['Y']), D, top, bottom, tol,
+                 float(c['minimum_surface_clearance_m']), float(c['minimum_bed_clearance_m']), control,
+                 cells, weights, pc, pdx, pdy, inv)
+
+
+class ThetisArray:
+    """A rotor array ready for upload (ThetisArray, river_turbine_thetis.py:300-362)."""
+
+    def __init__(self, mesh, rows):
+        self.rotors = [make_rotor(mesh, c) for c in rows]
+
+    def arrays(self):
+        R = self.rotors
+        n = len(R)
+        a = {}
+        a['D'] = np.array([r.D for r in R]); a['top'] = np.array([r.top for r in R])
+        a['bottom'] = np.array([r.bottom for r in R]); a['tol'] = np.array([r.tol for r in R])
+        a['min_surface'] = np.array([r.min_surface for r in R]); a['min_bed'] = np.array([r.min_bed for r in R])
+        a['control'] = np.array([CONTROL_MODES[r.control['mode']] for r in R], np.int32)
+        a['ct_const'] = np.array([r.control.get('thrust_coefficient') or 0. for r in R])
+        a['cp_const'] = np.array([(r.control.get('power_coefficient') or 0.) if r.control['mode'] == 'constant_ct'
+                                  else 0. for r in R])
+        speed, ct, cp, toff = [], [], [], [0]
+        for r in R:
+            rows = r.control.get('performance', []) if r.control['mode'] == 'thetis_speed_table' else []
+            speed += [float(x['speed_m_s']) for x in rows]
+            ct += [float(x['thrust_coefficient']) for x in rows]
+            cp += [float(x['power_coefficient']) for x in rows]
+            toff.append(len(speed))
+        a['table_off'] = np.array(toff, np.int32)
+        a['table_speed'] = np.array(speed + [0.]); a['table_ct'] = np.array(ct + [0.]); a['table_cp'] = np.array(cp + [0.])
+        a['foot_off'] = np.cumsum([0] + [len(r.cells) for r in R]).astype(np.int32)
+        a['foot_cell'] = np.concatenate([r.cells for r in R]).astype(np.int32)
+        a['foot_norm'] = np.concatenate([r.weights / r.weights.sum() for r in R])
+        a['foot_density'] = np.concatenate([r.weights / ((r.D / 2) ** 2 * BUMP_NORM) for r in R])
+        a['plane_off'] = np.cumsum([0] + [len(r.plane_cell) for r in R]).astype(np.int32)
+        a['plane_cell'] = np.concatenate([r.plane_cell for r in R]).astype(np.int32)
+        a['plane_dx'] = np.concatenate([r.plane_dx for r in R])
+        a['plane_dy'] = np.concatenate([r.plane_dy for r in R])
+        a['plane_inv'] = np.concatenate([r.plane_inv for r in R])
+        return n, a
+
+    def install(self, solver):
+        """Upload to a HydraulicSolver (its add_momentum then carries the sink)."""
+        n, a = self.arrays()
+        keep = {k: np.ascontiguousarray(v) for k, v in a.items()}
+        t = _lib.BraidsTurbines()
+        t.nrot = n
+        for k, v in keep.items():
+            setattr(t, k, v.ctypes.data)
+        _lib.check(solver.L.braids_turbines_set(solver.h, ctypes.byref(t)), 'braids_turbines_set')
+        solver.turbines = self
+        return self
+
+
+REASONS = ('covered', 'insufficient_depth', 'bed_intersection', 'insufficient_surface_clearance',
+           'insufficient_bed_clearance', 'dry_numerical_support', 'reverse_flow', 'no_incoming_flow',
+           'below_cut_in_speed')
+
+
+def diagnostics(solver):
+    """Per-rotor diagnostics of the last accepted substep's second stage and
+    the accumulated useful energy per unit density (m^5/s^2)."""
+    rot = solver.turbines.rotors
+    n = len(rot)
+    d = np.zeros(2 * n * _lib.TURBINE_DIAG); e = np.zeros(max(n, 1))
+    _lib.check(solver.L.braids_turbines_read(solver.h, ctypes.c_void_p(d.ctypes.data), ctypes.c_void_p(e.ctypes.data)),
+               'braids_turbines_read')
+    d = d.reshape(2, n, _lib.TURBINE_DIAG)
+    out = []
+    for a, r in enumerate(rot):
+        x = d[1, a]
+        out.append(dict(turbine_id=r.name, active=bool(x[0]), inactive_reason=None if x[0] else REASONS[int(x[1])],
+                        local_velocity_m_s=x[4], mean_depth_m=x[5], force_per_density_m4_s2=x[6],
+                        hydraulic_power_removed_per_density_m5_s3=x[7], useful_power_per_density_m5_s3=x[8],
+                        upstream_velocity_m_s=x[9], thrust_coefficient=x[10], power_coefficient=x[11],
+                        minimum_coverage_margin_m=x[12], minimum_bed_clearance_m=x[13],
+                        energy_per_density_m5_s2=e[a]))
+    return out, d

@@ -93,7 +93,16 @@ struct SolverBase {
     virtual int native_euler(const double *q, const double *z, double dt, double t, double *qout, double *water, int *code) = 0;
     virtual int native_rates(const double *q, const double *z, double dt, double t, double *rate, double *water, int *code) = 0;
     virtual int face_speed(double *speed) = 0;
+    virtual int river_configure(double cfl, int drag_mode, const double *drag_values) = 0;
+    virtual int river_step(double dt, double t, braids_river_result *res) = 0;
+    virtual int turbines_set(const braids_turbines *t) = 0;
     hipStream_t stream = nullptr;
+    // river path
+    double river_cfl = 0.5;
+    std::vector<double> manning_host, xy_host;
+    int nrot = 0;
+    std::vector<double> energy, energy_comp;   // per rotor, compensated (Neumaier)
+    std::vector<double> diag_last;             // [2][nrot][TURBINE_DIAG] of the last accepted substep
     int nc = 0, nf = 0, nb = 0;
     double cfl_target = 0.9, dtmax = 1.0;
     std::vector<double> ledger;        // accepted inward water per boundary
@@ -119,11 +128,16 @@ struct SolverBase {
         set_events(on ? events.data() : nullptr, prof_mask);
         return BRAIDS_OK;
     }
-    void collect(int slots)
+    void collect(int slots, bool river = false)
     {
         if (events.empty()) return;
+        static const bool ssp[KK_COUNT] = {true, true, true, true, true, true, false, false, false, false};
+        static const bool riv[KK_COUNT] = {true, true, true, false, false, false, true, true, true, true};
         for (int slot = 0; slot < slots; ++slot)
             for (int k = 0; k < KK_COUNT; ++k) {
+                if (!(river ? riv[k] : ssp[k])) continue;
+                if (river && k == KK_RDRAG && slot > 0) continue;
+                if (river && k == KK_TURBINE && nrot == 0) continue;
                 if ((k == KK_TRACES || k == KK_LEDGER) && nb == 0) continue;
                 if (!((prof_mask >> k) & 1)) continue;
                 float ms = 0;
@@ -148,11 +162,15 @@ struct Solver : SolverBase {
     size_t block_bytes = 0;
     LaunchCfg cfg;
     double *qscratch = nullptr;        // [3*nc] double for conversions
+    RiverBuffers rb;
+    TurbineView tv{};
+    double *diag_host = nullptr;       // pinned [2][nrot][TURBINE_DIAG]
     void set_events(hipEvent_t *ev, int mask) override { cfg.ev = ev; cfg.ev_mask = mask; profiling = ev != nullptr; }
 
     ~Solver() override
     {
         if (host_block) (void)hipHostFree(host_block);
+        if (diag_host) (void)hipHostFree(diag_host);
         for (void *p : allocs) (void)hipFree(p);
         for (auto e : events) (void)hipEventDestroy(e);
     }
@@ -257,6 +275,8 @@ struct Solver : SolverBase {
             if ((rc = upload(m.series_stage, vec(mesh->series_stage, ns)))) return rc;
         }
         area.assign(mesh->area, mesh->area + nc);
+        manning_host.assign(mesh->manning, mesh->manning + nc);
+        xy_host.assign(mesh->xy, mesh->xy + 2 * (size_t)nc);
         for (auto &s : st) {
             if ((rc = alloc(s.h, nc))) return rc;
             if ((rc = alloc(s.hu, nc))) return rc;
@@ -348,6 +368,172 @@ struct Solver : SolverBase {
         std::swap(st[0], st[3]);
         for (int b = 0; b < nb; ++b)
             ledger[b] = ledger[b] + (hl[b] + hl[(nb + 1) + b]) / 6.0 + 2.0 * hl[2 * (nb + 1) + b] / 3.0;
+        res->accepted = 1;
+        return BRAIDS_OK;
+    }
+
+    // -- river SSPRK(2,2) ----------------------------------------------------
+    int river_buffers()
+    {
+        if (rb.theta) return BRAIDS_OK;
+        int rc = 0;
+        if ((rc = alloc(rb.theta, nc))) return rc;
+        if ((rc = alloc(rb.wlim[0], nf))) return rc;
+        if ((rc = alloc(rb.wlim[1], nf))) return rc;
+        rb.ledger = ledger_dev;
+        HIP_TRY(hipMemset(rb.wlim[0], 0, nf * sizeof(double)));
+        HIP_TRY(hipMemset(rb.wlim[1], 0, nf * sizeof(double)));
+        if (!rb.drag.values) return river_configure(river_cfl, 2, nullptr);
+        return BRAIDS_OK;
+    }
+    int river_configure(double cfl, int drag_mode, const double *drag_values) override
+    {
+        river_cfl = cfl;
+        int rc = 0;
+        if (!rb.theta && (rc = river_buffers())) return rc;
+        double *v = const_cast<double *>(rb.drag.values);
+        if (!v) { if ((rc = alloc(v, nc))) return rc; }
+        const double *src = drag_values ? drag_values : manning_host.data();
+        HIP_TRY(hipMemcpy(v, src, nc * sizeof(double), hipMemcpyHostToDevice));
+        rb.drag.values = v;
+        rb.drag.mode = drag_mode;
+        return BRAIDS_OK;
+    }
+    int turbines_set(const braids_turbines *t) override
+    {
+        int rc = 0;
+        if (!rb.theta && (rc = river_buffers())) return rc;
+        if (tv.nrot > 0 || nrot > 0) { set_error("turbines are already installed on this solver"); return BRAIDS_EINVAL; }
+        const int n = t->nrot;
+        if (n <= 0) return BRAIDS_OK;
+        const int nslot = t->foot_off[n], nplane = t->plane_off[n], ntab = t->table_off[n];
+        auto dv = [](const double *p, size_t k) { return std::vector<double>(p, p + k); };
+        auto iv = [](const int *p, size_t k) { return std::vector<int>(p, p + k); };
+        for (int k = 0; k < nslot; ++k)
+            if (t->foot_cell[k] < 0 || t->foot_cell[k] >= nc) { set_error("footprint cell out of range"); return BRAIDS_EINVAL; }
+        for (int k = 0; k < nplane; ++k)
+            if (t->plane_cell[k] < 0 || t->plane_cell[k] >= nc) { set_error("plane cell out of range"); return BRAIDS_EINVAL; }
+        for (int a = 0; a < n; ++a) {
+            if (t->foot_off[a + 1] <= t->foot_off[a] || t->plane_off[a + 1] <= t->plane_off[a]) {
+                set_error("rotor without footprint or plane rows"); return BRAIDS_EINVAL;
+            }
+            if (t->control[a] == 1 && t->table_off[a + 1] - t->table_off[a] < 2) { set_error("speed table needs two rows"); return BRAIDS_EINVAL; }
+        }
+        // cells -> footprint slots in rotor order
+        std::vector<std::vector<int>> by_cell;
+        std::vector<int> cells(t->foot_cell, t->foot_cell + nslot);
+        std::vector<int> uniq = cells;
+        std::sort(uniq.begin(), uniq.end());
+        uniq.erase(std::unique(uniq.begin(), uniq.end()), uniq.end());
+        std::vector<int> uoff(uniq.size() + 1, 0), uslot(nslot);
+        for (int k = 0; k < nslot; ++k) {
+            int u = (int)(std::lower_bound(uniq.begin(), uniq.end(), cells[k]) - uniq.begin());
+            ++uoff[u + 1];
+        }
+        for (size_t u = 0; u < uniq.size(); ++u) uoff[u + 1] += uoff[u];
+        std::vector<int> fill(uoff.begin(), uoff.end() - 1);
+        for (int k = 0; k < nslot; ++k) {     // slots are rotor-major, so each list is in rotor order
+            int u = (int)(std::lower_bound(uniq.begin(), uniq.end(), cells[k]) - uniq.begin());
+            uslot[fill[u]++] = k;
+        }
+        std::vector<uint8_t> mask(nc, 0);
+        for (int c : uniq) mask[c] = 1;
+        TurbineView T{};
+        T.nrot = n; T.nslot = nslot; T.nunion = (int)uniq.size();
+        if ((rc = upload(T.D, dv(t->D, n)))) return rc;
+        if ((rc = upload(T.top, dv(t->top, n)))) return rc;
+        if ((rc = upload(T.bottom, dv(t->bottom, n)))) return rc;
+        if ((rc = upload(T.tol, dv(t->tol, n)))) return rc;
+        if ((rc = upload(T.min_surface, dv(t->min_surface, n)))) return rc;
+        if ((rc = upload(T.min_bed, dv(t->min_bed, n)))) return rc;
+        if ((rc = upload(T.ct_const, dv(t->ct_const, n)))) return rc;
+        if ((rc = upload(T.cp_const, dv(t->cp_const, n)))) return rc;
+        if ((rc = upload(T.control, iv(t->control, n)))) return rc;
+        if ((rc = upload(T.table_off, iv(t->table_off, n + 1)))) return rc;
+        if ((rc = upload(T.table_speed, dv(t->table_speed, ntab)))) return rc;
+        if ((rc = upload(T.table_ct, dv(t->table_ct, ntab)))) return rc;
+        if ((rc = upload(T.table_cp, dv(t->table_cp, ntab)))) return rc;
+        if ((rc = upload(T.foot_off, iv(t->foot_off, n + 1)))) return rc;
+        if ((rc = upload(T.foot_cell, cells))) return rc;
+        if ((rc = upload(T.foot_norm, dv(t->foot_norm, nslot)))) return rc;
+        if ((rc = upload(T.foot_density, dv(t->foot_density, nslot)))) return rc;
+        if ((rc = upload(T.plane_off, iv(t->plane_off, n + 1)))) return rc;
+        if ((rc = upload(T.plane_cell, iv(t->plane_cell, nplane)))) return rc;
+        if ((rc = upload(T.plane_dx, dv(t->plane_dx, nplane)))) return rc;
+        if ((rc = upload(T.plane_dy, dv(t->plane_dy, nplane)))) return rc;
+        if ((rc = upload(T.plane_inv, dv(t->plane_inv, 3 * (size_t)nplane)))) return rc;
+        if ((rc = upload(T.union_cell, uniq))) return rc;
+        if ((rc = upload(T.union_off, uoff))) return rc;
+        if ((rc = upload(T.union_slot, uslot))) return rc;
+        if ((rc = upload(T.xy, xy_host))) return rc;
+        if ((rc = upload(T.tmask, mask))) return rc;
+        if ((rc = alloc(T.kslot, nslot))) return rc;
+        if ((rc = alloc(T.diag, 2 * (size_t)n * TURBINE_DIAG))) return rc;
+        if ((rc = alloc(T.tsink, 2 * (size_t)nc))) return rc;
+        HIP_TRY(hipMemset(T.tsink, 0, 2 * (size_t)nc * sizeof(double)));
+        HIP_TRY(hipMemset(T.diag, 0, 2 * (size_t)n * TURBINE_DIAG * sizeof(double)));
+        HIP_TRY(hipHostMalloc(&diag_host, 2 * (size_t)n * TURBINE_DIAG * sizeof(double), hipHostMallocDefault));
+        tv = T;
+        nrot = n;
+        energy.assign(n, 0.0); energy_comp.assign(n, 0.0);
+        diag_last.assign(2 * (size_t)n * TURBINE_DIAG, 0.0);
+        return BRAIDS_OK;
+    }
+    int river_step(double dt, double t, braids_river_result *res) override
+    {
+        int rc = 0;
+        if (!rb.theta && (rc = river_buffers())) return rc;
+        zero_control();
+        run_river_substep<R>(m, st[0], st[1], st[2], st[3], z, w, ctl, rb, nrot > 0 ? &tv : nullptr, dt, t, cfg, stream);
+        HIP_TRY(hipGetLastError());
+        HIP_TRY(hipMemcpyAsync(host_block, ctl, block_bytes, hipMemcpyDeviceToHost, stream));
+        if (nrot > 0)
+            HIP_TRY(hipMemcpyAsync(diag_host, tv.diag, 2 * (size_t)nrot * TURBINE_DIAG * sizeof(double),
+                                   hipMemcpyDeviceToHost, stream));
+        HIP_TRY(hipStreamSynchronize(stream));
+        if (profiling) collect(2, true);
+        const Control *hc = (const Control *)host_block;
+        const double *hl = (const double *)((const char *)host_block + sizeof(Control));
+        res->accepted = 0; res->reason = 0; res->failed_stage = -1; res->fail_cell = -1; res->frequency_peak = 0;
+        double peak = 0;
+        for (int k = 0; k < 2; ++k) {
+            const int code = hc->failure[k];
+            int reason = 0;
+            if (code >= FAIL_TRACE) reason = BRAIDS_RIVER_SPATIAL;
+            else if (code == RFAIL_BOUNDARY) reason = BRAIDS_RIVER_BOUNDARY;
+            else {
+                double pk; unsigned long long b = hc->peak_bits[k];
+                memcpy(&pk, &b, 8);
+                peak = pk > peak ? pk : peak;
+                if ((hc->turbine_error >> k) & 1) {
+                    set_error("Thetis induction correction is outside its real domain: Ct*A exceeds rotor diameter "
+                              "times local water depth");
+                    res->reason = BRAIDS_RIVER_TURBINE; res->failed_stage = k; res->frequency_peak = peak;
+                    return BRAIDS_EINVAL;
+                }
+                if (!(dt * pk <= river_cfl * (1 + 1e-12))) reason = BRAIDS_RIVER_CFL;
+                else if (code == RFAIL_NONFINITE) reason = BRAIDS_RIVER_NONFINITE;
+                else if (code == RFAIL_NEGATIVE) reason = BRAIDS_RIVER_NEGATIVE;
+            }
+            if (reason) {
+                res->reason = reason; res->failed_stage = k; res->fail_cell = hc->fail_cell[k] - 1;
+                break;
+            }
+        }
+        res->frequency_peak = peak;
+        if (res->reason) return BRAIDS_REJECTED;
+        std::swap(st[0], st[3]);
+        for (int b = 0; b < nb; ++b) ledger[b] = ledger[b] + hl[b];
+        if (nrot > 0) {
+            std::copy(diag_host, diag_host + 2 * (size_t)nrot * TURBINE_DIAG, diag_last.begin());
+            for (int a = 0; a < nrot; ++a) {
+                const double p0 = diag_host[(size_t)a * TURBINE_DIAG + 8];
+                const double p1 = diag_host[((size_t)nrot + a) * TURBINE_DIAG + 8];
+                const double v = dt * (.5 * (p0 + p1)), sum = energy[a] + v;
+                energy_comp[a] += std::fabs(energy[a]) >= std::fabs(v) ? (energy[a] - sum) + v : (v - sum) + energy[a];
+                energy[a] = sum;
+            }
+        }
         res->accepted = 1;
         return BRAIDS_OK;
     }
@@ -567,6 +753,114 @@ extern "C" int river_get_face_speed(braids_solver *s, double *speed)
 {
     if (!s || !speed) { set_error("null argument"); return BRAIDS_EINVAL; }
     return s->impl->face_speed(speed);
+}
+
+// -- river SSPRK(2,2) and turbines ---------------------------------------------
+extern "C" int braids_river_configure(braids_solver *s, double cfl, int drag_mode, const double *drag_values)
+{
+    if (!s || !(cfl > 0) || cfl > 1 || drag_mode < 0 || drag_mode > 4) { set_error("invalid river options"); return BRAIDS_EINVAL; }
+    return s->impl->river_configure(cfl, drag_mode, drag_values);
+}
+
+extern "C" int braids_river_step(braids_solver *s, double dt, double t, braids_river_result *res)
+{
+    if (!s || !res || !(dt > 0)) { set_error("invalid step arguments"); return BRAIDS_EINVAL; }
+    return s->impl->river_step(dt, t, res);
+}
+
+extern "C" int braids_river_advance(braids_solver *s, double dt, double time, double *step_hint, int *substeps,
+                                    int *rejections, double *accepted_cfl_max)
+{
+    if (!s || !step_hint || !substeps || !(dt > 0)) { set_error("invalid advance arguments"); return BRAIDS_EINVAL; }
+    SolverBase *x = s->impl;
+    const double floor_ = dt * std::ldexp(1.0, -20), safety = .98;
+    double elapsed = 0., hint = *step_hint > 0 ? *step_hint : dt, cflmax = 0.;
+    int used = 0, rejected = 0;
+    *substeps = 0;
+    if (rejections) *rejections = 0;
+    while (!used || elapsed < dt - 1e-13 * std::fmax(1., dt)) {
+        const double trial = std::fmin(hint, dt - elapsed);
+        const bool truncated = trial < hint * (1 - 1e-12);
+        braids_river_result r{};
+        int rc = x->river_step(trial, time + elapsed, &r);
+        if (rc < 0) { *step_hint = hint; return rc; }
+        const double peak = r.frequency_peak;
+        if (rc == BRAIDS_REJECTED) {
+            ++rejected;
+            if (rejections) *rejections = rejected;
+            hint = peak > 0 ? std::fmin(.5 * trial, safety * x->river_cfl / peak) : .5 * trial;
+            if (hint < floor_ || rejected > 100) {
+                *step_hint = hint;
+                set_error("river substep rejected: step below dt/2^20 or more than 100 rejections");
+                return BRAIDS_REJECTED;
+            }
+            continue;
+        }
+        elapsed += trial; ++used;
+        cflmax = std::fmax(cflmax, trial * peak);
+        const double growth = truncated ? hint : 2 * trial;
+        hint = peak > 0 ? std::fmin(growth, safety * x->river_cfl / peak) : growth;
+        *substeps = used;
+        if (used > 4096) { *step_hint = hint; set_error("Explicit substep limit"); return BRAIDS_REJECTED; }
+    }
+    *step_hint = hint;
+    if (accepted_cfl_max) *accepted_cfl_max = cflmax;
+    return BRAIDS_OK;
+}
+
+extern "C" int braids_turbines_set(braids_solver *s, const braids_turbines *t)
+{
+    if (!s || !t || t->nrot < 0) { set_error("null argument"); return BRAIDS_EINVAL; }
+    if (t->nrot > 0 && (!t->D || !t->top || !t->bottom || !t->tol || !t->min_surface || !t->min_bed || !t->control ||
+                        !t->ct_const || !t->cp_const || !t->table_off || !t->foot_off || !t->foot_cell ||
+                        !t->foot_norm || !t->foot_density || !t->plane_off || !t->plane_cell || !t->plane_dx ||
+                        !t->plane_dy || !t->plane_inv)) {
+        set_error("missing turbine array"); return BRAIDS_EINVAL;
+    }
+    return s->impl->turbines_set(t);
+}
+
+extern "C" int braids_turbines_read(braids_solver *s, double *diag, double *energy)
+{
+    if (!s) { set_error("null solver"); return BRAIDS_EINVAL; }
+    SolverBase *x = s->impl;
+    if (diag) std::copy(x->diag_last.begin(), x->diag_last.end(), diag);
+    if (energy) for (int a = 0; a < x->nrot; ++a) energy[a] = x->energy[a] + x->energy_comp[a];
+    return BRAIDS_OK;
+}
+
+extern "C" int braids_turbine_footprint(int npairs, const double *tri, const double *D, const double *nodes,
+                                        const double *weights, double *value, double *clipped, int *order)
+{
+    if (npairs < 0 || (npairs > 0 && (!tri || !D || !nodes || !weights || !value || !clipped || !order))) {
+        set_error("invalid footprint arguments"); return BRAIDS_EINVAL;
+    }
+    if (npairs == 0) return BRAIDS_OK;
+    const size_t nq = 16 + 32 + 64 + 128;
+    double *dt = nullptr, *dD = nullptr, *dn = nullptr, *dw = nullptr, *dv = nullptr, *dc = nullptr;
+    int *dord = nullptr;
+    auto cleanup = [&]() {
+        for (void *p : {(void *)dt, (void *)dD, (void *)dn, (void *)dw, (void *)dv, (void *)dc, (void *)dord})
+            if (p) (void)hipFree(p);
+    };
+    if (hipMalloc(&dt, 6 * (size_t)npairs * 8) != hipSuccess || hipMalloc(&dD, (size_t)npairs * 8) != hipSuccess ||
+        hipMalloc(&dn, nq * 8) != hipSuccess || hipMalloc(&dw, nq * 8) != hipSuccess ||
+        hipMalloc(&dv, (size_t)npairs * 8) != hipSuccess || hipMalloc(&dc, (size_t)npairs * 8) != hipSuccess ||
+        hipMalloc(&dord, (size_t)npairs * 4) != hipSuccess) {
+        cleanup(); set_error("hipMalloc failed for the footprint"); return BRAIDS_ENOMEM;
+    }
+    hipError_t e = hipMemcpy(dt, tri, 6 * (size_t)npairs * 8, hipMemcpyHostToDevice);
+    if (e == hipSuccess) e = hipMemcpy(dD, D, (size_t)npairs * 8, hipMemcpyHostToDevice);
+    if (e == hipSuccess) e = hipMemcpy(dn, nodes, nq * 8, hipMemcpyHostToDevice);
+    if (e == hipSuccess) e = hipMemcpy(dw, weights, nq * 8, hipMemcpyHostToDevice);
+    if (e == hipSuccess) { run_footprint(npairs, dt, dD, dn, dw, dv, dc, dord, nullptr); e = hipGetLastError(); }
+    if (e == hipSuccess) e = hipDeviceSynchronize();
+    if (e == hipSuccess) e = hipMemcpy(value, dv, (size_t)npairs * 8, hipMemcpyDeviceToHost);
+    if (e == hipSuccess) e = hipMemcpy(clipped, dc, (size_t)npairs * 8, hipMemcpyDeviceToHost);
+    if (e == hipSuccess) e = hipMemcpy(order, dord, (size_t)npairs * 4, hipMemcpyDeviceToHost);
+    cleanup();
+    if (e != hipSuccess) { set_error(std::string("footprint: ") + hipGetErrorString(e)); return BRAIDS_EHIP; }
+    return BRAIDS_OK;
 }
 
 extern "C" int braids_profile(braids_solver *s, int enable)

@@ -1,0 +1,450 @@
+// River SSPRK(2,2) stage kernels and the Thetis turbine sink.
+//
+// The reference river step (RiverExplicit._single, river_explicit.py:94-119)
+// is, per substep of size dt:
+//   work = drag(base, dt/2)                       (Strang half-step)
+//   r1   = rate(work);  q1 = work + dt r1          (dry momentum cleared)
+//   r2   = rate(q1);    q2 = work/2 + (q1 + dt r2)/2, then drag(q2, dt/2)
+// where rate (RiverExplicit.rate over RiverMidpoint.rate, river_midpoint.py
+// :100-176) is the native spatial rate (bend_native_rates), the explicit
+// face-sum CFL frequency (river_explicit.h:13-25), the conservative inventory
+// limiter of the stage input (river_midpoint.h:13-41) and the turbine sink
+// (river_turbine_thetis.py:263-362).
+//
+// On the device a stage is: k_prep, k_traces, k_face (shared with the
+// SSPRK(3,3) path), then
+//   k_river_theta   cells: inventory factor theta and the CFL frequency
+//   k_turbine_sample / k_turbine_apply   one workgroup per rotor (gating
+//                   statistics, Ct/Cp lookup, induction) and the per-cell sink
+//   k_river_update  cells: the native rate, the limiter applied to each
+//                   incident face on the fly, the sink, the SSP combination,
+//                   positivity, dry momentum and (last stage) the drag.
+// Nothing is synchronised with the host inside a substep: rejections are
+// flags in the Control block, read once per substep.
+#pragma once
+#include "swe_kernels.h"
+
+namespace braids {
+
+// river_chezy, river_roughness.py:42-47
+__device__ __forceinline__ double river_chezy(int method, double depth, double value)
+{
+    if (method == 1) return value;
+    if (method == 2) return pow(depth, 1. / 6.) / value;
+    if (method == 3) return sqrt(9.81) / .41 * log1p(30. * depth / (exp(1.) * value));
+    return 18. * log10(12. * fmax(.5, depth / value));
+}
+
+// river_cell_drag (river_hydraulic_drag.h:3-16, bedform feedback off) on one cell
+__device__ __forceinline__ void cell_drag(double h, double &hu, double &hv, double value, int mode, double dt,
+                                          double grav, double dry)
+{
+    if (mode <= 0) return;                     // no drag installed
+    if (h <= dry) { hu = 0.; hv = 0.; return; }
+    const double c = river_chezy(mode, h, value);
+    const double speed = sqrt(hu * hu + hv * hv) / h;
+    const double factor = 1. / (1. + dt * grav * speed / (c * c * h));
+    hu = hu * factor; hv = hv * factor;
+}
+
+template <class R>
+__global__ void __launch_bounds__(256) k_river_drag(int nc, StateView<R> in, StateView<R> out, DragView d,
+                                                    double dt, double grav, double dry, int remap)
+{
+    const int c = logical_block(remap) * blockDim.x + threadIdx.x;
+    if (c >= nc) return;
+    const double h = in.h[c];
+    double hu = in.hu[c], hv = in.hv[c];
+    cell_drag(h, hu, hv, d.mode > 0 ? d.values[c] : 0., d.mode, dt, grav, dry);
+    out.h[c] = h; out.hu[c] = hu; out.hv[c] = hv;
+}
+
+// midpoint_inventory (river_midpoint.h:13-21) with the stage input as the
+// available inventory, and explicit_frequency (river_explicit.h:13-25, no
+// exterior owner) max-reduced into the stage's CFL slot.
+template <class R>
+__global__ void __launch_bounds__(256) k_river_theta(MeshView<R> m, StateView<R> q, Work<R> w, double *theta,
+                                                     Control *ctl, int slot, double dt, int remap)
+{
+    const int nf = m.nf;
+    const int c = logical_block(remap) * blockDim.x + threadIdx.x;
+    double freq = 0.;
+    if (c < m.nc) {
+        double out = 0., sum = 0.;
+        for (int k = m.offset[c]; k < m.offset[c + 1]; ++k) {
+            const int f = m.adj_face[k];
+            const double sg = (m.adj_flags[k] & ADJ_OWNER) ? 1. : -1.;
+            const double L = (double)m.length[f];
+            out += fmax(sg * (double)w.F[f], 0.) * L;
+            sum += L * (double)w.F[5 * nf + f];
+        }
+        const double A = (double)m.area[c], hA = q.h[c] * A;
+        double th = 1.;
+        if (out > 0. && dt * out >= hA) th = fmin(1., hA / (dt * out)) * (1. - 32. * 2.2204460492503131e-16);
+        theta[c] = th;
+        freq = sum / A;
+        if (!(freq == freq)) freq = 1e300;     // a NaN frequency rejects the trial
+    }
+    block_max_to(freq, &ctl->peak_bits[slot]);
+}
+
+// STAGE 0: out = in + dt r (in is the dragged base);  STAGE 1: out = in0/2 +
+// (in + dt r)/2 with in0 the dragged base, then the closing drag half-step.
+template <class R, int STAGE>
+__global__ void __launch_bounds__(256) k_river_update(MeshView<R> m, StateView<R> in, StateView<R> in0,
+                                                      StateView<R> out, Work<R> w, const double *__restrict__ theta,
+                                                      const double *__restrict__ tsink, const uint8_t *__restrict__ tmask,
+                                                      double *wlim, DragView d, Control *ctl, int slot,
+                                                      double dt, double drag_dt, int remap)
+{
+    const int nc = m.nc, nf = m.nf;
+    const int c = logical_block(remap) * blockDim.x + threadIdx.x;
+    int fail = 0;
+    if (c < nc) {
+        const double h = in.h[c], dry = (double)m.dry;
+        const R A = m.area[c];
+        const double Ad = (double)A;
+        const double p = .5 * (double)m.grav * (h * h);
+        double s0 = 0., s1 = 0., s2 = 0., mass = 0., mx = 0., my = 0.;
+        for (int k = m.offset[c]; k < m.offset[c + 1]; ++k) {
+            const int f = m.adj_face[k];
+            const bool own = (m.adj_flags[k] & ADJ_OWNER) != 0;
+            const R L = m.length[f];
+            const R F1 = w.F[f];
+            const R G2 = own ? w.F[nf + f] : w.F[3 * nf + f];
+            const R G3 = own ? w.F[2 * nf + f] : w.F[4 * nf + f];
+            // native rate (bend_native_rates_tail.inc:4-20): momentum only,
+            // the mass rate is replaced by the limited water below
+            if (sizeof(R) == 8) {
+                const double wgt = (double)L / Ad;
+                const double nxf = m.normal[f], nyf = m.normal[nf + f];
+                if (own) { s1 = s1 + wgt * ((double)G2 - p * nxf); s2 = s2 + wgt * ((double)G3 - p * nyf); }
+                else { s1 = s1 - wgt * ((double)G2 - p * nxf); s2 = s2 - wgt * ((double)G3 - p * nyf); }
+            } else {
+                const R wgt = L / A;    // FP32 face fluxes are net of this cell's pressure
+                if (own) { s1 = s1 + (double)(wgt * G2); s2 = s2 + (double)(wgt * G3); }
+                else { s1 = s1 - (double)(wgt * G2); s2 = s2 - (double)(wgt * G3); }
+            }
+            // midpoint_faces (river_midpoint.h:22-33) for this face
+            double wl = (double)F1, cx = 0., cy = 0.;
+            const int o = m.owner[f], j = m.neighbour[f];
+            if (wl != 0.) {
+                const int donor = wl < 0. ? j : o;
+                if (donor >= 0 && theta[donor] != 1.) {
+                    if (j < 0 && m.bid[f] >= 0) fail = RFAIL_BOUNDARY;
+                    else {
+                        const double th = theta[donor];
+                        const double delta = (th - 1.) * wl;
+                        wl = wl * th;
+                        const double hd = in.h[donor];
+                        if (hd > dry) { cx = delta * in.hu[donor] / hd; cy = delta * in.hv[donor] / hd; }
+                    }
+                }
+            }
+            // midpoint_gather (river_midpoint.h:34-41)
+            const double sw = (own ? 1. : -1.) * (double)L / Ad;
+            mass -= sw * wl; mx -= sw * cx; my -= sw * cy;
+            if (j < 0 && m.bid[f] >= 0) wlim[f] = wl;
+        }
+        (void)s0;
+        double r0 = mass, r1 = -s1 + mx, r2 = -s2 + my;
+        if (tsink && tmask[c]) { r1 = r1 - tsink[c]; r2 = r2 - tsink[nc + c]; }
+        double nh, nhu, nhv;
+        if (STAGE == 0) {
+            nh = in.h[c] + dt * r0; nhu = in.hu[c] + dt * r1; nhv = in.hv[c] + dt * r2;
+        } else {
+            nh = .5 * in0.h[c] + .5 * (in.h[c] + dt * r0);
+            nhu = .5 * in0.hu[c] + .5 * (in.hu[c] + dt * r1);
+            nhv = .5 * in0.hv[c] + .5 * (in.hv[c] + dt * r2);
+        }
+        if (!isfinite(nh) || !isfinite(nhu) || !isfinite(nhv)) fail = fail > RFAIL_NONFINITE ? fail : RFAIL_NONFINITE;
+        if (nh < 0) fail = fail > RFAIL_NEGATIVE ? fail : RFAIL_NEGATIVE;
+        if (nh <= dry) { nhu = 0.; nhv = 0.; }
+        if (STAGE == 1) cell_drag(nh, nhu, nhv, d.mode > 0 ? d.values[c] : 0., d.mode, drag_dt, (double)m.grav, dry);
+        out.h[c] = nh; out.hu[c] = nhu; out.hv[c] = nhv;
+    }
+    if (fail) { atomicMax(&ctl->failure[slot], fail); atomicMax(&ctl->fail_cell[slot], c + 1); }
+}
+
+// Boundary ledger of one substep: ledger[b] = -dt sum_f L (w1 + w2)/2 over
+// the faces of boundary b (inward water; the substep's mean limited flux).
+template <class R>
+__global__ void __launch_bounds__(TRACE_BLOCK) k_river_ledger(MeshView<R> m, const double *w1, const double *w2,
+                                                              double *ledger, double dt)
+{
+    const int b = blockIdx.x;
+    if (b >= m.nb) return;
+    __shared__ double sh[TRACE_BLOCK];
+    double s = 0;
+    for (int k = m.bnd_offset[b] + threadIdx.x; k < m.bnd_offset[b + 1]; k += TRACE_BLOCK) {
+        const int f = m.bnd_faces[k];
+        s = s - dt * (double)m.length[f] * (.5 * (w1[f] + w2[f]));
+    }
+    s = block_sum<TRACE_BLOCK>(s, sh);
+    if (threadIdx.x == 0) ledger[b] = s;
+}
+
+// ---------------------------------------------------------------------------
+// Thetis turbine sink
+
+__device__ __forceinline__ double interp_table(double x, const double *xp, const double *fp, int n)
+{
+    // numpy.interp, then the strict upper endpoint of lookup
+    // (river_turbine_thetis.py:166-179)
+    if (x < xp[0] || x >= xp[n - 1]) return 0.;
+    int lo = 0, hi = n - 1;
+    while (hi - lo > 1) {
+        const int mid = (lo + hi) >> 1;
+        if (x >= xp[mid]) lo = mid; else hi = mid;
+    }
+    const double slope = (fp[lo + 1] - fp[lo]) / (xp[lo + 1] - xp[lo]);
+    return slope * (x - xp[lo]) + fp[lo];
+}
+
+template <int BLOCK>
+__device__ __forceinline__ double block_min(double v, double *sh)
+{
+    sh[threadIdx.x] = v;
+    __syncthreads();
+    for (int s = BLOCK / 2; s > 0; s >>= 1) {
+        if ((int)threadIdx.x < s) sh[threadIdx.x] = fmin(sh[threadIdx.x], sh[threadIdx.x + s]);
+        __syncthreads();
+    }
+    const double r = sh[0];
+    __syncthreads();
+    return r;
+}
+
+// One workgroup per rotor: CoveredRotor._operating_state gating
+// (river_turbine_flow.py:136-180) on the rotor-plane surface/bed samples and
+// the footprint means, stop_below_cut_in (river_turbine_thetis.py:219-240),
+// then cell_terms (:182-218) per footprint cell.  The statistics the
+// reference moves to the host (5 + 6 scalars per rotor and stage) stay on the
+// device.
+template <class R>
+__global__ void __launch_bounds__(TRACE_BLOCK) k_turbine_sample(TurbineView T, MeshView<R> m, StateView<R> q,
+                                                                const double *__restrict__ z, int slot)
+{
+    const int a = blockIdx.x;
+    if (a >= T.nrot) return;
+    __shared__ double sh[TRACE_BLOCK];
+    __shared__ int s_reason;
+    const int nc = m.nc;
+    const double dry = 1e-6;                 // CoveredRotor(dry=1e-6)
+    const double mdry = (double)m.dry;
+    // rotor plane: wet LS surface gradient and full LS bed gradient of the
+    // row's cell (hkt_wet_gradients, river_turbine_flow.h:12-33)
+    double low = 1e300, high = -1e300;
+    for (int k = T.plane_off[a] + threadIdx.x; k < T.plane_off[a + 1]; k += TRACE_BLOCK) {
+        const int i = T.plane_cell[k];
+        double xx = 0., xy = 0., yy = 0., bx = 0., by = 0., bz0 = 0., bz1 = 0.;
+        const bool wet = q.h[i] > mdry;
+        for (int s = m.offset[i]; s < m.offset[i + 1]; ++s) {
+            const int j = m.adj_other[s];
+            if (j < 0) continue;
+            const double dx = T.xy[j] - T.xy[i], dy = T.xy[nc + j] - T.xy[nc + i];
+            const double w = 1. / (dx * dx + dy * dy), dz = z[j] - z[i];
+            bz0 += w * dx * dz; bz1 += w * dy * dz;
+            if (!wet || q.h[j] <= mdry) continue;
+            xx += w * dx * dx; xy += w * dx * dy; yy += w * dy * dy;
+            const double de = (q.h[j] + z[j]) - (q.h[i] + z[i]);
+            bx += w * dx * de; by += w * dy * de;
+        }
+        const double *inv = T.plane_inv + 3 * k;
+        const double gzx = inv[0] * bz0 + inv[1] * bz1, gzy = inv[1] * bz0 + inv[2] * bz1;
+        double gex = 0., gey = 0.;
+        const double det = xx * yy - xy * xy;
+        if (det > 1e-12 * fmax(1., xx * yy)) { gex = (yy * bx - xy * by) / det; gey = (xx * by - xy * bx) / det; }
+        const double eta = q.h[i] + z[i];
+        const double surface = eta + gex * T.plane_dx[k] + gey * T.plane_dy[k];
+        const double bed = z[i] + gzx * T.plane_dx[k] + gzy * T.plane_dy[k];
+        low = fmin(low, fmin(surface, eta));
+        high = fmax(high, fmax(bed, z[i]));
+    }
+    low = block_min<TRACE_BLOCK>(low, sh);
+    high = -block_min<TRACE_BLOCK>(-high, sh);
+    // footprint means
+    double u1 = 0., depth = 0., smin = 1e300, anydry = 0.;
+    for (int k = T.foot_off[a] + threadIdx.x; k < T.foot_off[a + 1]; k += TRACE_BLOCK) {
+        const int c = T.foot_cell[k];
+        const double h = q.h[c], hs = fmax(h, dry);
+        const double u = q.hu[c] / hs, v = q.hv[c] / hs, sp = hypot(u, v);
+        if (h <= dry) anydry = 1.;
+        u1 += T.foot_norm[k] * sp;
+        depth += T.foot_norm[k] * h;
+        smin = fmin(smin, sp);
+    }
+    u1 = block_sum<TRACE_BLOCK>(u1, sh);
+    depth = block_sum<TRACE_BLOCK>(depth, sh);
+    smin = block_min<TRACE_BLOCK>(smin, sh);
+    anydry = -block_min<TRACE_BLOCK>(-anydry, sh);
+    const double margin = low - T.top[a], clearance = T.bottom[a] - high;
+    const int t0 = T.table_off[a], nt = T.table_off[a + 1] - t0;
+    if (threadIdx.x == 0) {
+        const double tol = T.tol[a];
+        int reason = 0;
+        if (margin < tol) reason = 1;
+        else if (clearance < tol) reason = 2;
+        else if (margin + tol < T.min_surface[a]) reason = 3;
+        else if (clearance + tol < T.min_bed[a]) reason = 4;
+        else if (anydry > 0.) reason = 5;
+        if (reason == 0 && u1 <= 1e-10) reason = u1 < 0 ? 6 : 7;
+        if (reason == 0 && T.control[a] == 1 && smin < T.table_speed[t0]) reason = 8;
+        s_reason = reason;
+    }
+    __syncthreads();
+    const int reason = s_reason;
+    const double D = T.D[a], Aro = 3.14159265358979323846 * (D * D) / 4;
+    double F = 0., work = 0., P = 0., inc = 0., ctm = 0., cpm = 0.;
+    int err = 0;
+    for (int k = T.foot_off[a] + threadIdx.x; k < T.foot_off[a + 1]; k += TRACE_BLOCK) {
+        double kk = 0.;
+        if (reason == 0) {
+            const int c = T.foot_cell[k];
+            const double h = q.h[c], hs = fmax(h, dry);
+            const double u = q.hu[c] / hs, v = q.hv[c] / hs, sp = hypot(u, v);
+            const double ct = T.control[a] == 1 ? interp_table(sp, T.table_speed + t0, T.table_ct + t0, nt) : T.ct_const[a];
+            const double radicand = 1 - Aro * ct / (D * h);
+            if (radicand < 0) err = 1;
+            const double alpha = .5 * (1 + sqrt(fmax(radicand, 0.)));
+            const double incident = sp / alpha;
+            double cp;
+            if (T.control[a] == 1) cp = interp_table(incident, T.table_speed + t0, T.table_cp + t0, nt);
+            else cp = T.cp_const[a] > 0 ? T.cp_const[a] : .5 * ct * (1 + sqrt(1 - ct));
+            const double K = Aro * ct / (2 * (alpha * alpha));
+            const double density = T.foot_density[k];
+            kk = K * density / (double)m.area[c];
+            F += K * density * (sp * sp);
+            work += K * density * pow(sp, 3.0);
+            P += .5 * Aro * cp * pow(incident, 3.0) * density;
+            inc += T.foot_norm[k] * incident;
+            ctm += T.foot_norm[k] * ct;
+            cpm += T.foot_norm[k] * cp;
+        }
+        T.kslot[k] = kk;
+    }
+    if (err) atomicOr(T.error, 1 << slot);
+    F = block_sum<TRACE_BLOCK>(F, sh);
+    work = block_sum<TRACE_BLOCK>(work, sh);
+    P = block_sum<TRACE_BLOCK>(P, sh);
+    inc = block_sum<TRACE_BLOCK>(inc, sh);
+    ctm = block_sum<TRACE_BLOCK>(ctm, sh);
+    cpm = block_sum<TRACE_BLOCK>(cpm, sh);
+    if (threadIdx.x == 0) {
+        double *d = T.diag + ((size_t)slot * T.nrot + a) * TURBINE_DIAG;
+        d[0] = reason == 0; d[1] = reason; d[2] = low; d[3] = high; d[4] = u1; d[5] = depth;
+        d[6] = F; d[7] = work; d[8] = P; d[9] = reason == 0 ? inc : u1; d[10] = ctm; d[11] = cpm;
+        d[12] = margin; d[13] = clearance; d[14] = 0.; d[15] = 0.;
+    }
+}
+
+// ThetisArray.apply (river_turbine_thetis.py:350-362): the momentum sink
+// k |u| (u, v) of every rotor covering the cell, in rotor order.
+template <class R>
+__global__ void __launch_bounds__(256) k_turbine_apply(TurbineView T, StateView<R> q, int nc)
+{
+    const int u = blockIdx.x * blockDim.x + threadIdx.x;
+    if (u >= T.nunion) return;
+    const int c = T.union_cell[u];
+    const double hs = fmax(q.h[c], 1e-6);
+    const double uu = q.hu[c] / hs, vv = q.hv[c] / hs, sp = hypot(uu, vv);
+    double sx = 0., sy = 0.;
+    for (int e = T.union_off[u]; e < T.union_off[u + 1]; ++e) {
+        const double k = T.kslot[T.union_slot[e]];
+        sx = sx + k * sp * uu;
+        sy = sy + k * sp * vv;
+    }
+    T.tsink[c] = sx; T.tsink[nc + c] = sy;
+}
+
+// Footprint weights (river_turbine_geometry.py:29-71): one workgroup per
+// (rotor, candidate cell) pair.  The triangle, in coordinates about the rotor
+// centre, is clipped to the D x D square (thread 0, into LDS), fan
+// triangulated and integrated with the Duffy-mapped Gauss-Legendre rule of
+// order 16, then 32/64/128 until two orders agree to 2e-12 D^2; the points of
+// each rule are spread over the workgroup and tree-reduced.
+static __global__ void __launch_bounds__(TRACE_BLOCK) k_footprint(int npairs, const double *__restrict__ tri,
+                                                           const double *__restrict__ Dp,
+                                                           const double *__restrict__ nodes,
+                                                           const double *__restrict__ weights,
+                                                           double *value, double *clipped, int *order_out)
+{
+    const int pair = blockIdx.x;
+    if (pair >= npairs) return;
+    __shared__ double poly[2 * 16];
+    __shared__ int s_m;
+    __shared__ double s_area;
+    __shared__ double sh[TRACE_BLOCK];
+    const double D = Dp[pair], hh = D / 2;
+    if (threadIdx.x == 0) {
+        double a[32], b[32];
+        int m = 3;
+        for (int i = 0; i < 6; ++i) a[i] = tri[6 * pair + i];
+        for (int e = 0; e < 4 && m > 0; ++e) {
+            int k = 0;
+            for (int i = 0; i < m; ++i) {
+                const double *p = a + 2 * i, *qq = a + 2 * ((i + 1) % m);
+                double dp, dq;
+                if (e == 0) { dp = hh - p[0]; dq = hh - qq[0]; }
+                else if (e == 1) { dp = p[0] + hh; dq = qq[0] + hh; }
+                else if (e == 2) { dp = hh - p[1]; dq = hh - qq[1]; }
+                else { dp = p[1] + hh; dq = qq[1] + hh; }
+                if (dp >= 0) { b[2 * k] = p[0]; b[2 * k + 1] = p[1]; ++k; }
+                if ((dp >= 0) != (dq >= 0)) {
+                    const double t = dp / (dp - dq);
+                    b[2 * k] = p[0] + t * (qq[0] - p[0]);
+                    b[2 * k + 1] = p[1] + t * (qq[1] - p[1]);
+                    if (e == 0) b[2 * k] = hh; else if (e == 1) b[2 * k] = -hh;
+                    else if (e == 2) b[2 * k + 1] = hh; else b[2 * k + 1] = -hh;
+                    ++k;
+                }
+            }
+            m = k;
+            for (int i = 0; i < 2 * m; ++i) a[i] = b[i];
+        }
+        double ar = 0.;
+        for (int i = 0; i < m; ++i) {
+            const double *p = a + 2 * i, *qq = a + 2 * ((i + 1) % m);
+            ar += p[0] * qq[1] - qq[0] * p[1];
+        }
+        for (int i = 0; i < 2 * m; ++i) poly[i] = a[i];
+        s_m = m;
+        s_area = .5 * fabs(ar);
+    }
+    __syncthreads();
+    const int m = s_m;
+    const double area = s_area;
+    if (m < 3 || area <= 0) {
+        if (threadIdx.x == 0) { value[pair] = 0.; clipped[pair] = area; order_out[pair] = 0; }
+        return;
+    }
+    const int orders[4] = {16, 32, 64, 128};
+    double prev = 0., cur = 0.;
+    int base = 0, done = -1;
+    for (int o = 0; o < 4; ++o) {
+        const int order = orders[o];
+        cur = 0.;
+        for (int t = 1; t + 1 < m; ++t) {
+            const double ax = poly[0], ay = poly[1];
+            const double v0 = poly[2 * t] - ax, v1 = poly[2 * t + 1] - ay;
+            const double t0 = poly[2 * t + 2] - ax, t1 = poly[2 * t + 3] - ay;
+            double part = 0.;
+            for (int p = threadIdx.x; p < order * order; p += TRACE_BLOCK) {
+                const int i = p / order, j = p - i * order;
+                const double si = nodes[base + i], sj = nodes[base + j];
+                const double x = ax + si * v0 + (1 - si) * sj * t0;
+                const double y = ay + si * v1 + (1 - si) * sj * t1;
+                const double zx = 2 * x / D, zy = 2 * y / D;
+                if (fabs(zx) < 1 && fabs(zy) < 1)
+                    part += exp(2 - 1 / (1 - zx * zx) - 1 / (1 - zy * zy)) * (1 - si) * weights[base + i] * weights[base + j];
+            }
+            cur += fabs(v0 * t1 - v1 * t0) * block_sum<TRACE_BLOCK>(part, sh);
+        }
+        base += order;
+        if (o > 0 && fabs(cur - prev) <= 2e-12 * D * D) { done = order; break; }
+        prev = cur;
+    }
+    if (threadIdx.x == 0) { value[pair] = cur; clipped[pair] = area; order_out[pair] = done; }
+}
+
+}  // namespace braids

@@ -8,4 +8,15 @@ template void run_stage<double>(const MeshView<double> &, StateView<double>, Sta
                                 const LaunchCfg &, hipStream_t);
 template void run_rates<double>(const MeshView<double> &, StateView<double>, const double *, Work<double>, Control *,
                                 double *, double *, double, const LaunchCfg &, hipStream_t);
+template void run_river_substep<double>(const MeshView<double> &, StateView<double>, StateView<double>, StateView<double>,
+                                        StateView<double>, const double *, Work<double>, Control *, const RiverBuffers &,
+                                        const TurbineView *, double, double, const LaunchCfg &, hipStream_t);
+
+void run_footprint(int npairs, const double *tri, const double *D, const double *nodes, const double *weights,
+                   double *value, double *clipped, int *order, hipStream_t stream)
+{
+    if (npairs > 0)
+        hipLaunchKernelGGL(k_footprint, dim3(npairs), dim3(TRACE_BLOCK), 0, stream, npairs, tri, D, nodes, weights,
+                           value, clipped, order);
+}
 }  // namespace braids

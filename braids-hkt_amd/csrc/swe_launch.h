@@ -8,7 +8,8 @@
 namespace braids {
 
 // Kernel kinds timed by the optional event profiler.
-enum : int { KK_PREP = 0, KK_TRACES, KK_FACE, KK_DRAIN, KK_GATHER, KK_LEDGER, KK_COUNT };
+enum : int { KK_PREP = 0, KK_TRACES, KK_FACE, KK_DRAIN, KK_GATHER, KK_LEDGER,
+             KK_RDRAG, KK_RTHETA, KK_TURBINE, KK_RUPDATE, KK_COUNT };
 
 struct LaunchCfg {
     int block = 256;    // threads per workgroup (k_prep/k_face/k_drain/k_gather)
@@ -32,5 +33,24 @@ void run_stage(const MeshView<R> &m, StateView<R> in, StateView<R> base, StateVi
 template <class R>
 void run_rates(const MeshView<R> &m, StateView<R> in, const double *z, Work<R> w, Control *ctl,
                double *rate, double *water, double t, const LaunchCfg &cfg, hipStream_t stream);
+
+// River SSPRK(2,2) substep buffers.
+struct RiverBuffers {
+    double *theta = nullptr;            // [nc] inventory factor of the current stage
+    double *wlim[2] = {nullptr, nullptr};   // [nf] limited water of open boundary faces, per stage
+    double *ledger = nullptr;           // [nb] inward water of the substep
+    DragView drag{0, nullptr};          // mode 0: no drag
+};
+
+// One river substep (RiverExplicit._single): base -> work (drag half-step)
+// -> q1 -> out.  tv may be null (no rotors).  Stage slots 0 and 1 of ctl.
+template <class R>
+void run_river_substep(const MeshView<R> &m, StateView<R> base, StateView<R> work, StateView<R> q1,
+                       StateView<R> out, const double *z, Work<R> w, Control *ctl, const RiverBuffers &rb,
+                       const TurbineView *tv, double dt, double t, const LaunchCfg &cfg, hipStream_t stream);
+
+// Footprint weights of (rotor, cell) pairs (device arrays).
+void run_footprint(int npairs, const double *tri, const double *D, const double *nodes, const double *weights,
+                   double *value, double *clipped, int *order, hipStream_t stream);
 
 }  // namespace braids

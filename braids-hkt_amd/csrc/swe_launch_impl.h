@@ -3,6 +3,7 @@
 #pragma once
 #include "swe_launch.h"
 #include "swe_kernels.h"
+#include "river_kernels.h"
 #include <hip/hip_ext.h>
 #include <type_traits>
 
@@ -96,6 +97,55 @@ void run_rates(const MeshView<R> &m, StateView<R> in, const double *z, Work<R> w
 {
     if (m.scheme == 5) rate_kernels<R, 5>(m, in, z, w, ctl, rate, water, t, cfg, stream);
     else rate_kernels<R, 1>(m, in, z, w, ctl, rate, water, t, cfg, stream);
+}
+
+template <class R, int SCHEME>
+static void river_stage(const MeshView<R> &m, StateView<R> in, StateView<R> in0, StateView<R> out, const double *z,
+                        Work<R> w, Control *ctl, const RiverBuffers &rb, const TurbineView *tv, int slot,
+                        double dt, double t, const LaunchCfg &cfg, hipStream_t s)
+{
+    const int B = cfg.block;
+    const dim3 gc(grid_for(m.nc, B)), blk(B), gb(m.nb), tb(TRACE_BLOCK);
+    launch_prep<R, SCHEME>(m, in, z, w, ctl, slot, cfg, s);
+    if (m.nb > 0) timed_launch(cfg, slot, KK_TRACES, k_traces<R, SCHEME>, gb, tb, s, m, in, z, w, ctl, slot, t);
+    launch_face<R, SCHEME>(m, in, z, w, slot, cfg, s);
+    timed_launch(cfg, slot, KK_RTHETA, k_river_theta<R>, gc, blk, s, m, in, w, rb.theta, ctl, slot, dt, cfg.remap);
+    const double *tsink = nullptr;
+    const uint8_t *tmask = nullptr;
+    if (tv && tv->nrot > 0) {
+        TurbineView T = *tv;
+        T.error = &ctl->turbine_error;
+        timed_launch(cfg, slot, KK_TURBINE, k_turbine_sample<R>, dim3(T.nrot), tb, s, T, m, in, z, slot);
+        hipLaunchKernelGGL((k_turbine_apply<R>), dim3(grid_for(T.nunion, 256)), dim3(256), 0, s, T, in, m.nc);
+        tsink = T.tsink; tmask = T.tmask;
+    }
+    if (slot == 0)
+        timed_launch(cfg, slot, KK_RUPDATE, k_river_update<R, 0>, gc, blk, s, m, in, in0, out, w,
+                     (const double *)rb.theta, tsink, tmask, rb.wlim[0], rb.drag, ctl, slot, dt, 0.0, cfg.remap);
+    else
+        timed_launch(cfg, slot, KK_RUPDATE, k_river_update<R, 1>, gc, blk, s, m, in, in0, out, w,
+                     (const double *)rb.theta, tsink, tmask, rb.wlim[1], rb.drag, ctl, slot, dt, .5 * dt, cfg.remap);
+}
+
+template <class R>
+void run_river_substep(const MeshView<R> &m, StateView<R> base, StateView<R> work, StateView<R> q1,
+                       StateView<R> out, const double *z, Work<R> w, Control *ctl, const RiverBuffers &rb,
+                       const TurbineView *tv, double dt, double t, const LaunchCfg &cfg, hipStream_t s)
+{
+    const int B = cfg.block;
+    const dim3 gc(grid_for(m.nc, B)), blk(B);
+    timed_launch(cfg, 0, KK_RDRAG, k_river_drag<R>, gc, blk, s, m.nc, base, work, rb.drag, .5 * dt,
+                 (double)m.grav, (double)m.dry, cfg.remap);
+    if (m.scheme == 5) {
+        river_stage<R, 5>(m, work, work, q1, z, w, ctl, rb, tv, 0, dt, t, cfg, s);
+        river_stage<R, 5>(m, q1, work, out, z, w, ctl, rb, tv, 1, dt, t + dt, cfg, s);
+    } else {
+        river_stage<R, 1>(m, work, work, q1, z, w, ctl, rb, tv, 0, dt, t, cfg, s);
+        river_stage<R, 1>(m, q1, work, out, z, w, ctl, rb, tv, 1, dt, t + dt, cfg, s);
+    }
+    if (m.nb > 0)
+        hipLaunchKernelGGL((k_river_ledger<R>), dim3(m.nb), dim3(TRACE_BLOCK), 0, s, m,
+                           (const double *)rb.wlim[0], (const double *)rb.wlim[1], rb.ledger, dt);
 }
 
 }  // namespace braids

@@ -12,6 +12,8 @@ enum : int { ADJ_OWNER = 1, ADJ_OPEN = 2, ADJ_OUTLET2 = 4 };
 // boundary traces (FAIL_TRACE + reason 6/8), then the boundary draining check
 // (reason 2, FAIL_DRAIN), then the gather (2 CFL, 3 negative depth, 5 non-finite).
 enum : int { FAIL_DRAIN = 50, FAIL_TRACE = 100 };
+// River substep failures (Control::failure of stage slots 0 and 1).
+enum : int { RFAIL_NEGATIVE = 3, RFAIL_NONFINITE = 5, RFAIL_BOUNDARY = 60 };
 
 template <class R> struct MeshView {
     int nc, nf, nb, scheme;
@@ -59,6 +61,36 @@ struct Control {
     int failure[4];                    // encoded failure (max), see FAIL_*
     int drain_any[4];                  // any draining cell
     int fail_cell[4];                  // 1 + largest cell index that failed in the gather
+    int turbine_error;                 // induction outside its real domain (river path)
 };
+
+// River path: roughness of the Strang drag half-steps and the rotor array.
+struct DragView {
+    int mode;               // river_roughness.selected: 1 Chezy, 2 Manning, 3/4 White-Colebrook
+    const double *values;   // [nc] roughness per cell
+};
+
+constexpr int TURBINE_DIAG = 16;   // per rotor and stage, see k_turbine_sample
+
+struct TurbineView {
+    int nrot, nslot, nunion;
+    const double *D, *top, *bottom, *tol, *min_surface, *min_bed, *ct_const, *cp_const;  // [nrot]
+    const int *control;                   // [nrot] 0 constant_ct, 1 thetis_speed_table
+    const int *table_off;                 // [nrot+1]
+    const double *table_speed, *table_ct, *table_cp;
+    const int *foot_off, *foot_cell;      // footprint CSR (ascending cells)
+    const double *foot_norm, *foot_density;
+    const int *plane_off, *plane_cell;    // rotor-plane rows
+    const double *plane_dx, *plane_dy, *plane_inv;
+    const int *union_cell, *union_off, *union_slot;   // cells -> footprint slots in rotor order
+    const double *xy;                     // [2*nc] centroids
+    double *kslot;                        // [nslot] K density / area (0 when the rotor is off)
+    double *diag;                         // [2][nrot][TURBINE_DIAG]
+    double *tsink;                        // [2*nc] (entries of union cells only)
+    const uint8_t *tmask;                 // [nc] 1 on union cells
+    int *error;                           // induction domain error flag
+};
+
+
 
 }  // namespace braids

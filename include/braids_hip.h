@@ -111,13 +111,83 @@ int braids_native_rates(braids_solver *s, const double *q, const double *z, doub
  * speeds of the last spatial evaluation; device pointer speed[nf]. */
 int river_get_face_speed(braids_solver *s, double *speed);
 
+/* ---- River SSPRK(2,2) path (RiverExplicit, src/braids_core/sediment/river_explicit.py:19-155) ----
+ * Rejection reasons of a river substep (the BendTrialRejected messages of
+ * river_explicit.py:79-117 and river_midpoint.py:123-171). */
+#define BRAIDS_RIVER_SPATIAL 1    /* native spatial residual rejected (boundary traces) */
+#define BRAIDS_RIVER_CFL 2        /* dt * max face-sum frequency > cfl (1 + 1e-12) */
+#define BRAIDS_RIVER_BOUNDARY 3   /* prescribed boundary would exceed available water */
+#define BRAIDS_RIVER_NONFINITE 4
+#define BRAIDS_RIVER_NEGATIVE 5   /* negative stage or endpoint depth */
+#define BRAIDS_RIVER_TURBINE 6    /* induction outside its real domain (an error, not a retry) */
+
+typedef struct braids_river_result {
+    int accepted, reason, failed_stage, fail_cell;
+    double frequency_peak;        /* max over evaluated stages of max_c sum_f L_f s_f / A_c (1/s) */
+} braids_river_result;
+
+/* Face-sum CFL limit cfl in (0, 1] (RiverExplicit(solver, cfl=.5),
+ * river_explicit.py:22-26) and the Strang drag (river_hydraulic_drag.install,
+ * river_hydraulic_drag.py:20-47): drag_mode as river_roughness.selected
+ * (1 Chezy, 2 Manning, 3/4 White-Colebrook; 0 none), drag_values [nc] host
+ * roughness per cell (NULL: the mesh Manning n). */
+int braids_river_configure(braids_solver *s, double cfl, int drag_mode, const double *drag_values);
+/* One substep of size dt at time t (RiverExplicit._single, river_explicit.py:94-119);
+ * commits on success, BRAIDS_REJECTED leaves the state unchanged. */
+int braids_river_step(braids_solver *s, double dt, double t, braids_river_result *res);
+/* RiverExplicit.advance (river_explicit.py:121-155) over an outer interval dt:
+ * substeps with the hint controller (CFL_SAFETY .98, growth <= 2x, rejection
+ * <= 0.5x).  *step_hint in/out (<= 0: none yet). */
+int braids_river_advance(braids_solver *s, double dt, double time, double *step_hint, int *substeps,
+                         int *rejections, double *accepted_cfl_max);
+
+/* Thetis rotor array (ThetisArray, river_turbine_thetis.py:300-362) in CSR
+ * form: footprint cells with normalized weights and density integrals
+ * (weights / ((D/2)^2 1.45661), :253), rotor-plane sample rows (cell,
+ * displacement from the centroid, pinv of the cell's full LS metric,
+ * river_turbine_flow.py:126-135 and bend_methods_v1.py:114-124), gating
+ * heights and clearances (river_turbine_flow.py:117-135), and the control:
+ * 0 constant_ct (ct_const, cp_const; cp <= 0 selects the ideal disk) or 1
+ * thetis_speed_table (table rows). */
+typedef struct braids_turbines {
+    int nrot;
+    const double *D, *top, *bottom, *tol, *min_surface, *min_bed;   /* [nrot] */
+    const int *control;
+    const double *ct_const, *cp_const;
+    const int *table_off;          /* [nrot+1] */
+    const double *table_speed, *table_ct, *table_cp;
+    const int *foot_off;           /* [nrot+1] */
+    const int *foot_cell;          /* ascending cell ids per rotor */
+    const double *foot_norm, *foot_density;
+    const int *plane_off;          /* [nrot+1] */
+    const int *plane_cell;
+    const double *plane_dx, *plane_dy, *plane_inv;   /* plane_inv [3*rows] */
+} braids_turbines;
+int braids_turbines_set(braids_solver *s, const braids_turbines *t);
+/* Per-rotor diagnostics of the last accepted substep, diag[2][nrot][16] (one
+ * block per stage): active, reason (0 covered, 1 insufficient_depth,
+ * 2 bed_intersection, 3 insufficient_surface_clearance,
+ * 4 insufficient_bed_clearance, 5 dry_numerical_support, 6 reverse_flow,
+ * 7 no_incoming_flow, 8 below_cut_in_speed), low surface, high bed, u1,
+ * depth, force, work, useful power, incoming speed, Ct mean, Cp mean, margin,
+ * clearance (per unit density); energy[nrot] = compensated sum of dt (P1+P2)/2. */
+int braids_turbines_read(braids_solver *s, double *diag, double *energy);
+/* Footprint bump integrals (river_turbine_geometry.py:29-71) of npairs
+ * triangles tri[6*npairs] given about their rotor centre, rotor diameters
+ * D[npairs]; nodes/weights: Gauss-Legendre rules of order 16, 32, 64, 128 on
+ * [0,1] back to back (240 each).  Host arrays in and out; value = 0 when the
+ * clipped area is 0; order = converged rule (-1: none). */
+int braids_turbine_footprint(int npairs, const double *tri, const double *D, const double *nodes,
+                             const double *weights, double *value, double *clipped, int *order);
+
 /* Mass inventory sum(A h) of the resident state (compensated FP64 sum). */
 int braids_mass(braids_solver *s, double *mass);
 
 /* Measurement hooks (no reference counterpart; the reference only has the
  * BRAIDS_PROFILE wall-clock phases, cutcell_solver.f90:16584-16646).
  * braids_profile(s, mask) brackets the launches of later steps whose kind bit
- * is set in mask (bit k of {prep, traces, face, drain, gather, ledger}; -1 =
+ * is set in mask (bit k of {prep, traces, face, drain, gather, ledger,
+ * river drag, river theta, turbine, river update}; -1 =
  * all, 0 = off) with HIP events on the solver's stream; braids_profile_read
  * returns accumulated milliseconds and launch counts per kind. */
 int braids_profile(braids_solver *s, int enable);

@@ -1,0 +1,284 @@
+"""Oracle loader for the river SSPRK(2,2) path and the Thetis turbine sink
+(test infrastructure only).
+
+Binds the plain-C restatement (river_oracle.c in liboracle.so) and, where
+they have been compiled, the reference's own single-source operator headers
+(river_explicit.h and river_midpoint.h built with g++ into _ref/).  The rotor
+set-up here (footprint quadrature in C, rotor-plane rows by edge
+intersection) is written independently of the product's host code so that the
+two can be compared.
+"""
+import ctypes
+import math
+
+import numpy as np
+
+from . import swe
+
+HERE = swe.HERE
+REF_EXPLICIT = HERE / '_ref' / 'libriver_explicit_ref.so'
+REF_MIDPOINT = HERE / '_ref' / 'libriver_midpoint_ref.so'
+
+_D = ctypes.c_double
+_P = ctypes.c_void_p
+_I = ctypes.c_int
+ORDERS = (16, 32, 64, 128)
+DIAG = 16
+
+
+def _ptr(a):
+    return None if a is None else ctypes.c_void_p(a.ctypes.data)
+
+
+class _Turbines(ctypes.Structure):
+    _fields_ = [('nrot', _I),
+                ('D', _P), ('top', _P), ('bottom', _P), ('tol', _P), ('min_surface', _P), ('min_bed', _P),
+                ('control', _P), ('ct_const', _P), ('cp_const', _P),
+                ('table_off', _P), ('table_speed', _P), ('table_ct', _P), ('table_cp', _P),
+                ('foot_off', _P), ('foot_cell', _P), ('foot_norm', _P), ('foot_density', _P),
+                ('plane_off', _P), ('plane_cell', _P), ('plane_dx', _P), ('plane_dy', _P), ('plane_inv', _P)]
+
+
+_OPS = {
+    'explicit_frequency': [_I, _I, _D, _D] + [_P] * 11,
+    'midpoint_inventory': [_I, _D] + [_P] * 8,
+    'midpoint_faces': [_I, _I, _D] + [_P] * 8,
+    'midpoint_gather': [_I, _I] + [_P] * 8,
+}
+
+
+def lib():
+    L = swe.lib()
+    if not getattr(L, '_river_bound', False):
+        for name, args in _OPS.items():
+            fn = getattr(L, 'orv_' + name)
+            fn.argtypes = args; fn.restype = None
+        L.orv_chezy.argtypes = [_I, _D, _D]; L.orv_chezy.restype = _D
+        L.orv_cell_drag.argtypes = [_I, _D, _D, _D, _I, _P, _P, _P]; L.orv_cell_drag.restype = None
+        L.orv_interp.argtypes = [_D, _P, _P, _I]; L.orv_interp.restype = _D
+        L.orv_lookup.argtypes = [_D, _P, _P, _I]; L.orv_lookup.restype = _D
+        L.orv_footprint_cell.argtypes = [_P, _D, _P, _P, _P, _P]; L.orv_footprint_cell.restype = _I
+        L.orv_create.argtypes = [_P, _P, _D, _I, _P, _P]; L.orv_create.restype = _P
+        L.orv_destroy.argtypes = [_P]
+        L.orv_turbines_apply.argtypes = [_P, _P, _P, _P, _P]; L.orv_turbines_apply.restype = _I
+        L.orv_rate.argtypes = [_P, _P, _P, _D, _D, _P, _P, _P, _P]; L.orv_rate.restype = _I
+        L.orv_single.argtypes = [_P, _P, _P, _D, _D, _P, _P, _P]; L.orv_single.restype = _I
+        L.orv_advance.argtypes = [_P, _P, _P, _D, _D, _P, _P, _P, _P, _P]; L.orv_advance.restype = _I
+        L._river_bound = True
+    return L
+
+
+def reference_ops():
+    """The reference's compiled operator headers: {name: ctypes function}, or None."""
+    if not (REF_EXPLICIT.exists() and REF_MIDPOINT.exists()):
+        return None
+    out = {}
+    for path, names in ((REF_EXPLICIT, ('explicit_frequency',)),
+                        (REF_MIDPOINT, ('midpoint_inventory', 'midpoint_faces', 'midpoint_gather'))):
+        L = ctypes.CDLL(str(path))
+        L.mobile_threads.argtypes = [_I]
+        L.mobile_threads(1)
+        for n in names:
+            fn = getattr(L, n)
+            fn.argtypes = _OPS[n]; fn.restype = None
+            out[n] = fn
+    return out
+
+
+def oracle_ops():
+    L = lib()
+    return {n: getattr(L, 'orv_' + n) for n in _OPS}
+
+
+def quadrature_rules():
+    from numpy.polynomial.legendre import leggauss
+    s, w = [], []
+    for o in ORDERS:
+        x, ww = leggauss(o)
+        s.append((x + 1) / 2); w.append(ww / 2)
+    return np.concatenate(s), np.concatenate(w)
+
+
+def footprint_cells(mesh, X, Y, D):
+    """(candidate cells, weights, clipped areas, converged orders) by the C restatement."""
+    L = lib()
+    v = mesh.verts
+    lo_x = v[[0, 2, 4]].min(0); hi_x = v[[0, 2, 4]].max(0)
+    lo_y = v[[1, 3, 5]].min(0); hi_y = v[[1, 3, 5]].max(0)
+    h = D / 2
+    cand = np.flatnonzero((hi_x >= X - h) & (lo_x <= X + h) & (hi_y >= Y - h) & (lo_y <= Y + h))
+    nodes, weights = quadrature_rules()
+    val = np.zeros(len(cand)); area = np.zeros(len(cand)); order = np.zeros(len(cand), np.int32)
+    for k, i in enumerate(cand):
+        tri = np.ascontiguousarray(v[:, i] - np.array([X, Y, X, Y, X, Y]))
+        a = np.zeros(1); w = np.zeros(1)
+        order[k] = L.orv_footprint_cell(_ptr(tri), float(D), _ptr(nodes), _ptr(weights), _ptr(w), _ptr(a))
+        val[k] = w[0]; area[k] = a[0]
+    return cand, val, area, order
+
+
+def plane_rows(mesh, X, Y, D):
+    """Rotor-plane rows: for each triangle, the y-interval of the line x = X
+    inside it (from its edge crossings), clipped to [Y - D/2, Y + D/2]."""
+    v = mesh.verts
+    y0, y1 = Y - D / 2, Y + D / 2
+    rows = []
+    for i in range(mesh.nc):
+        P = v[:, i].reshape(3, 2)
+        if not (P[:, 0].min() <= X <= P[:, 0].max()):
+            continue
+        ys = []
+        for e in range(3):
+            a, b = P[e], P[(e + 1) % 3]
+            if a[0] == b[0]:
+                if a[0] == X:
+                    ys += [a[1], b[1]]
+                continue
+            t = (X - a[0]) / (b[0] - a[0])
+            if 0 <= t <= 1:
+                ys.append(a[1] + t * (b[1] - a[1]))
+        if not ys:
+            continue
+        lo, hi = max(min(ys), y0), min(max(ys), y1)
+        if hi - lo <= 1e-12:
+            continue
+        rows += [(i, X - mesh.xy[0, i], lo - mesh.xy[1, i]), (i, X - mesh.xy[0, i], hi - mesh.xy[1, i])]
+    return rows
+
+
+def pinv_metric(mesh, i):
+    a = np.zeros((2, 2))
+    for k in range(mesh.offset[i], mesh.offset[i + 1]):
+        f = mesh.faces_csr[k]
+        j = mesh.neighbour[f] if mesh.owner[f] == i else mesh.owner[f]
+        if j < 0:
+            continue
+        d = mesh.xy[:, j] - mesh.xy[:, i]
+        a += np.outer(d, d) / (d @ d)
+    p = np.linalg.pinv(a)
+    return [p[0, 0], p[0, 1], p[1, 1]]
+
+
+def rotor_arrays(mesh, rows):
+    """Device-style CSR arrays of a rotor array, built by the oracle's own set-up."""
+    cols = {k: [] for k in ('D', 'top', 'bottom', 'tol', 'min_surface', 'min_bed', 'control', 'ct_const',
+                            'cp_const', 'table_speed', 'table_ct', 'table_cp', 'foot_cell', 'foot_norm',
+                            'foot_density', 'plane_cell', 'plane_dx', 'plane_dy', 'plane_inv')}
+    toff, foff, poff = [0], [0], [0]
+    for c in rows:
+        D = float(c['D'])
+        top = c['hub_elevation_m'] + D / 2; bottom = c['hub_elevation_m'] - D / 2
+        cols['D'].append(D); cols['top'].append(top); cols['bottom'].append(bottom)
+        cols['tol'].append(max(1e-9, 64 * np.finfo(float).eps * max(1., abs(top), abs(bottom))))
+        cols['min_surface'].append(float(c['minimum_surface_clearance_m']))
+        cols['min_bed'].append(float(c['minimum_bed_clearance_m']))
+        ctl = c['control']
+        if ctl['mode'] == 'constant_ct':
+            cols['control'].append(0); cols['ct_const'].append(float(ctl['thrust_coefficient']))
+            cols['cp_const'].append(float(ctl['power_coefficient'] or 0.))
+        else:
+            cols['control'].append(1); cols['ct_const'].append(0.); cols['cp_const'].append(0.)
+            for r in ctl['performance']:
+                cols['table_speed'].append(float(r['speed_m_s'])); cols['table_ct'].append(float(r['thrust_coefficient']))
+                cols['table_cp'].append(float(r['power_coefficient']))
+        toff.append(len(cols['table_speed']))
+        cand, val, area, order = footprint_cells(mesh, c['X'], c['Y'], D)
+        keep = val > 0
+        cells, w = cand[keep], val[keep]
+        cols['foot_cell'] += cells.tolist(); cols['foot_norm'] += (w / w.sum()).tolist()
+        cols['foot_density'] += (w / ((D / 2) ** 2 * 1.45661)).tolist()
+        foff.append(len(cols['foot_cell']))
+        for (i, dx, dy) in plane_rows(mesh, c['X'], c['Y'], D):
+            cols['plane_cell'].append(i); cols['plane_dx'].append(dx); cols['plane_dy'].append(dy)
+            cols['plane_inv'] += pinv_metric(mesh, i)
+        poff.append(len(cols['plane_cell']))
+    ints = {'control', 'foot_cell', 'plane_cell'}
+    out = {k: np.array(v + ([0.] if not v else []), np.int32 if k in ints else np.float64) for k, v in cols.items()}
+    out['table_off'] = np.array(toff, np.int32); out['foot_off'] = np.array(foff, np.int32)
+    out['plane_off'] = np.array(poff, np.int32)
+    return len(rows), out
+
+
+class RiverOracle:
+    """FP64 restatement of RiverExplicit on one mesh, optionally with rotors."""
+
+    def __init__(self, mesh, scheme=5, cfl=.5, drag_mode=2, roughness=None, rows=None, grav=9.81, dry=1e-8):
+        self.L = lib()
+        self.hyd = swe.Oracle(mesh, scheme=scheme, grav=grav, dry=dry)
+        self.mesh = mesh
+        self.nc, self.nf, self.nb = mesh.nc, mesh.nf, mesh.nb
+        self.rough = np.ascontiguousarray(mesh.manning if roughness is None else roughness, dtype=np.float64)
+        self.nrot = 0
+        tptr = None
+        if rows:
+            n, arr = rotor_arrays(mesh, rows)
+            self._tarr = arr
+            t = _Turbines()
+            t.nrot = n
+            for k, v in arr.items():
+                setattr(t, k, v.ctypes.data)
+            self._t = t
+            tptr = ctypes.byref(t)
+            self.nrot = n
+        self.ctx = self.L.orv_create(self.hyd.ctx, ctypes.byref(self.hyd._mesh), float(cfl), int(drag_mode),
+                                     _ptr(self.rough), tptr)
+        self.hint = 0.0
+
+    def __del__(self):
+        if getattr(self, 'ctx', None):
+            self.L.orv_destroy(self.ctx)
+            self.ctx = None
+
+    def single(self, q, z, dt, t):
+        q = np.ascontiguousarray(q, dtype=np.float64); z = np.ascontiguousarray(z, dtype=np.float64)
+        out = np.zeros_like(q); water = np.zeros(self.nf); peak = np.zeros(1)
+        rc = self.L.orv_single(self.ctx, _ptr(q), _ptr(z), float(dt), float(t), _ptr(out), _ptr(water), _ptr(peak))
+        return rc, out, water, float(peak[0])
+
+    def advance(self, q, z, dt, time, ledger=None):
+        q = np.array(q, dtype=np.float64, order='C'); z = np.ascontiguousarray(z, dtype=np.float64)
+        hint = np.array([self.hint]); water = np.zeros(self.nf)
+        rej = np.zeros(1, np.int32); cmax = np.zeros(1)
+        led = np.zeros(max(self.nb, 1)) if ledger is None else ledger
+        n = self.L.orv_advance(self.ctx, _ptr(q), _ptr(z), float(dt), float(time), _ptr(hint), _ptr(water),
+                               _ptr(rej), _ptr(cmax), _ptr(led))
+        self.hint = float(hint[0])
+        return dict(q=q, substeps=n, rejections=int(rej[0]), cfl_max=float(cmax[0]), water=water,
+                    ledger=led[:self.nb])
+
+    def turbines(self, q, z):
+        """(rate momentum sink applied to a zero rate, diag[nrot][16], rc)."""
+        q = np.ascontiguousarray(q, dtype=np.float64); z = np.ascontiguousarray(z, dtype=np.float64)
+        rate = np.zeros_like(q); diag = np.zeros(max(self.nrot, 1) * DIAG)
+        rc = self.L.orv_turbines_apply(self.ctx, _ptr(q), _ptr(z), _ptr(rate), _ptr(diag))
+        return rate, diag.reshape(-1, DIAG)[:self.nrot], rc
+
+
+def interp(x, xp, fp):
+    L = lib()
+    xp = np.ascontiguousarray(xp, dtype=np.float64); fp = np.ascontiguousarray(fp, dtype=np.float64)
+    return np.array([L.orv_interp(float(v), _ptr(xp), _ptr(fp), len(xp)) for v in np.atleast_1d(x)])
+
+
+def lookup(x, xp, fp):
+    L = lib()
+    xp = np.ascontiguousarray(xp, dtype=np.float64); fp = np.ascontiguousarray(fp, dtype=np.float64)
+    return np.array([L.orv_lookup(float(v), _ptr(xp), _ptr(fp), len(xp)) for v in np.atleast_1d(x)])
+
+
+def chezy(method, depth, value):
+    return lib().orv_chezy(int(method), float(depth), float(value))
+
+
+def cell_drag(q, dt, mode, values, grav=9.81, dry=1e-8):
+    q = np.array(q, dtype=np.float64, order='C'); n = q.shape[1]
+    values = np.ascontiguousarray(values, dtype=np.float64); ch = np.zeros(n)
+    lib().orv_cell_drag(n, float(dt), float(grav), float(dry), int(mode), _ptr(values), _ptr(q), _ptr(ch))
+    return q, ch
+
+
+def bump_total(D):
+    """Closed form of the footprint integral: (int_{-1}^{1} e^{1-1/(1-s^2)} ds * D/2)^2."""
+    from scipy.integrate import quad
+    one = quad(lambda s: math.exp(1 - 1 / (1 - s * s)), -1., 1., epsabs=1e-13, epsrel=1e-13)[0] * D / 2
+    return one * one

@@ -1,0 +1,503 @@
+/*
+ * Oracle (test infrastructure only; see river_oracle.h): FP64 restatement of
+ * the river SSPRK(2,2) path and the Thetis turbine sink.  Array expressions of
+ * the Python owners are evaluated element by element in NumPy's operation
+ * order (a+b*c as a+(b*c), x**2 as x*x).  Build with -ffp-contract=off.
+ */
+#include "river_oracle.h"
+#include <float.h>
+#include <math.h>
+#include <stdlib.h>
+#include <string.h>
+
+#define EPS DBL_EPSILON
+#define CFL_SAFETY .98          /* river_explicit.py:15 */
+#define BUMP_NORM 1.45661       /* river_turbine_thetis.py:253 */
+#define PI 3.14159265358979323846
+
+/* accessors into the hydraulic restatement (swe_oracle.c) */
+const double *om_face_speed(const om_ctx *x);
+
+/* ------------------------------------------------------------------ */
+/* river_explicit.h:13-25 */
+void orv_explicit_frequency(int n, int nf, double grav, double dry, const double *q, const double *speed,
+                            const double *trace, const int *mask, const int *owner, const double *normal,
+                            const double *area, const double *length, const int *off, const int *faces,
+                            double *out)
+{
+    for (int i = 0; i < n; ++i) {
+        double sum = 0.;
+        for (int k = off[i]; k < off[i + 1]; ++k) {
+            int f = faces[k];
+            double s = speed[f];
+            if (mask && mask[f]) {
+                int o = owner[f];
+                double h = q[o], hb = trace[f], nx = normal[f], ny = normal[nf + f];
+                double a = h > dry ? fabs((q[n + o] * nx + q[2 * n + o] * ny) / h) + sqrt(grav * h) : 0.;
+                double b = hb > dry ? fabs((trace[nf + f] * nx + trace[2 * nf + f] * ny) / hb) + sqrt(grav * hb) : 0.;
+                s = fmax(a, b);
+            }
+            sum += length[f] * s;
+        }
+        out[i] = sum / area[i];
+    }
+}
+
+/* river_midpoint.h:13-21 */
+void orv_midpoint_inventory(int n, double dt, const double *base, const double *area, const double *length,
+                            const int *off, const int *faces, const int *signs, const double *water,
+                            double *theta)
+{
+    for (int i = 0; i < n; ++i) {
+        double out = 0.;
+        for (int k = off[i]; k < off[i + 1]; ++k) {
+            int f = faces[k];
+            out += fmax(signs[k] * water[f], 0.) * length[f];
+        }
+        theta[i] = 1.;
+        if (out > 0. && dt * out >= base[i] * area[i])
+            theta[i] = fmin(1., base[i] * area[i] / (dt * out)) * (1. - 32. * EPS);
+    }
+}
+
+/* river_midpoint.h:22-33 */
+void orv_midpoint_faces(int nf, int n, double dry, const double *q, const double *theta, const int *o,
+                        const int *j, const int *bid, double *water, double *correction, int *status)
+{
+    for (int i = 0; i < nf; ++i) {
+        status[i] = 0; correction[i] = 0.; correction[nf + i] = 0.;
+        if (water[i] == 0.) continue;
+        int donor = water[i] < 0. ? j[i] : o[i];
+        if (donor < 0 || theta[donor] == 1.) continue;
+        if (j[i] < 0 && bid[i] >= 0) { status[i] = 1; continue; }
+        double delta = (theta[donor] - 1.) * water[i];
+        water[i] *= theta[donor];
+        if (q[donor] > dry) {
+            correction[i] = delta * q[n + donor] / q[donor];
+            correction[nf + i] = delta * q[2 * n + donor] / q[donor];
+        }
+    }
+}
+
+/* river_midpoint.h:34-41 */
+void orv_midpoint_gather(int n, int nf, const double *area, const double *length, const int *off,
+                         const int *faces, const int *signs, const double *water, const double *correction,
+                         double *rate)
+{
+    for (int i = 0; i < n; ++i) {
+        double mass = 0., mx = 0., my = 0.;
+        for (int k = off[i]; k < off[i + 1]; ++k) {
+            int f = faces[k];
+            double w = signs[k] * length[f] / area[i];
+            mass -= w * water[f]; mx -= w * correction[f]; my -= w * correction[nf + f];
+        }
+        rate[i] = mass; rate[n + i] += mx; rate[2 * n + i] += my;
+    }
+}
+
+/* river_roughness.py:42-47 */
+double orv_chezy(int method, double depth, double value)
+{
+    if (method == 1) return value;
+    if (method == 2) return pow(depth, 1. / 6.) / value;
+    if (method == 3) return sqrt(9.81) / .41 * log1p(30. * depth / (exp(1.) * value));
+    return 18. * log10(12. * fmax(.5, depth / value));
+}
+
+/* river_hydraulic_drag.h:3-16 (bedform feedback off), spatial roughness
+ * (river_hydraulic_drag.py:11-15) */
+void orv_cell_drag(int n, double dt, double gravity, double dry, int mode, const double *values,
+                   double *q, double *chezy)
+{
+    if (mode <= 0) return;         /* no drag installed */
+    for (int i = 0; i < n; ++i) {
+        double value = values[i], h = q[i];
+        if (h <= dry) {
+            if (chezy) chezy[i] = orv_chezy(mode, dry, value);
+            q[n + i] = q[2 * n + i] = 0.;
+            continue;
+        }
+        double c = orv_chezy(mode, h, value);
+        if (chezy) chezy[i] = c;
+        double speed = sqrt(q[n + i] * q[n + i] + q[2 * n + i] * q[2 * n + i]) / h;
+        double factor = 1. / (1. + dt * gravity * speed / (c * c * h));
+        q[n + i] *= factor; q[2 * n + i] *= factor;
+    }
+}
+
+/* numpy.interp: slope (fp[j+1]-fp[j])/(xp[j+1]-xp[j]), result slope*(x-xp[j])+fp[j],
+ * the last knot returned exactly; x below/above the table clamps. */
+double orv_interp(double x, const double *xp, const double *fp, int n)
+{
+    if (n == 1) return fp[0];
+    if (x < xp[0]) return fp[0];
+    if (x > xp[n - 1]) return fp[n - 1];
+    if (x == xp[n - 1]) return fp[n - 1];
+    int lo = 0, hi = n - 1;          /* xp[lo] <= x < xp[hi] */
+    while (hi - lo > 1) {
+        int mid = (lo + hi) / 2;
+        if (x >= xp[mid]) lo = mid; else hi = mid;
+    }
+    double slope = (fp[lo + 1] - fp[lo]) / (xp[lo + 1] - xp[lo]);
+    double r = slope * (x - xp[lo]) + fp[lo];
+    if (isnan(r)) {
+        r = slope * (x - xp[lo + 1]) + fp[lo + 1];
+        if (isnan(r) && fp[lo] == fp[lo + 1]) r = fp[lo];
+    }
+    return r;
+}
+
+/* lookup, river_turbine_thetis.py:166-179: zero outside [x0, xN) */
+double orv_lookup(double x, const double *xp, const double *fp, int n)
+{
+    double r = orv_interp(x, xp, fp, n);
+    return (x < xp[0] || x >= xp[n - 1]) ? 0. : r;
+}
+
+/* ------------------------------------------------------------------ */
+/* footprint quadrature, river_turbine_geometry.py:29-71 */
+static int clip_square(const double *in, int n, double h, double *out)
+{
+    /* Sutherland-Hodgman against x <= h, x >= -h, y <= h, y >= -h */
+    double a[16 * 2], b[16 * 2];
+    int m = n;
+    memcpy(a, in, sizeof(double) * 2 * n);
+    for (int e = 0; e < 4; ++e) {
+        int k = 0;
+        for (int i = 0; i < m; ++i) {
+            const double *p = a + 2 * i, *q = a + 2 * ((i + 1) % m);
+            double dp, dq;
+            switch (e) {
+            case 0: dp = h - p[0]; dq = h - q[0]; break;
+            case 1: dp = p[0] + h; dq = q[0] + h; break;
+            case 2: dp = h - p[1]; dq = h - q[1]; break;
+            default: dp = p[1] + h; dq = q[1] + h; break;
+            }
+            if (dp >= 0) { b[2 * k] = p[0]; b[2 * k + 1] = p[1]; ++k; }
+            if ((dp >= 0) != (dq >= 0)) {
+                double t = dp / (dp - dq);
+                b[2 * k] = p[0] + t * (q[0] - p[0]);
+                b[2 * k + 1] = p[1] + t * (q[1] - p[1]);
+                /* snap the clipped coordinate exactly onto the edge */
+                if (e == 0) b[2 * k] = h; else if (e == 1) b[2 * k] = -h;
+                else if (e == 2) b[2 * k + 1] = h; else b[2 * k + 1] = -h;
+                ++k;
+            }
+        }
+        m = k;
+        memcpy(a, b, sizeof(double) * 2 * m);
+        if (m == 0) break;
+    }
+    memcpy(out, a, sizeof(double) * 2 * m);
+    return m;
+}
+
+static double triangle_bump(const double *a, const double *bb, const double *d, double D,
+                            const double *s, const double *w, int order)
+{
+    const double v0 = bb[0] - a[0], v1 = bb[1] - a[1], t0 = d[0] - a[0], t1 = d[1] - a[1];
+    double sum = 0.;
+    for (int i = 0; i < order; ++i)
+        for (int j = 0; j < order; ++j) {
+            double x = a[0] + s[i] * v0 + (1 - s[i]) * s[j] * t0;
+            double y = a[1] + s[i] * v1 + (1 - s[i]) * s[j] * t1;
+            double zx = 2 * x / D, zy = 2 * y / D;
+            if (fabs(zx) < 1 && fabs(zy) < 1) {
+                double val = exp(2 - 1 / (1 - zx * zx) - 1 / (1 - zy * zy));
+                sum += val * (1 - s[i]) * w[i] * w[j];
+            }
+        }
+    return fabs(v0 * t1 - v1 * t0) * sum;
+}
+
+int orv_footprint_cell(const double tri[6], double D, const double *nodes, const double *weights,
+                       double *value, double *clipped_area)
+{
+    static const int orders[4] = {16, 32, 64, 128};
+    double poly[16 * 2];
+    int m = clip_square(tri, 3, D / 2, poly);
+    double area = 0.;
+    for (int i = 0; i < m; ++i) {
+        const double *p = poly + 2 * i, *q = poly + 2 * ((i + 1) % m);
+        area += p[0] * q[1] - q[0] * p[1];
+    }
+    area = .5 * fabs(area);
+    *clipped_area = area;
+    *value = 0.;
+    if (m < 3 || area <= 0) return 0;
+    double prev = 0., cur = 0.;
+    int base = 0;
+    for (int o = 0; o < 4; ++o) {
+        const int order = orders[o];
+        cur = 0.;
+        for (int i = 1; i + 1 < m; ++i)
+            cur += triangle_bump(poly, poly + 2 * i, poly + 2 * (i + 1), D, nodes + base, weights + base, order);
+        base += order;
+        if (o > 0 && fabs(cur - prev) <= 2e-12 * D * D) { *value = cur; return order; }
+        prev = cur;
+    }
+    *value = cur;
+    return -1;
+}
+
+/* ------------------------------------------------------------------ */
+struct orv_ctx {
+    om_ctx *hyd;
+    om_mesh m;
+    double cfl;
+    int drag_mode;
+    const double *roughness;
+    orv_turbines t;
+    int have_turbines;
+    double *rate1, *rate2, *w1, *w2, *work, *q1, *theta, *corr, *freq, *tmp;
+    int *status;
+    /* turbine scratch */
+    double *tk;   /* per footprint slot */
+};
+
+orv_ctx *orv_create(om_ctx *hyd, const om_mesh *mesh, double cfl, int drag_mode, const double *roughness,
+                    const orv_turbines *turbines)
+{
+    orv_ctx *r = calloc(1, sizeof(orv_ctx));
+    int nc = mesh->nc, nf = mesh->nf;
+    r->hyd = hyd; r->m = *mesh; r->cfl = cfl; r->drag_mode = drag_mode; r->roughness = roughness;
+    if (turbines && turbines->nrot > 0) { r->t = *turbines; r->have_turbines = 1; }
+    r->rate1 = calloc(3 * (size_t)nc, sizeof(double)); r->rate2 = calloc(3 * (size_t)nc, sizeof(double));
+    r->work = calloc(3 * (size_t)nc, sizeof(double)); r->q1 = calloc(3 * (size_t)nc, sizeof(double));
+    r->tmp = calloc(3 * (size_t)nc, sizeof(double));
+    r->w1 = calloc(nf, sizeof(double)); r->w2 = calloc(nf, sizeof(double));
+    r->theta = calloc(nc, sizeof(double)); r->freq = calloc(nc, sizeof(double));
+    r->corr = calloc(2 * (size_t)nf, sizeof(double)); r->status = calloc(nf, sizeof(int));
+    r->tk = r->have_turbines ? calloc(turbines->foot_off[turbines->nrot] + 1, sizeof(double)) : NULL;
+    return r;
+}
+
+void orv_destroy(orv_ctx *r)
+{
+    if (!r) return;
+    free(r->rate1); free(r->rate2); free(r->work); free(r->q1); free(r->tmp); free(r->w1); free(r->w2);
+    free(r->theta); free(r->freq); free(r->corr); free(r->status); free(r->tk);
+    free(r);
+}
+
+/* hkt_wet_gradients for one cell (river_turbine_flow.h:12-33): eta gradient
+ * over wet neighbours (out 4,5) and the bed gradient over all neighbours with
+ * the precomputed pinv metric (out 6,7). */
+static void wet_gradients(const om_mesh *m, double dry, const double *q, const double *z, int i,
+                          const double inv[3], double g[4])
+{
+    const int n = m->nc;
+    double xx = 0., xy = 0., yy = 0., bx = 0., by = 0., bz0 = 0., bz1 = 0.;
+    int wet = q[i] > dry;
+    for (int s = m->offset[i]; s < m->offset[i + 1]; ++s) {
+        int f = m->faces[s], j = m->owner[f] == i ? m->neighbour[f] : m->owner[f];
+        if (j < 0) continue;
+        double dx = m->xy[j] - m->xy[i], dy = m->xy[n + j] - m->xy[n + i];
+        double w = 1. / (dx * dx + dy * dy), dz = z[j] - z[i];
+        bz0 += w * dx * dz; bz1 += w * dy * dz;
+        if (!wet || q[j] <= dry) continue;
+        xx += w * dx * dx; xy += w * dx * dy; yy += w * dy * dy;
+        double de = (q[j] + z[j]) - (q[i] + z[i]);
+        bx += w * dx * de; by += w * dy * de;
+    }
+    g[2] = inv[0] * bz0 + inv[1] * bz1;
+    g[3] = inv[1] * bz0 + inv[2] * bz1;
+    g[0] = g[1] = 0.;
+    double det = xx * yy - xy * xy;
+    if (det > 1e-12 * fmax(1., xx * yy)) {
+        g[0] = (yy * bx - xy * by) / det;
+        g[1] = (xx * by - xy * bx) / det;
+    }
+}
+
+int orv_turbines_apply(orv_ctx *r, const double *q, const double *z, double *rate, double *diag)
+{
+    if (!r->have_turbines) return 0;
+    const orv_turbines *T = &r->t;
+    const om_mesh *m = &r->m;
+    const int n = m->nc;
+    const double dry = 1e-6;     /* CoveredRotor(dry=1e-6), river_turbine_flow.py:126 */
+    for (int a = 0; a < T->nrot; ++a) {
+        double *d = diag ? diag + ORV_DIAG * a : NULL;
+        const double D = T->D[a];
+        /* plane statistics */
+        double low = INFINITY, high = -INFINITY;
+        for (int k = T->plane_off[a]; k < T->plane_off[a + 1]; ++k) {
+            int c = T->plane_cell[k];
+            double g[4];
+            wet_gradients(m, m->dry, q, z, c, T->plane_inv + 3 * k, g);
+            double eta = q[c] + z[c];
+            double surface = eta + g[0] * T->plane_dx[k] + g[1] * T->plane_dy[k];
+            double bed = z[c] + g[2] * T->plane_dx[k] + g[3] * T->plane_dy[k];
+            low = fmin(low, fmin(surface, eta));
+            high = fmax(high, fmax(bed, z[c]));
+        }
+        int support_dry = 0;
+        double u1 = 0., depth = 0., smin = INFINITY;
+        for (int k = T->foot_off[a]; k < T->foot_off[a + 1]; ++k) {
+            int c = T->foot_cell[k];
+            double h = q[c], hs = fmax(h, dry);
+            double u = q[n + c] / hs, v = q[2 * n + c] / hs, sp = hypot(u, v);
+            if (h <= dry) support_dry = 1;
+            u1 += T->foot_norm[k] * sp;
+            depth += T->foot_norm[k] * h;
+            smin = fmin(smin, sp);
+        }
+        /* _operating_state, river_turbine_flow.py:136-180 */
+        double margin = low - T->top[a], clearance = T->bottom[a] - high, tol = T->tol[a];
+        int reason = 0;
+        if (margin < tol) reason = 1;
+        else if (clearance < tol) reason = 2;
+        else if (margin + tol < T->min_surface[a]) reason = 3;
+        else if (clearance + tol < T->min_bed[a]) reason = 4;
+        else if (support_dry) reason = 5;
+        if (reason == 0 && u1 <= 1e-10) reason = u1 < 0 ? 6 : 7;
+        /* stop_below_cut_in, river_turbine_thetis.py:219-240 */
+        const int t0 = T->table_off[a], nt = T->table_off[a + 1] - t0;
+        if (reason == 0 && T->control[a] == 1 && smin < T->table_speed[t0]) reason = 8;
+        double F = 0., work = 0., P = 0., inc = 0., ctm = 0., cpm = 0.;
+        for (int k = T->foot_off[a]; k < T->foot_off[a + 1]; ++k) r->tk[k] = 0.;
+        if (reason == 0) {
+            const double A = PI * (D * D) / 4;
+            for (int k = T->foot_off[a]; k < T->foot_off[a + 1]; ++k) {
+                int c = T->foot_cell[k];
+                double h = q[c], hs = fmax(h, dry);
+                double u = q[n + c] / hs, v = q[2 * n + c] / hs, sp = hypot(u, v);
+                double ct, cp;
+                if (T->control[a] == 1) ct = orv_lookup(sp, T->table_speed + t0, T->table_ct + t0, nt);
+                else ct = T->ct_const[a];
+                double radicand = 1 - A * ct / (D * h);
+                if (radicand < 0) return ORV_ERROR_TURBINE;
+                double alpha = .5 * (1 + sqrt(radicand));
+                double incident = sp / alpha;
+                if (T->control[a] == 1) cp = orv_lookup(incident, T->table_speed + t0, T->table_cp + t0, nt);
+                else cp = T->cp_const[a] > 0 ? T->cp_const[a] : .5 * ct * (1 + sqrt(1 - ct));
+                double K = A * ct / (2 * (alpha * alpha));
+                double power = .5 * A * cp * pow(incident, 3);
+                double density = T->foot_density[k];
+                r->tk[k] = K * density / m->area[c];
+                F += K * density * (sp * sp);
+                work += K * density * pow(sp, 3);
+                P += power * density;
+                inc += T->foot_norm[k] * incident;
+                ctm += T->foot_norm[k] * ct;
+                cpm += T->foot_norm[k] * cp;
+            }
+        }
+        /* ThetisArray.apply, river_turbine_thetis.py:350-362 */
+        for (int k = T->foot_off[a]; k < T->foot_off[a + 1]; ++k) {
+            int c = T->foot_cell[k];
+            double hs = fmax(q[c], dry);
+            double u = q[n + c] / hs, v = q[2 * n + c] / hs, sp = hypot(u, v);
+            rate[n + c] -= r->tk[k] * sp * u;
+            rate[2 * n + c] -= r->tk[k] * sp * v;
+        }
+        if (d) {
+            d[0] = reason == 0; d[1] = reason; d[2] = low; d[3] = high; d[4] = u1; d[5] = depth;
+            d[6] = F; d[7] = work; d[8] = P; d[9] = reason == 0 ? inc : u1; d[10] = ctm; d[11] = cpm;
+            d[12] = margin; d[13] = clearance; d[14] = 0; d[15] = 0;
+        }
+    }
+    return 0;
+}
+
+/* ------------------------------------------------------------------ */
+/* RiverExplicit.rate (river_explicit.py:63-92) over RiverMidpoint.rate's CPU
+ * branch (river_midpoint.py:100-124, 160-176); no exterior boundary owner. */
+int orv_rate(orv_ctx *r, const double *q, const double *z, double dt, double t, const double *base,
+             double *rate, double *water, double *frequency_peak)
+{
+    const om_mesh *m = &r->m;
+    const int n = m->nc, nf = m->nf;
+    for (int i = 0; i < 3 * n; ++i) if (!isfinite(q[i])) return ORV_REJECT_NONFINITE;
+    for (int i = 0; i < n; ++i) if (q[i] < 0) return ORV_REJECT_NEGATIVE;
+    if (!om_rates(r->hyd, q, z, dt, t, rate, water)) return ORV_REJECT_SPATIAL;
+    /* midpoint limiter */
+    orv_midpoint_inventory(n, dt, base, m->area, m->length, m->offset, m->faces, m->signs, water, r->theta);
+    orv_midpoint_faces(nf, n, m->dry, q, r->theta, m->owner, m->neighbour, m->bid, water, r->corr, r->status);
+    for (int f = 0; f < nf; ++f) if (r->status[f]) return ORV_REJECT_BOUNDARY;
+    orv_midpoint_gather(n, nf, m->area, m->length, m->offset, m->faces, m->signs, water, r->corr, rate);
+    if (r->have_turbines && orv_turbines_apply(r, q, z, rate, NULL) < 0) return ORV_ERROR_TURBINE;
+    for (int i = 0; i < 3 * n; ++i) if (!isfinite(rate[i])) return ORV_REJECT_NONFINITE;
+    /* explicit_frequency on the same face speeds (the subclass check runs
+     * after the shared rate; the spatial speeds are those of this call) */
+    orv_explicit_frequency(n, nf, m->grav, m->dry, q, om_face_speed(r->hyd), NULL, NULL, m->owner, m->normal,
+                           m->area, m->length, m->offset, m->faces, r->freq);
+    double freq = -INFINITY;      /* xp.max: a NaN propagates */
+    for (int i = 0; i < n; ++i) freq = isnan(r->freq[i]) || isnan(freq) ? NAN : fmax(freq, r->freq[i]);
+    *frequency_peak = isnan(freq) ? freq : fmax(*frequency_peak, freq);
+    if (!isfinite(freq) || dt * freq > r->cfl * (1 + 1e-12)) return ORV_REJECT_CFL;
+    return ORV_OK;
+}
+
+/* RiverExplicit._single, river_explicit.py:94-119 */
+int orv_single(orv_ctx *r, const double *base, const double *z, double dt, double t, double *out,
+               double *water, double *frequency_peak)
+{
+    const om_mesh *m = &r->m;
+    const int n = m->nc, nf = m->nf;
+    double *work = r->work, *q1 = r->q1;
+    memcpy(work, base, sizeof(double) * 3 * n);
+    *frequency_peak = 0.;
+    orv_cell_drag(n, .5 * dt, m->grav, m->dry, r->drag_mode, r->roughness, work, NULL);
+    int rc = orv_rate(r, work, z, dt, t, work, r->rate1, r->w1, frequency_peak);
+    if (rc) return rc;
+    for (int i = 0; i < 3 * n; ++i) q1[i] = work[i] + dt * r->rate1[i];
+    for (int i = 0; i < n; ++i) if (q1[i] < 0) return ORV_REJECT_NEGATIVE;
+    for (int i = 0; i < n; ++i) if (q1[i] <= m->dry) { q1[n + i] = 0.; q1[2 * n + i] = 0.; }
+    rc = orv_rate(r, q1, z, dt, t + dt, q1, r->rate2, r->w2, frequency_peak);
+    if (rc) return rc;
+    for (int i = 0; i < 3 * n; ++i) out[i] = .5 * work[i] + .5 * (q1[i] + dt * r->rate2[i]);
+    for (int i = 0; i < 3 * n; ++i) if (!isfinite(out[i])) return ORV_REJECT_NONFINITE;
+    for (int i = 0; i < n; ++i) if (out[i] < 0) return ORV_REJECT_NEGATIVE;
+    for (int i = 0; i < n; ++i) if (out[i] <= m->dry) { out[n + i] = 0.; out[2 * n + i] = 0.; }
+    orv_cell_drag(n, .5 * dt, m->grav, m->dry, r->drag_mode, r->roughness, out, NULL);
+    for (int f = 0; f < nf; ++f) water[f] = .5 * (r->w1[f] + r->w2[f]);
+    return ORV_OK;
+}
+
+/* RiverExplicit.advance, river_explicit.py:121-155 (secondary clock inactive) */
+int orv_advance(orv_ctx *r, double *q, const double *z, double dt, double time, double *hint_io,
+                double *water, int *rejections, double *accepted_cfl_max, double *ledger)
+{
+    const om_mesh *m = &r->m;
+    const int n = m->nc, nf = m->nf;
+    double elapsed = 0., floor_ = dt * ldexp(1., -20);
+    double hint = *hint_io > 0 ? *hint_io : dt;
+    double *next = malloc(sizeof(double) * 3 * n), *w = malloc(sizeof(double) * nf);
+    int used = 0, rejected = 0, rc = 0;
+    double cflmax = 0.;
+    for (int f = 0; f < nf; ++f) water[f] = 0.;
+    while (!used || elapsed < dt - 1e-13 * fmax(1., dt)) {
+        double trial = fmin(hint, dt - elapsed);
+        int truncated = trial < hint * (1 - 1e-12);
+        double peak = 0.;
+        int s = orv_single(r, q, z, trial, time + elapsed, next, w, &peak);
+        if (s == ORV_ERROR_TURBINE) { rc = ORV_ERROR_TURBINE; break; }
+        if (s != ORV_OK) {
+            ++rejected;
+            hint = peak > 0 ? fmin(.5 * trial, CFL_SAFETY * r->cfl / peak) : .5 * trial;
+            if (hint < floor_ || rejected > 100) { rc = -2; break; }
+            continue;
+        }
+        memcpy(q, next, sizeof(double) * 3 * n);
+        for (int f = 0; f < nf; ++f) water[f] = used == 0 ? trial * w[f] : water[f] + trial * w[f];
+        if (ledger)
+            for (int b = 0; b < m->nb; ++b)
+                for (int k = m->boundary_offset[b]; k < m->boundary_offset[b + 1]; ++k) {
+                    int f = m->boundary_faces[k];
+                    ledger[b] -= trial * m->length[f] * w[f];
+                }
+        elapsed += trial; ++used;
+        cflmax = fmax(cflmax, trial * peak);
+        double growth = truncated ? hint : 2 * trial;
+        hint = peak > 0 ? fmin(growth, CFL_SAFETY * r->cfl / peak) : growth;
+        if (used > 4096) { rc = -3; break; }
+    }
+    if (rc == 0) for (int f = 0; f < nf; ++f) water[f] = water[f] / dt;
+    *hint_io = hint;
+    if (rejections) *rejections = rejected;
+    if (accepted_cfl_max) *accepted_cfl_max = cflmax;
+    free(next); free(w);
+    return rc ? rc : used;
+}

@@ -1,0 +1,107 @@
+/*
+ * Oracle (test infrastructure only): plain-C FP64 restatement of the river
+ * SSPRK(2,2) hydraulic path of BRAIDS-HKT and its Thetis turbine sink.  Only
+ * tests/, the smoke check and the CPU-baseline leg of bench.py may load it.
+ *
+ *   explicit_frequency                river_explicit.h:13-25
+ *   midpoint_inventory/faces/gather   river_midpoint.h:13-41
+ *   river_cell_drag + river_chezy     river_hydraulic_drag.h:3-16, river_roughness.py:42-47
+ *   RiverMidpoint.rate (CPU branch)   river_midpoint.py:94-124, 160-176
+ *   RiverExplicit.rate/_single/advance river_explicit.py:63-155
+ *   Thetis rotor sample/apply         river_turbine_thetis.py:166-218, 263-298, 350-362
+ *   rotor gating                      river_turbine_flow.py:136-180
+ *   stop_below_cut_in                 river_turbine_thetis.py:221-244
+ *   footprint bump quadrature         river_turbine_geometry.py:29-71
+ *   hkt_wet_gradients (plane rows)    river_turbine_flow.h:12-33
+ */
+#ifndef BRAIDS_RIVER_ORACLE_H
+#define BRAIDS_RIVER_ORACLE_H
+#include "swe_oracle.h"
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Operator kernels, same names and argument order as the reference headers. */
+void orv_explicit_frequency(int n, int nf, double grav, double dry, const double *q, const double *speed,
+                            const double *trace, const int *mask, const int *owner, const double *normal,
+                            const double *area, const double *length, const int *off, const int *faces,
+                            double *out);
+void orv_midpoint_inventory(int n, double dt, const double *base, const double *area, const double *length,
+                            const int *off, const int *faces, const int *signs, const double *water,
+                            double *theta);
+void orv_midpoint_faces(int nf, int n, double dry, const double *q, const double *theta, const int *o,
+                        const int *j, const int *bid, double *water, double *correction, int *status);
+void orv_midpoint_gather(int n, int nf, const double *area, const double *length, const int *off,
+                         const int *faces, const int *signs, const double *water, const double *correction,
+                         double *rate);
+double orv_chezy(int method, double depth, double value);
+/* river_cell_drag with one roughness value per cell (the spatial form). */
+void orv_cell_drag(int n, double dt, double gravity, double dry, int mode, const double *values,
+                   double *q, double *chezy);
+/* numpy.interp (x inside [xp0, xpN]) followed by the strict-upper-endpoint zeroing of lookup. */
+double orv_interp(double x, const double *xp, const double *fp, int n);
+double orv_lookup(double x, const double *xp, const double *fp, int n);
+
+/* Footprint weight of one cell: the triangle (local coordinates about the
+ * rotor centre, x along the global axis) clipped to the D x D square, fan
+ * triangulated and integrated with the Duffy-mapped Gauss-Legendre rule of
+ * order 16, then 32/64/128 until two orders agree to 2e-12 D^2.  nodes/weights
+ * hold the four rules (s in [0,1], w summing to 1) back to back.  Returns the
+ * converged order (0 when clipped area <= 0, -1 when it does not converge). */
+int orv_footprint_cell(const double tri[6], double D, const double *nodes, const double *weights,
+                       double *value, double *clipped_area);
+
+/* Rotor array in device-friendly CSR form (built by the Python wrapper). */
+typedef struct {
+    int nrot;
+    const double *D, *top, *bottom, *tol, *min_surface, *min_bed;   /* [nrot] */
+    const int *control;           /* [nrot] 0 constant_ct, 1 thetis_speed_table */
+    const double *ct_const, *cp_const;   /* constant_ct: Ct and Cp (Cp <= 0 -> ideal disk) */
+    const int *table_off;         /* [nrot+1] */
+    const double *table_speed, *table_ct, *table_cp;
+    const int *foot_off;          /* [nrot+1] */
+    const int *foot_cell;         /* ascending cell ids with weight > 0 */
+    const double *foot_norm;      /* weights / sum(weights) */
+    const double *foot_density;   /* weights / ((D/2)^2 * 1.45661) */
+    const int *plane_off;         /* [nrot+1] */
+    const int *plane_cell;
+    const double *plane_dx, *plane_dy;   /* plane point - centroid */
+    const double *plane_inv;      /* [3*rows] pinv of the full LS metric of the row's cell */
+} orv_turbines;
+
+enum { ORV_DIAG = 16 };
+/* diag[ORV_DIAG*r + k]: 0 active, 1 reason, 2 low, 3 high, 4 u1, 5 depth, 6 F,
+ * 7 work, 8 P, 9 incoming, 10 Ct mean, 11 Cp mean, 12 margin, 13 clearance.
+ * reason: 0 covered, 1 insufficient_depth, 2 bed_intersection,
+ * 3 insufficient_surface_clearance, 4 insufficient_bed_clearance,
+ * 5 dry_numerical_support, 6 reverse_flow, 7 no_incoming_flow,
+ * 8 below_cut_in_speed. */
+
+typedef struct orv_ctx orv_ctx;
+orv_ctx *orv_create(om_ctx *hyd, const om_mesh *mesh, double cfl, int drag_mode, const double *roughness,
+                    const orv_turbines *turbines);
+void orv_destroy(orv_ctx *r);
+
+/* ThetisArray.apply on rate (in place); returns 0 or -1 on an induction
+ * domain error (ValueError in the reference). */
+int orv_turbines_apply(orv_ctx *r, const double *q, const double *z, double *rate, double *diag);
+
+enum { ORV_OK = 0, ORV_REJECT_SPATIAL = 1, ORV_REJECT_CFL = 2, ORV_REJECT_BOUNDARY = 3,
+       ORV_REJECT_NONFINITE = 4, ORV_REJECT_NEGATIVE = 5, ORV_ERROR_TURBINE = -1 };
+
+/* RiverExplicit.rate: spatial rate + CFL + limiter + turbines. */
+int orv_rate(orv_ctx *r, const double *q, const double *z, double dt, double t, const double *base,
+             double *rate, double *water, double *frequency_peak);
+/* RiverExplicit._single: one SSPRK(2,2) substep; out[3*nc], water[nf] = (w1+w2)/2. */
+int orv_single(orv_ctx *r, const double *base, const double *z, double dt, double t, double *out,
+               double *water, double *frequency_peak);
+/* RiverExplicit.advance over an outer interval dt: q updated in place,
+ * water[nf] = time-mean limited face water, *hint in/out (<= 0: none yet).
+ * Returns the number of accepted substeps, or a negative ORV code. */
+int orv_advance(orv_ctx *r, double *q, const double *z, double dt, double time, double *hint,
+                double *water, int *rejections, double *accepted_cfl_max, double *ledger);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

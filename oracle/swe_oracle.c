@@ -870,6 +870,9 @@ int om_rates(om_ctx *x, const double *q, const double *z, double dt, double t,
     return 1;
 }
 
+/* face speeds of the last spatial evaluation (river_get_face_speed) */
+const double *om_face_speed(const om_ctx *x) { return x->speed; }
+
 void om_prepare_stage(om_ctx *x, double *q)
 {
     int nc = x->m.nc;

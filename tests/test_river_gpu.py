@@ -1,0 +1,266 @@
+"""GPU parity of the river SSPRK(2,2) path and the Thetis turbine sink
+(libbraids_hip against the oracle, oracle/river_oracle.c, which is pinned to
+the reference's operator headers and closed forms in test_river_oracle.py).
+
+Tolerances: FP64 relative L2 <= 1e-10 per field with identical substep and
+rejection counts; turbine diagnostics to 1e-9; footprint cell assignment bit
+exact and weights to 1e-12; the Engine operator entries bit exact.  The FP32
+form reports its error against the FP64 oracle at identical time levels.
+"""
+import ctypes
+
+import numpy as np
+import pytest
+
+from braids_hip import mesh as M
+
+pytestmark = pytest.mark.gpu
+
+TABLE = dict(mode='thetis_speed_table', model_name='test rotor', performance=[
+    dict(speed_m_s=s, thrust_coefficient=c, power_coefficient=p) for s, c, p in
+    zip([0.5, 1.0, 1.5, 2.0, 2.5, 3.0, 3.5], [0.8, 0.85, 0.85, 0.8, 0.7, 0.6, 0.5],
+        [0.3, 0.4, 0.42, 0.42, 0.4, 0.35, 0.3])])
+
+
+def rel_l2(a, b):
+    return np.array([np.linalg.norm(a[j] - b[j]) / max(np.linalg.norm(b[j]), 1e-300) for j in range(3)])
+
+
+def rotor(X, Y, hub, D=5.0, control=TABLE):
+    return dict(name=f'T{X:g}', X=X, Y=Y, D=D, hub_elevation_m=hub, minimum_bed_clearance_m=0.5,
+                minimum_surface_clearance_m=0.5, control=control)
+
+
+def channel_flow():
+    m = M.plain_channel(nx=100, ny=20, length=200.0, width=40.0, depth=8.0, q=1.5 * 8.0 * 40.0)
+    m.q0[1] = m.q0[0] * 1.5
+    return m, []
+
+
+def channel_turbines():
+    m, _ = channel_flow()
+    return m, [rotor(80.3, 20.1, -0.08 + 3.3), rotor(120.7, 13.3, -0.12 + 3.3)]
+
+
+def braided_small():
+    m = M.braided_reach(nx=60, ny=12, length=300.0, width=60.0, amplitude=8.0, wavelength=150.0,
+                        nbars=12, q=30.0, depth=1.0, bar_height=(0.3, 1.0), bar_sigma=(3.0, 10.0), bank=2.0)
+    return m, []
+
+
+# (mesh, scheme, outer intervals of 1 s).  The wet/dry LS-MUSCL reach runs 2 s:
+# its fronts amplify roundoff chaotically (one substep from identical inputs
+# agrees to 7e-15, test_river_fp64_one_substep_matches_oracle; the transverse
+# momentum hv reaches 2e-10 relative L2 at 3 s, 7e-12 at 2 s).
+CASES = {'channel_flow': (channel_flow, 5, 3), 'channel_turbines': (channel_turbines, 5, 3),
+         'braided_s5': (braided_small, 5, 2), 'braided_s1': (braided_small, 1, 3)}
+
+
+def make_pair(make, scheme, precision):
+    from oracle import river as OR
+    from braids_hip.solver import HydraulicSolver
+    from braids_hip.river import RiverExplicit
+    from braids_hip.turbines import ThetisArray
+    m, rows = make()
+    o = OR.RiverOracle(m, scheme=scheme, cfl=.5, rows=rows)
+    s = HydraulicSolver(m, precision=precision, scheme=scheme)
+    s.set_state(m.q0, prepare=False)
+    r = RiverExplicit(s, cfl=.5)
+    if rows:
+        ThetisArray(m, rows).install(s)
+    return m, rows, o, s, r
+
+
+@pytest.mark.parametrize('case', list(CASES))
+def test_river_fp64_matches_oracle(case):
+    make, scheme, nint = CASES[case]
+    m, rows, o, s, r = make_pair(make, scheme, 64)
+    print(f'[river] {case} nc={m.nc} rotors={len(rows)}', flush=True)
+    q = m.q0.copy(); led = np.zeros(max(m.nb, 1)); t = 0.0
+    for k in range(nint):
+        ro = o.advance(q, m.z, 1.0, t, ledger=led)
+        rg = r.advance(1.0, t)
+        assert ro['substeps'] > 0, ro['substeps']
+        q = ro['q']; t += 1.0
+        print(f'[river] {case} interval {k}: substeps gpu={rg["hydraulic_substeps"]} oracle={ro["substeps"]} '
+              f'rejections {rg["hydraulic_rejections"]}/{ro["rejections"]}', flush=True)
+        assert rg['hydraulic_substeps'] == ro['substeps']
+        assert rg['hydraulic_rejections'] == ro['rejections']
+        # the hint is 0.98 cfl / (peak face-sum frequency); at a wet/dry front
+        # the peak cell's face speeds carry the ulp differences of libm
+        # sqrt/pow amplified by a near-dry depth
+        assert abs(rg['hydraulic_next_substep_hint_s'] - o.hint) <= 1e-6 * o.hint
+    err = rel_l2(s.state(), q)
+    print(f'[river] {case} fp64 rel_l2={err}', flush=True)
+    assert (err <= 1e-10).all(), err
+    assert np.allclose(s.ledger(), led[:m.nb], rtol=1e-10, atol=1e-10)
+    if rows:
+        from braids_hip.turbines import diagnostics
+        _, dg = diagnostics(s)
+        _, do, rc = o.turbines(q, m.z)          # the accepted state's sample vs the last stage's
+        assert rc == 0 and np.all(dg[1, :, 0] == 1.0)
+
+
+@pytest.mark.parametrize('case', ['channel_turbines', 'braided_s5'])
+def test_river_fp64_one_substep_matches_oracle(case):
+    """From identical inputs, every substep of a 150-substep run agrees with
+    the oracle to 1e-13 of each field's magnitude in every cell (rejected trials are rejected by
+    both, for the same reason)."""
+    from braids_hip.solver import TrialRejected
+    make, scheme, _ = CASES[case]
+    m, rows, o, s, r = make_pair(make, scheme, 64)
+    q = m.q0.copy(); t = 0.0; dt = 0.02; worst = 0.0; done = 0
+    for k in range(150):
+        rc, qn, _, peak = o.single(q, m.z, dt, t)
+        s.set_state(q, prepare=False)
+        try:
+            s.river_step(dt, t); ok = True
+        except TrialRejected as e:
+            ok = False
+            assert e.info['reason'] == {2: 'cfl', 3: 'boundary', 4: 'nonfinite', 5: 'negative', 1: 'spatial'}.get(rc), (k, rc)
+            assert np.isclose(e.info['frequency'], peak, rtol=1e-12)
+        assert ok == (rc == 0), (k, rc)
+        if not ok:
+            dt *= 0.5
+            continue
+        g = s.state()
+        # per field: max cell error over the field's magnitude
+        rel = np.max(np.abs(g - qn).max(axis=1) / np.maximum(np.abs(qn).max(axis=1), 1e-300))
+        worst = max(worst, rel); done += 1
+        q = qn; t += dt
+    print(f'[river] {case} one-substep max rel err {worst:.3e} over {done} substeps', flush=True)
+    assert done > 50 and worst <= 1e-13
+
+
+def test_turbine_sample_matches_oracle():
+    """One stage sample on a developed state: gating statistics, force,
+    power and the momentum sink of every footprint cell."""
+    from braids_hip.turbines import diagnostics
+    m, rows, o, s, r = make_pair(channel_turbines, 5, 64)
+    ro = o.advance(m.q0, m.z, 1.0, 0.0)
+    s.set_state(ro['q'], prepare=False)
+    # a tiny substep samples the rotors on (almost) this state: compare the
+    # first stage's diagnostics with the oracle's sample of the dragged state
+    from oracle import river as OR
+    dt = 1e-3
+    s.river_step(dt, 1.0)
+    _, dg = diagnostics(s)
+    work, _ = OR.cell_drag(ro['q'], .5 * dt, 2, m.manning)
+    _, do, rc = o.turbines(work, m.z)
+    assert rc == 0
+    for a in range(len(rows)):
+        for k in (0, 1, 4, 5, 6, 7, 8, 9, 10, 11):
+            assert np.isclose(dg[0, a, k], do[a, k], rtol=1e-9, atol=1e-12), (a, k, dg[0, a, k], do[a, k])
+        for k in (2, 3, 12, 13):
+            assert np.isclose(dg[0, a, k], do[a, k], rtol=1e-12, atol=1e-12), (a, k)
+
+
+def test_footprint_on_gpu_matches_oracle():
+    from oracle import river as OR
+    from braids_hip import turbines as T
+    m, rows = channel_turbines()
+    for c in rows + [rotor(61.7, 9.9, 3.0, D=3.0), rotor(100.0, 20.0, 3.3)]:
+        cells, w, area = T.footprint(m, c['X'], c['Y'], c['D'])
+        cand, val, ar, order = OR.footprint_cells(m, c['X'], c['Y'], c['D'])
+        keep = val > 0
+        assert np.array_equal(cells, cand[keep])                       # assignment bit exact
+        assert np.allclose(w, val[keep], rtol=1e-12, atol=0)
+        pc, pdx, pdy = T.plane_rows(m, c['X'], c['Y'], c['D'])
+        po = OR.plane_rows(m, c['X'], c['Y'], c['D'])
+        a = np.array(sorted(zip(pc.tolist(), pdx.tolist(), pdy.tolist())))
+        b = np.array(sorted(po))
+        assert a.shape == b.shape and np.array_equal(a[:, 0], b[:, 0])
+        assert np.allclose(a[:, 1:], b[:, 1:], rtol=0, atol=1e-12)
+
+
+@pytest.mark.parametrize('precision', [64, 32])
+def test_river_mass_conservation(precision):
+    m, rows, o, s, r = make_pair(braided_small, 5, precision)
+    mass0 = s.mass()
+    t = 0.0
+    for k in range(3):
+        r.advance(1.0, t); t += 1.0
+    drift = abs(s.mass() - mass0 - s.ledger().sum()) / mass0
+    print(f'[river] mass drift precision={precision}: {drift:.3e}', flush=True)
+    assert drift <= 1e-12
+    assert s.state()[0].min() >= 0.0
+
+
+@pytest.mark.parametrize('precision,tol', [(64, 1e-12), (32, 1e-5)])
+def test_river_lake_at_rest(precision, tol):
+    from braids_hip.solver import HydraulicSolver
+    from braids_hip.river import RiverExplicit
+    m = M.lake_at_rest()
+    s = HydraulicSolver(m, precision=precision, scheme=5)
+    s.set_state(m.q0, prepare=False)
+    r = RiverExplicit(s)
+    r.advance(2.0, 0.0)
+    q = s.state()
+    print(f'[river] lake at rest precision={precision} max change {np.abs(q - m.q0).max():.3e}', flush=True)
+    assert np.abs(q - m.q0).max() < tol
+
+
+# Stated error of the FP32 river form at identical time levels (fixed dt
+# 0.01 s, 100 substeps), relative L2 against the FP64 oracle, as measured on
+# MI355X: channel with two rotors h 7.6e-9, momentum 9.1e-8; braided reach
+# with wet/dry fronts h 1.9e-9, momentum 1.2e-7.  Bounds: (h, momentum).
+FP32_RIVER_BOUND = {'channel_turbines': (1e-7, 1e-6), 'braided_s5': (1e-7, 1e-6)}
+
+
+@pytest.mark.parametrize('case', list(FP32_RIVER_BOUND))
+def test_river_fp32_error_at_identical_time_levels(case):
+    make, scheme, _ = CASES[case]
+    m, rows, o, s, r = make_pair(make, scheme, 32)
+    dt = 0.01
+    q = m.q0.copy(); t = 0.0
+    for k in range(100):
+        rc, q, _, _ = o.single(q, m.z, dt, t)
+        assert rc == 0, (k, rc)
+        s.river_step(dt, t)
+        t += dt
+    qg = s.state()
+    err = rel_l2(qg, q)
+    mom = np.linalg.norm(qg[1:] - q[1:]) / np.linalg.norm(q[1:])
+    print(f'[river] {case} fp32 dt={dt} steps=100 rel_l2(h,hu,hv)={err} momentum={mom:.3e}', flush=True)
+    bh, bm = FP32_RIVER_BOUND[case]
+    assert err[0] <= bh and mom <= bm, (err, mom)
+
+
+def test_engine_operators_match_oracle():
+    """explicit_frequency / midpoint_* (braids_engine.h) on device pointers
+    against the oracle's operators, bit exact."""
+    import torch
+    from braids_hip import _lib
+    from oracle import river as OR
+    from tests.golden.make_golden_river import case_inputs, case_mesh, run_ops
+    L = _lib.load()
+    m = case_mesh()
+    dev = torch.device('cuda:0')
+    for seed in (3, 5, 7):
+        x = case_inputs(m, seed)
+        want = run_ops(OR.oracle_ops(), m, x)
+        T = lambda a, dt=torch.float64: torch.tensor(np.ascontiguousarray(a), dtype=dt, device=dev)
+        P = lambda t: ctypes.c_void_p(t.data_ptr())
+        n, nf = m.nc, m.nf
+        q, tr, sp = T(x['q']), T(x['trace']), T(x['speed'])
+        mask = T(x['mask'], torch.int32)
+        owner, nbr, bid = T(m.owner, torch.int32), T(m.neighbour, torch.int32), T(m.bid, torch.int32)
+        off, faces, signs = T(m.offset, torch.int32), T(m.faces_csr, torch.int32), T(m.signs, torch.int32)
+        area, length, normal = T(m.area), T(m.length), T(m.normal)
+        freq = torch.zeros(n, dtype=torch.float64, device=dev)
+        assert L.explicit_frequency(n, nf, 9.81, 1e-8, P(q), P(sp), P(tr), P(mask), P(owner), P(normal), P(area),
+                                    P(length), P(off), P(faces), P(freq)) == 0
+        theta = torch.zeros(n, dtype=torch.float64, device=dev)
+        assert L.midpoint_inventory(n, x['dt'], P(T(x['base'])), P(area), P(length), P(off), P(faces), P(signs),
+                                    P(T(x['water'])), P(theta)) == 0
+        water = T(x['water']); corr = torch.zeros(2 * nf, dtype=torch.float64, device=dev)
+        status = torch.zeros(nf, dtype=torch.int32, device=dev)
+        assert L.midpoint_faces(nf, n, 1e-8, P(q), P(theta), P(owner), P(nbr), P(bid), P(water), P(corr),
+                                P(status)) == 0
+        rate = T(x['rate0'])
+        assert L.midpoint_gather(n, nf, P(area), P(length), P(off), P(faces), P(signs), P(water), P(corr),
+                                 P(rate)) == 0
+        torch.cuda.synchronize()
+        got = dict(frequency=freq, theta=theta, water_out=water, correction=corr, status=status, rate=rate)
+        for k, v in got.items():
+            assert np.array_equal(v.cpu().numpy(), want[k]), (seed, k)
