@@ -1,17 +1,25 @@
 """Throughput of the MI355X hydraulic time step (one JSON line on stdout).
 
-A step is one accepted SSPRK(3,3) step of the augmented f-wave solver over the
-whole mesh (three Euler stages: reconstruction, face fluxes, draining, gather,
-Manning), driven by the accepted-step controller with its retries.
+A step is one accepted time step of the whole mesh, all Runge-Kutta stages:
+  river   (default) one SSPRK(2,2) substep of the river owner: Strang drag
+          half-steps, two stages of reconstruction, augmented f-wave face
+          fluxes, face-sum CFL frequency, inventory limiter, Thetis turbine
+          sink and update, driven by the substep hint controller with its
+          rejections;
+  ssp33   one accepted SSPRK(3,3) step of the cut-cell solver (three Euler
+          stages with draining and Manning), with its retries.
 
-Workload (default): the braided synthetic reach, 5 km x 400 m, about 500k
+Workload (default): the braided synthetic reach, 5 km x 400 m, 498,576
 triangles (1768 x 141 quads of 2.83 m, split), sinusoidal thalweg, 200 seeded
 bars with wet/dry fronts, four Manning zones, total-discharge inlet
-Q = 1500 m^3/s and a stage outlet; FP32 flux/update with FP64 depth, bed,
-free-surface differencing and ledgers.  Inputs are resident in HBM before the
-timed region.
+Q = 1500 m^3/s, stage outlet, and 12 rotors of D = 5 m (tabulated Ct/Cp vs
+speed) sited in thalweg scour pools, starting from a 1 m/s stream in water
+deeper than 0.5 m.  FP32
+reconstruction/flux arithmetic with FP64 conserved state, bed, free-surface
+differencing and ledgers.  Inputs are resident in HBM before the timed region.
 
     python bench.py [--gpus 1] [--steps K] [--warmup W] [--precision 32|64]
+                    [--integrator river|ssp33] [--workload NAME]
 """
 import argparse
 import json
@@ -27,9 +35,11 @@ for p in (ROOT, ROOT / 'braids-hkt_amd'):
 
 METRIC = 'cell-updates/s; batched scenarios/h + p50/p99 latency vs plain, rel L2≤1e-6'
 HBM_PEAK_GBS = 8000.0
+STAGE_KINDS = {'ssp33': ('prep', 'traces', 'face', 'drain', 'gather', 'ledger'),
+               'river': ('prep', 'traces', 'face', 'rtheta', 'rupdate')}
 
 
-def algorithmic_bytes(kind, m, precision, scheme, stage_mix=(1, 1, 1)):
+def algorithmic_bytes(kind, m, precision, scheme, nslot=0):
     """Compulsory bytes of one launch: every array element the kernel must
     read or write, counted once (neighbour re-reads, incidence duplication and
     cache misses are implementation traffic and are excluded)."""
@@ -49,11 +59,20 @@ def algorithmic_bytes(kind, m, precision, scheme, stage_mix=(1, 1, 1)):
         return nc * cell + nadj * (8 + 4 * r)
     if kind == 'drain':
         return nc * (4 + 8 + r + 1 + r) + nadj * 8 + nf * (r + r)
+    if kind == 'rtheta':
+        return nc * (4 + 8 + r + 8) + nadj * 5 + nf * 3 * r          # offset, h, area, theta out; incidence; F1, speed, length
+    if kind == 'rupdate':
+        cell = 4 + state + state / 2 + state + r + 8 / 2 + 1 + 8      # offset, in, base (stage 2), out, area, drag (stage 2), mask, theta
+        return nc * cell + nadj * 5 + nf * (5 * r + r + 12)           # fluxes, length, owner/neighbour/bid
+    if kind == 'rdrag':
+        return nc * (state + state + 8)
+    if kind == 'turbine':
+        return nslot * (4 + 8 + 8 + 24 + 8 + 8)                       # footprint cell, weights, state, area, k out
     return 0
 
 
 def stage_bytes(m, precision, scheme):
-    """Compulsory HBM bytes of one whole Euler stage, each array touched once:
+    """Compulsory HBM bytes of one whole stage, each array touched once:
     nc*(6 s_q + 8 + 2 s_g + 2 s_g o2) + nf*(8 + 3 s_g + 2 s_g o2), with s_q the
     conserved-state width (FP64 in both forms here), s_g the geometry width,
     the bed in FP64 and o2 = 1 for LS-MUSCL."""
@@ -76,31 +95,49 @@ def measured_traffic(workload, precision, kind):
 
 def build_workload(name):
     from braids_hip import mesh as M
-    if name == 'braided_reach_500k':
+    rows = []
+    if name == 'braided_reach_500k_12t':
+        m, rows = M.braided_turbine_reach()
+    elif name == 'braided_reach_500k':
         m = M.braided_reach()
     elif name == 'plain_channel_25k':
         m = M.plain_channel()
     else:
         raise SystemExit(f'unknown workload {name}')
-    return m.renumbered('hilbert')
+    return m.renumbered('hilbert'), rows
 
 
-def cpu_baseline(m, scheme, seconds=12.0):
+def cpu_baseline(m, rows, scheme, integrator, seconds=12.0):
     """The oracle (plain-C FP64 restatement, one host thread) on a bounded
     sample of the same workload: whole-mesh accepted steps from the same
     initial state until about `seconds` of CPU work."""
-    from oracle import swe
-    o = swe.Oracle(m, scheme=scheme, cfl_target=0.9, dtmax=1.0)
-    q = m.q0.copy(); t = 0.0; hint = 1.0; steps = 0
-    t0 = time.perf_counter()
-    while time.perf_counter() - t0 < seconds:
-        q, t, hint, n, _, _ = o.advance(q, m.z, 1, time=t, step_hint=hint)
-        steps += n
-        if n == 0:
-            break
+    steps = 0
+    if integrator == 'river':
+        from oracle import river as OR
+        o = OR.RiverOracle(m, scheme=scheme, cfl=.5, rows=rows)
+        q = m.q0.copy(); t = 0.0; o.hint = 0.05
+        t0 = time.perf_counter()
+        while time.perf_counter() - t0 < seconds:
+            dt = o.hint                       # one outer interval per hinted substep
+            r = o.advance(q, m.z, dt, t)
+            if r['substeps'] <= 0:
+                break
+            q = r['q']; t += dt; steps += r['substeps']
+        what = f'SSPRK(2,2) substeps with {len(rows)} rotors'
+    else:
+        from oracle import swe
+        o = swe.Oracle(m, scheme=scheme, cfl_target=0.9, dtmax=1.0)
+        q = m.q0.copy(); t = 0.0; hint = 1.0
+        t0 = time.perf_counter()
+        while time.perf_counter() - t0 < seconds:
+            q, t, hint, n, _, _ = o.advance(q, m.z, 1, time=t, step_hint=hint)
+            steps += n
+            if n == 0:
+                break
+        what = 'SSPRK(3,3) steps'
     el = time.perf_counter() - t0
     return dict(value=m.nc * steps / el, unit='cell-updates/s', cores=1, kind='port',
-                sample=f'{steps} accepted SSPRK(3,3) steps of the full {m.nc}-cell mesh from the initial state, '
+                sample=f'{steps} accepted {what} of the full {m.nc}-cell mesh from the initial state, '
                        f'FP64 oracle restatement, {el:.1f} s on one host core')
 
 
@@ -111,7 +148,8 @@ def main():
     ap.add_argument('--warmup', type=int, default=20)
     ap.add_argument('--precision', type=int, default=32, choices=(32, 64))
     ap.add_argument('--scheme', type=int, default=5, choices=(1, 5))
-    ap.add_argument('--workload', default='braided_reach_500k')
+    ap.add_argument('--integrator', default='river', choices=('river', 'ssp33'))
+    ap.add_argument('--workload', default=None)
     ap.add_argument('--block', type=int, default=256)
     ap.add_argument('--remap', type=int, default=1)
     ap.add_argument('--no-cpu-baseline', action='store_true')
@@ -120,43 +158,66 @@ def main():
         print('bench.py measures one GPU: this build runs the hydraulic step on a single MI355X '
               '(multi-GPU sharding is a design note only); rerun with --gpus 1', file=sys.stderr)
         sys.exit(2)
+    workload = args.workload or ('braided_reach_500k_12t' if args.integrator == 'river' else 'braided_reach_500k')
 
     import ctypes
     import torch
     from braids_hip import _lib
     from braids_hip.solver import HydraulicSolver
 
-    m = build_workload(args.workload)
-    print(f'[bench] workload {args.workload}: {m.nc} cells, {m.nf} faces, precision {args.precision}', file=sys.stderr, flush=True)
+    m, rows = build_workload(workload)
+    print(f'[bench] workload {workload}: {m.nc} cells, {m.nf} faces, {len(rows)} rotors, '
+          f'precision {args.precision}, integrator {args.integrator}', file=sys.stderr, flush=True)
     torch.cuda.init()
     stream = torch.cuda.current_stream()
     s = HydraulicSolver(m, precision=args.precision, scheme=args.scheme, cfl_target=0.9, dtmax=1.0,
                         block=args.block, remap=args.remap, stream=stream)
-    s.set_state(m.q0)
+    nslot = 0
+    if args.integrator == 'river':
+        from braids_hip.river import RiverExplicit
+        from braids_hip.turbines import ThetisArray
+        s.set_state(m.q0, prepare=False)
+        RiverExplicit(s, cfl=.5)
+        if rows:
+            arr = ThetisArray(m, rows).install(s)
+            nslot = sum(len(r.cells) for r in arr.rotors)
+
+        def run(n):
+            return s.river_run(n)
+    else:
+        s.set_state(m.q0)
+
+        def run(n):
+            return s.advance(n)
     # warmup; its last steps time every kernel kind to find the dominant one
-    n, _ = s.advance(max(args.warmup - 5, 0))
+    run(max(args.warmup - 5, 1))
     s.profile(True)
-    s.advance(5)
+    run(5)
     warm = {k: v for k, v in s.profile_read().items() if v[1] > 0}
     dom = max(warm, key=lambda k: warm[k][0])
-    # timed region: HIP events bracket only the dominant kernel's launches
+    # timed region: kernel-stamped HIP events on the dominant kernel only
     s.profile([dom])
     torch.cuda.synchronize()
     t0 = time.perf_counter()
-    acc, retries = s.advance(args.steps)
+    acc, retries = run(args.steps)
     torch.cuda.synchronize()
     el = time.perf_counter() - t0
     prof = s.profile_read()
     s.profile(False)
     q = s.state()
     finite = bool((q == q).all()) and float(q[0].min()) >= 0.0
+    active = None
+    if rows:
+        from braids_hip.turbines import diagnostics
+        active = sum(d['active'] for d in diagnostics(s)[0])
 
-    # dominant kernel and its roofline
-    kinds = {k: v for k, v in warm.items()}
     dom_ms, dom_n = prof[dom]
     avg_s = dom_ms / dom_n / 1e3
-    abytes = algorithmic_bytes(dom, m, args.precision, args.scheme)
+    abytes = algorithmic_bytes(dom, m, args.precision, args.scheme, nslot)
     achieved = abytes / avg_s / 1e9
+    stage_ms = sum(v[0] / v[1] for k, v in warm.items() if k in STAGE_KINDS[args.integrator])
+    sbytes = stage_bytes(m, args.precision, args.scheme)
+    traffic = measured_traffic(workload, args.precision, dom)
 
     # STREAM-copy achievable bandwidth on the same device
     nbytes = 1 << 30
@@ -169,9 +230,6 @@ def main():
     stream_gbs = 2 * nbytes / (ms.value / 1e3) / 1e9
     del src, dst
 
-    stage_ms = sum(v[0] / v[1] for k, v in warm.items() if k in ('prep', 'traces', 'face', 'drain', 'gather', 'ledger'))
-    sbytes = stage_bytes(m, args.precision, args.scheme)
-    traffic = measured_traffic(args.workload, args.precision, dom)
     line = {
         'metric': METRIC,
         'value': m.nc * acc / el,
@@ -185,23 +243,26 @@ def main():
         'vs_baseline': None,
         'dtype': 'f32' if args.precision == 32 else 'f64',
         'data': 'synthetic',
-        'config': {'workload': args.workload, 'cells': m.nc, 'faces': m.nf, 'scheme': args.scheme,
-                   'integrator': 'SSPRK(3,3)', 'cfl': 0.9,
+        'config': {'workload': workload, 'cells': m.nc, 'faces': m.nf, 'scheme': args.scheme,
+                   'integrator': 'SSPRK(2,2) river' if args.integrator == 'river' else 'SSPRK(3,3)',
+                   'cfl': 0.5 if args.integrator == 'river' else 0.9,
                    'precision': ('fp32 reconstruction/flux arithmetic; fp64 conserved state, bed, free-surface '
                                  'differencing and ledgers') if args.precision == 32 else 'fp64',
-                   'turbines': 0, 'renumbering': 'hilbert', 'block': args.block, 'xcd_remap': bool(args.remap),
-                   'retries': retries, 'state_finite_nonnegative': finite},
+                   'turbines': len(rows), 'turbines_active': active, 'renumbering': 'hilbert',
+                   'block': args.block, 'xcd_remap': bool(args.remap),
+                   ('rejections' if args.integrator == 'river' else 'retries'): retries,
+                   'state_finite_nonnegative': finite},
         'roofline': {'bound': 'hbm', 'kernel': dom, 'achieved': achieved, 'peak': HBM_PEAK_GBS, 'unit': 'GB/s',
                      'frac': achieved / HBM_PEAK_GBS, 'traffic': traffic,
                      'algorithmic_bytes_per_launch': abytes, 'avg_launch_us': avg_s * 1e6,
                      'stream_copy_GBs': stream_gbs,
-                     'warmup_kernels_ms_per_launch': {k: v[0] / v[1] for k, v in kinds.items()},
+                     'warmup_kernels_ms_per_launch': {k: v[0] / v[1] for k, v in warm.items()},
                      'stage': {'algorithmic_bytes': sbytes, 'kernels_us': stage_ms * 1e3,
                                'achieved': sbytes / (stage_ms / 1e3) / 1e9}},
     }
     if not args.no_cpu_baseline:
         print('[bench] cpu baseline (oracle on one host core)', file=sys.stderr, flush=True)
-        line['cpu_baseline'] = cpu_baseline(m, args.scheme)
+        line['cpu_baseline'] = cpu_baseline(m, rows, args.scheme, args.integrator)
     print(json.dumps(line), flush=True)
 
 

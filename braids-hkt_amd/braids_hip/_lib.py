@@ -73,6 +73,7 @@ SIGNATURES = {
     'braids_river_configure': (_I, [_P, _D, _I, _P]),
     'braids_river_step': (_I, [_P, _D, _D, ctypes.POINTER(BraidsRiverResult)]),
     'braids_river_advance': (_I, [_P, _D, _D, _P, _P, _P, _P]),
+    'braids_river_run': (_I, [_P, _I, _P, _P, _P, _P]),
     'braids_turbines_set': (_I, [_P, ctypes.POINTER(BraidsTurbines)]),
     'braids_turbines_read': (_I, [_P, _P, _P]),
     'braids_turbine_footprint': (_I, [_I, _P, _P, _P, _P, _P, _P, _P]),

@@ -325,7 +325,8 @@ def plain_channel(nx=500, ny=25, length=2000.0, width=100.0, slope=1e-3, sigma=0
 def braided_reach(nx=1768, ny=141, length=5000.0, width=400.0, slope=1e-3, amplitude=60.0,
                   wavelength=800.0, nbars=200, q=1500.0, seed=2, depth=3.0,
                   bar_height=(0.5, 2.0), bar_sigma=(15.0, 60.0), bank=4.0,
-                  open_min_depth=None) -> Mesh:
+                  open_min_depth=None, pools=(), pool_depth=6.0, pool_sigma=25.0,
+                  velocity=0.0) -> Mesh:
     """Braided synthetic reach 5 km x 400 m: sinusoidal thalweg (amplitude
     60 m, wavelength 800 m), parabolic cross-section, 200 seeded Gaussian bars
     (height 0.5-2 m, sigma 15-60 m, centred away from the inlet and outlet
@@ -344,6 +345,8 @@ def braided_reach(nx=1768, ny=141, length=5000.0, width=400.0, slope=1e-3, ampli
         z = -slope * x + bank * ((y - thalweg(x)) / (0.5 * width)) ** 2
         for k in range(nbars):
             z = z + bh[k] * np.exp(-((x - bx[k]) ** 2 + (y - by[k]) ** 2) / (2 * bs[k] ** 2))
+        for (px, py) in pools:          # scour pools that site the turbines
+            z = z - pool_depth * np.exp(-((x - px) ** 2 + (y - py) ** 2) / (2 * pool_sigma ** 2))
         return z
 
     def manning(x, y):
@@ -362,7 +365,43 @@ def braided_reach(nx=1768, ny=141, length=5000.0, width=400.0, slope=1e-3, ampli
     shallow = (m.bid >= 0) & (m.q0[0][m.owner] < thr)
     m.bid[shallow] = -1
     m.build_incidence()
+    if velocity:
+        # a stream of `velocity` m/s in water deeper than 0.5 m; the shallow
+        # margins start at rest
+        m.q0[1] = np.where(m.q0[0] > 0.5, m.q0[0] * velocity, 0.0)
+    m.thalweg = thalweg
     return m
+
+
+THETIS_TABLE = dict(mode='thetis_speed_table', model_name='synthetic 5 m rotor', performance=[
+    dict(speed_m_s=s, thrust_coefficient=c, power_coefficient=p) for s, c, p in
+    zip([0.5, 1.0, 1.5, 2.0, 2.5, 3.0, 3.5], [0.80, 0.85, 0.85, 0.80, 0.70, 0.60, 0.50],
+        [0.30, 0.40, 0.42, 0.42, 0.40, 0.35, 0.30])])
+
+
+def turbine_sites(n=12, length=5000.0, width=400.0, amplitude=60.0, wavelength=800.0, first=0.3, spacing=200.0):
+    """n rotor sites on the thalweg, spaced along the reach from first*length."""
+    xs = first * length + spacing * np.arange(n)
+    ys = 0.5 * width + amplitude * np.sin(2 * np.pi * xs / wavelength)
+    return [(float(x), float(y)) for x, y in zip(xs, ys)]
+
+
+def braided_turbine_reach(nturbines=12, D=5.0, velocity=1.0, **kw):
+    """The braided reach with nturbines D-diameter rotors (tabulated Ct/Cp
+    vs speed) sited in scour pools on the thalweg, and an initial stream of
+    `velocity` m/s (in water deeper than 0.5 m) so the rotors start covered
+    and running.  Returns the mesh
+    and the turbine rows (rotor bottom 1 m above the highest bed cell
+    around the site)."""
+    sites = turbine_sites(nturbines, **{k: kw[k] for k in ('length', 'width', 'amplitude', 'wavelength') if k in kw})
+    m = braided_reach(pools=sites, velocity=velocity, **kw)
+    rows = []
+    for k, (x, y) in enumerate(sites):
+        near = (np.abs(m.xy[0] - x) < D) & (np.abs(m.xy[1] - y) < D)
+        zbed = float(m.z[near].max())
+        rows.append(dict(name=f'T{k + 1:02d}', X=x, Y=y, D=D, hub_elevation_m=zbed + 1.0 + D / 2,
+                         minimum_bed_clearance_m=0.5, minimum_surface_clearance_m=0.5, control=THETIS_TABLE))
+    return m, rows
 
 
 def lake_at_rest(nx=20, ny=6, dx=4.0, dy=4.0, stage=1.0, seed=3, bumps=True) -> Mesh:

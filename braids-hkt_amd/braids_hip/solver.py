@@ -184,6 +184,18 @@ class HydraulicSolver:
         self.time = t0 + float(dt)
         return n.value, rej.value, cmax.value
 
+    def river_run(self, nsteps, hint=None):
+        """Exactly nsteps accepted substeps from self.time; returns (accepted, rejections)."""
+        t = ctypes.c_double(self.time)
+        h = ctypes.c_double(hint or getattr(self, 'river_hint', None) or 0.01)
+        acc = ctypes.c_int(); rej = ctypes.c_int()
+        rc = _lib.check(self.L.braids_river_run(self.h, int(nsteps), ctypes.byref(t), ctypes.byref(h),
+                                                ctypes.byref(acc), ctypes.byref(rej)), 'braids_river_run')
+        self.time, self.river_hint = t.value, h.value
+        if rc == _lib.BRAIDS_REJECTED:
+            raise TrialRejected(_lib.last_error())
+        return acc.value, rej.value
+
     # -- measurement --------------------------------------------------------
     def profile(self, kinds=True):
         """kinds: True (all kernel kinds), False (off) or an iterable of kind names."""

@@ -132,7 +132,7 @@ struct SolverBase {
     {
         if (events.empty()) return;
         static const bool ssp[KK_COUNT] = {true, true, true, true, true, true, false, false, false, false};
-        static const bool riv[KK_COUNT] = {true, true, true, false, false, false, true, true, true, true};
+        static const bool riv[KK_COUNT] = {true, true, true, false, false, false, true, true, false, true};
         for (int slot = 0; slot < slots; ++slot)
             for (int k = 0; k < KK_COUNT; ++k) {
                 if (!(river ? riv[k] : ssp[k])) continue;
@@ -159,7 +159,7 @@ struct Solver : SolverBase {
     Control *ctl = nullptr;            // device block: Control + 4*nb ledger doubles
     double *ledger_dev = nullptr;
     void *host_block = nullptr;        // pinned readback
-    size_t block_bytes = 0;
+    size_t block_bytes = 0, head_bytes = 0;
     LaunchCfg cfg;
     double *qscratch = nullptr;        // [3*nc] double for conversions
     RiverBuffers rb;
@@ -294,13 +294,17 @@ struct Solver : SolverBase {
         if ((rc = alloc(qscratch, 3 * (size_t)nc))) return rc;
         HIP_TRY(hipMemset(w.grad, 0, 8 * (size_t)nc * sizeof(R)));
         HIP_TRY(hipMemset(w.trace, 0, 3 * (size_t)nf * sizeof(R)));
-        block_bytes = sizeof(Control) + 4 * (size_t)(nb + 1) * sizeof(double);
-        block_bytes = (block_bytes + 255) & ~(size_t)255;
+        // device block: Control, 4 stage ledgers (read back every step), then
+        // the partial-maximum slots (device only); zeroed together per step
+        head_bytes = sizeof(Control) + 4 * (size_t)(nb + 1) * sizeof(double);
+        const size_t part_off = (head_bytes + 255) & ~(size_t)255;
+        block_bytes = part_off + 2 * 4 * (size_t)RPART * sizeof(unsigned long long);
         char *blk = nullptr;
         if ((rc = alloc(blk, block_bytes))) return rc;
         ctl = (Control *)blk;
         ledger_dev = (double *)(blk + sizeof(Control));
-        HIP_TRY(hipHostMalloc(&host_block, block_bytes, hipHostMallocDefault));
+        w.rpart = (unsigned long long *)(blk + part_off);
+        HIP_TRY(hipHostMalloc(&host_block, head_bytes, hipHostMallocDefault));
         ledger.assign(nb, 0.0);
         hstate.resize(nc);
         return BRAIDS_OK;
@@ -340,8 +344,9 @@ struct Solver : SolverBase {
         run_stage<R>(m, st[0], st[0], st[1], z, w, ctl, ledger_dev, 0, dt, t, cfg, stream);
         run_stage<R>(m, st[1], st[0], st[2], z, w, ctl, ledger_dev + (nb + 1), 1, dt, t + dt, cfg, stream);
         run_stage<R>(m, st[2], st[0], st[3], z, w, ctl, ledger_dev + 2 * (nb + 1), 2, dt, t + .5 * dt, cfg, stream);
+        run_finalize<R>(w, ctl, stream);
         HIP_TRY(hipGetLastError());
-        HIP_TRY(hipMemcpyAsync(host_block, ctl, block_bytes, hipMemcpyDeviceToHost, stream));
+        HIP_TRY(hipMemcpyAsync(host_block, ctl, head_bytes, hipMemcpyDeviceToHost, stream));
         HIP_TRY(hipStreamSynchronize(stream));
         if (profiling) collect(3);
         const Control *hc = (const Control *)host_block;
@@ -437,7 +442,8 @@ struct Solver : SolverBase {
             uslot[fill[u]++] = k;
         }
         std::vector<uint8_t> mask(nc, 0);
-        for (int c : uniq) mask[c] = 1;
+        std::vector<int> uidx(nc, -1);
+        for (size_t u = 0; u < uniq.size(); ++u) { mask[uniq[u]] = 1; uidx[uniq[u]] = (int)u; }
         TurbineView T{};
         T.nrot = n; T.nslot = nslot; T.nunion = (int)uniq.size();
         if ((rc = upload(T.D, dv(t->D, n)))) return rc;
@@ -467,10 +473,9 @@ struct Solver : SolverBase {
         if ((rc = upload(T.union_slot, uslot))) return rc;
         if ((rc = upload(T.xy, xy_host))) return rc;
         if ((rc = upload(T.tmask, mask))) return rc;
+        if ((rc = upload(T.tuidx, uidx))) return rc;
         if ((rc = alloc(T.kslot, nslot))) return rc;
         if ((rc = alloc(T.diag, 2 * (size_t)n * TURBINE_DIAG))) return rc;
-        if ((rc = alloc(T.tsink, 2 * (size_t)nc))) return rc;
-        HIP_TRY(hipMemset(T.tsink, 0, 2 * (size_t)nc * sizeof(double)));
         HIP_TRY(hipMemset(T.diag, 0, 2 * (size_t)n * TURBINE_DIAG * sizeof(double)));
         HIP_TRY(hipHostMalloc(&diag_host, 2 * (size_t)n * TURBINE_DIAG * sizeof(double), hipHostMallocDefault));
         tv = T;
@@ -486,7 +491,7 @@ struct Solver : SolverBase {
         zero_control();
         run_river_substep<R>(m, st[0], st[1], st[2], st[3], z, w, ctl, rb, nrot > 0 ? &tv : nullptr, dt, t, cfg, stream);
         HIP_TRY(hipGetLastError());
-        HIP_TRY(hipMemcpyAsync(host_block, ctl, block_bytes, hipMemcpyDeviceToHost, stream));
+        HIP_TRY(hipMemcpyAsync(host_block, ctl, head_bytes, hipMemcpyDeviceToHost, stream));
         if (nrot > 0)
             HIP_TRY(hipMemcpyAsync(diag_host, tv.diag, 2 * (size_t)nrot * TURBINE_DIAG * sizeof(double),
                                    hipMemcpyDeviceToHost, stream));
@@ -805,6 +810,41 @@ extern "C" int braids_river_advance(braids_solver *s, double dt, double time, do
     }
     *step_hint = hint;
     if (accepted_cfl_max) *accepted_cfl_max = cflmax;
+    return BRAIDS_OK;
+}
+
+extern "C" int braids_river_run(braids_solver *s, int nsteps, double *time, double *step_hint, int *accepted,
+                                int *rejections)
+{
+    if (!s || !time || !step_hint || !accepted || nsteps < 0 || !(*step_hint > 0)) {
+        set_error("invalid run arguments"); return BRAIDS_EINVAL;
+    }
+    SolverBase *x = s->impl;
+    const double safety = .98;
+    double hint = *step_hint;
+    int rejected = 0;
+    *accepted = 0;
+    while (*accepted < nsteps) {
+        const double trial = hint;
+        braids_river_result r{};
+        int rc = x->river_step(trial, *time, &r);
+        if (rc < 0) { *step_hint = hint; if (rejections) *rejections = rejected; return rc; }
+        const double peak = r.frequency_peak;
+        if (rc == BRAIDS_REJECTED) {
+            ++rejected;
+            hint = peak > 0 ? std::fmin(.5 * trial, safety * x->river_cfl / peak) : .5 * trial;
+            if (rejected > 100 * (*accepted + 1)) {
+                *step_hint = hint; if (rejections) *rejections = rejected;
+                set_error("river substeps keep being rejected"); return BRAIDS_REJECTED;
+            }
+            continue;
+        }
+        *time = *time + trial;
+        ++*accepted;
+        hint = peak > 0 ? std::fmin(2 * trial, safety * x->river_cfl / peak) : 2 * trial;
+    }
+    *step_hint = hint;
+    if (rejections) *rejections = rejected;
     return BRAIDS_OK;
 }
 

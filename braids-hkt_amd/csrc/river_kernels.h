@@ -13,12 +13,13 @@
 //
 // On the device a stage is: k_prep, k_traces, k_face (shared with the
 // SSPRK(3,3) path), then
-//   k_river_theta   cells: inventory factor theta and the CFL frequency
-//   k_turbine_sample / k_turbine_apply   one workgroup per rotor (gating
-//                   statistics, Ct/Cp lookup, induction) and the per-cell sink
+//   k_river_theta   cells: inventory factor theta and the CFL frequency;
+//                   extra workgroups: one wavefront per rotor (gating
+//                   statistics, Ct/Cp lookup, induction, K per footprint cell)
 //   k_river_update  cells: the native rate, the limiter applied to each
-//                   incident face on the fly, the sink, the SSP combination,
-//                   positivity, dry momentum and (last stage) the drag.
+//                   incident face on the fly, the turbine sink, the SSP
+//                   combination, positivity, dry momentum and (last stage)
+//                   the drag.
 // Nothing is synchronised with the host inside a substep: rejections are
 // flags in the Control block, read once per substep.
 #pragma once
@@ -59,15 +60,181 @@ __global__ void __launch_bounds__(256) k_river_drag(int nc, StateView<R> in, Sta
     out.h[c] = h; out.hu[c] = hu; out.hv[c] = hv;
 }
 
+// ---------------------------------------------------------------------------
+// Thetis turbine sink
+
+__device__ __forceinline__ double interp_table(double x, const double *xp, const double *fp, int n)
+{
+    // numpy.interp, then the strict upper endpoint of lookup
+    // (river_turbine_thetis.py:166-179)
+    if (x < xp[0] || x >= xp[n - 1]) return 0.;
+    int lo = 0, hi = n - 1;
+    while (hi - lo > 1) {
+        const int mid = (lo + hi) >> 1;
+        if (x >= xp[mid]) lo = mid; else hi = mid;
+    }
+    const double slope = (fp[lo + 1] - fp[lo]) / (xp[lo + 1] - xp[lo]);
+    return slope * (x - xp[lo]) + fp[lo];
+}
+
+__device__ __forceinline__ double wave_sum_d(double v)
+{
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+    return v;
+}
+__device__ __forceinline__ double wave_min_d(double v)
+{
+    for (int o = 32; o > 0; o >>= 1) v = fmin(v, __shfl_xor(v, o, 64));
+    return v;
+}
+
+// One wavefront per rotor: CoveredRotor._operating_state gating
+// (river_turbine_flow.py:136-180) on the rotor-plane surface/bed samples and
+// the footprint means, stop_below_cut_in (river_turbine_thetis.py:219-240),
+// then cell_terms (:182-218) per footprint cell.  All reductions are
+// butterfly shuffles, so every lane holds the rotor's statistics and takes
+// the same gating decision; the statistics the reference moves to the host
+// (5 + 6 scalars per rotor and stage) stay on the device.
+template <class R>
+__device__ void turbine_sample_wave(const TurbineView &T, const MeshView<R> &m, StateView<R> q,
+                                    const double *__restrict__ z, int slot, int a, int lane)
+{
+    const int nc = m.nc;
+    const double dry = 1e-6;                 // CoveredRotor(dry=1e-6)
+    const double mdry = (double)m.dry;
+    // rotor plane: wet LS surface gradient and full LS bed gradient of the
+    // row's cell (hkt_wet_gradients, river_turbine_flow.h:12-33)
+    double low = 1e300, high = -1e300;
+    for (int k = T.plane_off[a] + lane; k < T.plane_off[a + 1]; k += 64) {
+        const int i = T.plane_cell[k];
+        double xx = 0., xy = 0., yy = 0., bx = 0., by = 0., bz0 = 0., bz1 = 0.;
+        const bool wet = q.h[i] > mdry;
+        for (int s = m.offset[i]; s < m.offset[i + 1]; ++s) {
+            const int j = m.adj_other[s];
+            if (j < 0) continue;
+            const double dx = T.xy[j] - T.xy[i], dy = T.xy[nc + j] - T.xy[nc + i];
+            const double w = 1. / (dx * dx + dy * dy), dz = z[j] - z[i];
+            bz0 += w * dx * dz; bz1 += w * dy * dz;
+            if (!wet || q.h[j] <= mdry) continue;
+            xx += w * dx * dx; xy += w * dx * dy; yy += w * dy * dy;
+            const double de = (q.h[j] + z[j]) - (q.h[i] + z[i]);
+            bx += w * dx * de; by += w * dy * de;
+        }
+        const double *inv = T.plane_inv + 3 * k;
+        const double gzx = inv[0] * bz0 + inv[1] * bz1, gzy = inv[1] * bz0 + inv[2] * bz1;
+        double gex = 0., gey = 0.;
+        const double det = xx * yy - xy * xy;
+        if (det > 1e-12 * fmax(1., xx * yy)) { gex = (yy * bx - xy * by) / det; gey = (xx * by - xy * bx) / det; }
+        const double eta = q.h[i] + z[i];
+        const double surface = eta + gex * T.plane_dx[k] + gey * T.plane_dy[k];
+        const double bed = z[i] + gzx * T.plane_dx[k] + gzy * T.plane_dy[k];
+        low = fmin(low, fmin(surface, eta));
+        high = fmax(high, fmax(bed, z[i]));
+    }
+    low = wave_min_d(low);
+    high = -wave_min_d(-high);
+    // footprint means
+    double u1 = 0., depth = 0., smin = 1e300, anydry = 0.;
+    for (int k = T.foot_off[a] + lane; k < T.foot_off[a + 1]; k += 64) {
+        const int c = T.foot_cell[k];
+        const double h = q.h[c], hs = fmax(h, dry);
+        const double u = q.hu[c] / hs, v = q.hv[c] / hs, sp = hypot(u, v);
+        if (h <= dry) anydry = 1.;
+        u1 += T.foot_norm[k] * sp;
+        depth += T.foot_norm[k] * h;
+        smin = fmin(smin, sp);
+    }
+    u1 = wave_sum_d(u1);
+    depth = wave_sum_d(depth);
+    smin = wave_min_d(smin);
+    anydry = -wave_min_d(-anydry);
+    const double margin = low - T.top[a], clearance = T.bottom[a] - high;
+    const int t0 = T.table_off[a], nt = T.table_off[a + 1] - t0;
+    const double tol = T.tol[a];
+    int reason = 0;
+    if (margin < tol) reason = 1;
+    else if (clearance < tol) reason = 2;
+    else if (margin + tol < T.min_surface[a]) reason = 3;
+    else if (clearance + tol < T.min_bed[a]) reason = 4;
+    else if (anydry > 0.) reason = 5;
+    if (reason == 0 && u1 <= 1e-10) reason = u1 < 0 ? 6 : 7;
+    if (reason == 0 && T.control[a] == 1 && smin < T.table_speed[t0]) reason = 8;
+    const double D = T.D[a], Aro = 3.14159265358979323846 * (D * D) / 4;
+    double F = 0., work = 0., P = 0., inc = 0., ctm = 0., cpm = 0.;
+    int err = 0;
+    for (int k = T.foot_off[a] + lane; k < T.foot_off[a + 1]; k += 64) {
+        double kk = 0.;
+        if (reason == 0) {
+            const int c = T.foot_cell[k];
+            const double h = q.h[c], hs = fmax(h, dry);
+            const double u = q.hu[c] / hs, v = q.hv[c] / hs, sp = hypot(u, v);
+            const double ct = T.control[a] == 1 ? interp_table(sp, T.table_speed + t0, T.table_ct + t0, nt) : T.ct_const[a];
+            const double radicand = 1 - Aro * ct / (D * h);
+            if (radicand < 0) err = 1;
+            const double alpha = .5 * (1 + sqrt(fmax(radicand, 0.)));
+            const double incident = sp / alpha;
+            double cp;
+            if (T.control[a] == 1) cp = interp_table(incident, T.table_speed + t0, T.table_cp + t0, nt);
+            else cp = T.cp_const[a] > 0 ? T.cp_const[a] : .5 * ct * (1 + sqrt(1 - ct));
+            const double K = Aro * ct / (2 * (alpha * alpha));
+            const double density = T.foot_density[k];
+            kk = K * density / (double)m.area[c];
+            F += K * density * (sp * sp);
+            work += K * density * pow(sp, 3.0);
+            P += .5 * Aro * cp * pow(incident, 3.0) * density;
+            inc += T.foot_norm[k] * incident;
+            ctm += T.foot_norm[k] * ct;
+            cpm += T.foot_norm[k] * cp;
+        }
+        T.kslot[k] = kk;
+    }
+    if (err) atomicOr(T.error, 1 << slot);
+    F = wave_sum_d(F); work = wave_sum_d(work); P = wave_sum_d(P);
+    inc = wave_sum_d(inc); ctm = wave_sum_d(ctm); cpm = wave_sum_d(cpm);
+    if (lane == 0) {
+        double *d = T.diag + ((size_t)slot * T.nrot + a) * TURBINE_DIAG;
+        d[0] = reason == 0; d[1] = reason; d[2] = low; d[3] = high; d[4] = u1; d[5] = depth;
+        d[6] = F; d[7] = work; d[8] = P; d[9] = reason == 0 ? inc : u1; d[10] = ctm; d[11] = cpm;
+        d[12] = margin; d[13] = clearance; d[14] = 0.; d[15] = 0.;
+    }
+}
+
+// ThetisArray.apply (river_turbine_thetis.py:350-362) for one cell: the
+// momentum sink k |u| (u, v) of every rotor covering it, in rotor order.
+__device__ __forceinline__ void turbine_sink(const TurbineView &T, int u, double h, double hu, double hv,
+                                             double &sx, double &sy)
+{
+    const double hs = fmax(h, 1e-6);
+    const double uu = hu / hs, vv = hv / hs, sp = hypot(uu, vv);
+    sx = 0.; sy = 0.;
+    for (int e = T.union_off[u]; e < T.union_off[u + 1]; ++e) {
+        const double k = T.kslot[T.union_slot[e]];
+        sx = sx + k * sp * uu;
+        sy = sy + k * sp * vv;
+    }
+}
+
 // midpoint_inventory (river_midpoint.h:13-21) with the stage input as the
 // available inventory, and explicit_frequency (river_explicit.h:13-25, no
 // exterior owner) max-reduced into the stage's CFL slot.
+// Workgroups past the ncb cell blocks sample the rotors, one wavefront each.
 template <class R>
 __global__ void __launch_bounds__(256) k_river_theta(MeshView<R> m, StateView<R> q, Work<R> w, double *theta,
-                                                     Control *ctl, int slot, double dt, int remap)
+                                                     Control *ctl, int slot, double dt, int remap,
+                                                     TurbineView T, const double *__restrict__ z, int ncb)
 {
     const int nf = m.nf;
-    const int c = logical_block(remap) * blockDim.x + threadIdx.x;
+    if ((int)blockIdx.x >= ncb) {
+        const int a = ((int)blockIdx.x - ncb) * (blockDim.x >> 6) + (threadIdx.x >> 6);
+        if (a < T.nrot) turbine_sample_wave<R>(T, m, q, z, slot, a, threadIdx.x & 63);
+        return;
+    }
+    int lb = blockIdx.x;
+    if (remap) {          // XCD-aware order over the cell blocks only
+        const int n = ncb, qq = n >> 3, r8 = n & 7, x = lb & 7, k = lb >> 3;
+        lb = x * qq + (x < r8 ? x : r8) + k;
+    }
+    const int c = lb * blockDim.x + threadIdx.x;
     double freq = 0.;
     if (c < m.nc) {
         double out = 0., sum = 0.;
@@ -85,7 +252,8 @@ __global__ void __launch_bounds__(256) k_river_theta(MeshView<R> m, StateView<R>
         freq = sum / A;
         if (!(freq == freq)) freq = 1e300;     // a NaN frequency rejects the trial
     }
-    block_max_to(freq, &ctl->peak_bits[slot]);
+    (void)ctl;
+    block_max_to(freq, w.rpart + rpart_index(RQ_PEAK, slot));
 }
 
 // STAGE 0: out = in + dt r (in is the dragged base);  STAGE 1: out = in0/2 +
@@ -93,9 +261,8 @@ __global__ void __launch_bounds__(256) k_river_theta(MeshView<R> m, StateView<R>
 template <class R, int STAGE>
 __global__ void __launch_bounds__(256) k_river_update(MeshView<R> m, StateView<R> in, StateView<R> in0,
                                                       StateView<R> out, Work<R> w, const double *__restrict__ theta,
-                                                      const double *__restrict__ tsink, const uint8_t *__restrict__ tmask,
-                                                      double *wlim, DragView d, Control *ctl, int slot,
-                                                      double dt, double drag_dt, int remap)
+                                                      TurbineView T, double *wlim, DragView d, Control *ctl,
+                                                      int slot, double dt, double drag_dt, int remap)
 {
     const int nc = m.nc, nf = m.nf;
     const int c = logical_block(remap) * blockDim.x + threadIdx.x;
@@ -148,7 +315,11 @@ __global__ void __launch_bounds__(256) k_river_update(MeshView<R> m, StateView<R
         }
         (void)s0;
         double r0 = mass, r1 = -s1 + mx, r2 = -s2 + my;
-        if (tsink && tmask[c]) { r1 = r1 - tsink[c]; r2 = r2 - tsink[nc + c]; }
+        if (T.nrot > 0 && T.tmask[c]) {
+            double sx, sy;
+            turbine_sink(T, T.tuidx[c], in.h[c], in.hu[c], in.hv[c], sx, sy);
+            r1 = r1 - sx; r2 = r2 - sy;
+        }
         double nh, nhu, nhv;
         if (STAGE == 0) {
             nh = in.h[c] + dt * r0; nhu = in.hu[c] + dt * r1; nhv = in.hv[c] + dt * r2;
@@ -182,179 +353,6 @@ __global__ void __launch_bounds__(TRACE_BLOCK) k_river_ledger(MeshView<R> m, con
     }
     s = block_sum<TRACE_BLOCK>(s, sh);
     if (threadIdx.x == 0) ledger[b] = s;
-}
-
-// ---------------------------------------------------------------------------
-// Thetis turbine sink
-
-__device__ __forceinline__ double interp_table(double x, const double *xp, const double *fp, int n)
-{
-    // numpy.interp, then the strict upper endpoint of lookup
-    // (river_turbine_thetis.py:166-179)
-    if (x < xp[0] || x >= xp[n - 1]) return 0.;
-    int lo = 0, hi = n - 1;
-    while (hi - lo > 1) {
-        const int mid = (lo + hi) >> 1;
-        if (x >= xp[mid]) lo = mid; else hi = mid;
-    }
-    const double slope = (fp[lo + 1] - fp[lo]) / (xp[lo + 1] - xp[lo]);
-    return slope * (x - xp[lo]) + fp[lo];
-}
-
-template <int BLOCK>
-__device__ __forceinline__ double block_min(double v, double *sh)
-{
-    sh[threadIdx.x] = v;
-    __syncthreads();
-    for (int s = BLOCK / 2; s > 0; s >>= 1) {
-        if ((int)threadIdx.x < s) sh[threadIdx.x] = fmin(sh[threadIdx.x], sh[threadIdx.x + s]);
-        __syncthreads();
-    }
-    const double r = sh[0];
-    __syncthreads();
-    return r;
-}
-
-// One workgroup per rotor: CoveredRotor._operating_state gating
-// (river_turbine_flow.py:136-180) on the rotor-plane surface/bed samples and
-// the footprint means, stop_below_cut_in (river_turbine_thetis.py:219-240),
-// then cell_terms (:182-218) per footprint cell.  The statistics the
-// reference moves to the host (5 + 6 scalars per rotor and stage) stay on the
-// device.
-template <class R>
-__global__ void __launch_bounds__(TRACE_BLOCK) k_turbine_sample(TurbineView T, MeshView<R> m, StateView<R> q,
-                                                                const double *__restrict__ z, int slot)
-{
-    const int a = blockIdx.x;
-    if (a >= T.nrot) return;
-    __shared__ double sh[TRACE_BLOCK];
-    __shared__ int s_reason;
-    const int nc = m.nc;
-    const double dry = 1e-6;                 // CoveredRotor(dry=1e-6)
-    const double mdry = (double)m.dry;
-    // rotor plane: wet LS surface gradient and full LS bed gradient of the
-    // row's cell (hkt_wet_gradients, river_turbine_flow.h:12-33)
-    double low = 1e300, high = -1e300;
-    for (int k = T.plane_off[a] + threadIdx.x; k < T.plane_off[a + 1]; k += TRACE_BLOCK) {
-        const int i = T.plane_cell[k];
-        double xx = 0., xy = 0., yy = 0., bx = 0., by = 0., bz0 = 0., bz1 = 0.;
-        const bool wet = q.h[i] > mdry;
-        for (int s = m.offset[i]; s < m.offset[i + 1]; ++s) {
-            const int j = m.adj_other[s];
-            if (j < 0) continue;
-            const double dx = T.xy[j] - T.xy[i], dy = T.xy[nc + j] - T.xy[nc + i];
-            const double w = 1. / (dx * dx + dy * dy), dz = z[j] - z[i];
-            bz0 += w * dx * dz; bz1 += w * dy * dz;
-            if (!wet || q.h[j] <= mdry) continue;
-            xx += w * dx * dx; xy += w * dx * dy; yy += w * dy * dy;
-            const double de = (q.h[j] + z[j]) - (q.h[i] + z[i]);
-            bx += w * dx * de; by += w * dy * de;
-        }
-        const double *inv = T.plane_inv + 3 * k;
-        const double gzx = inv[0] * bz0 + inv[1] * bz1, gzy = inv[1] * bz0 + inv[2] * bz1;
-        double gex = 0., gey = 0.;
-        const double det = xx * yy - xy * xy;
-        if (det > 1e-12 * fmax(1., xx * yy)) { gex = (yy * bx - xy * by) / det; gey = (xx * by - xy * bx) / det; }
-        const double eta = q.h[i] + z[i];
-        const double surface = eta + gex * T.plane_dx[k] + gey * T.plane_dy[k];
-        const double bed = z[i] + gzx * T.plane_dx[k] + gzy * T.plane_dy[k];
-        low = fmin(low, fmin(surface, eta));
-        high = fmax(high, fmax(bed, z[i]));
-    }
-    low = block_min<TRACE_BLOCK>(low, sh);
-    high = -block_min<TRACE_BLOCK>(-high, sh);
-    // footprint means
-    double u1 = 0., depth = 0., smin = 1e300, anydry = 0.;
-    for (int k = T.foot_off[a] + threadIdx.x; k < T.foot_off[a + 1]; k += TRACE_BLOCK) {
-        const int c = T.foot_cell[k];
-        const double h = q.h[c], hs = fmax(h, dry);
-        const double u = q.hu[c] / hs, v = q.hv[c] / hs, sp = hypot(u, v);
-        if (h <= dry) anydry = 1.;
-        u1 += T.foot_norm[k] * sp;
-        depth += T.foot_norm[k] * h;
-        smin = fmin(smin, sp);
-    }
-    u1 = block_sum<TRACE_BLOCK>(u1, sh);
-    depth = block_sum<TRACE_BLOCK>(depth, sh);
-    smin = block_min<TRACE_BLOCK>(smin, sh);
-    anydry = -block_min<TRACE_BLOCK>(-anydry, sh);
-    const double margin = low - T.top[a], clearance = T.bottom[a] - high;
-    const int t0 = T.table_off[a], nt = T.table_off[a + 1] - t0;
-    if (threadIdx.x == 0) {
-        const double tol = T.tol[a];
-        int reason = 0;
-        if (margin < tol) reason = 1;
-        else if (clearance < tol) reason = 2;
-        else if (margin + tol < T.min_surface[a]) reason = 3;
-        else if (clearance + tol < T.min_bed[a]) reason = 4;
-        else if (anydry > 0.) reason = 5;
-        if (reason == 0 && u1 <= 1e-10) reason = u1 < 0 ? 6 : 7;
-        if (reason == 0 && T.control[a] == 1 && smin < T.table_speed[t0]) reason = 8;
-        s_reason = reason;
-    }
-    __syncthreads();
-    const int reason = s_reason;
-    const double D = T.D[a], Aro = 3.14159265358979323846 * (D * D) / 4;
-    double F = 0., work = 0., P = 0., inc = 0., ctm = 0., cpm = 0.;
-    int err = 0;
-    for (int k = T.foot_off[a] + threadIdx.x; k < T.foot_off[a + 1]; k += TRACE_BLOCK) {
-        double kk = 0.;
-        if (reason == 0) {
-            const int c = T.foot_cell[k];
-            const double h = q.h[c], hs = fmax(h, dry);
-            const double u = q.hu[c] / hs, v = q.hv[c] / hs, sp = hypot(u, v);
-            const double ct = T.control[a] == 1 ? interp_table(sp, T.table_speed + t0, T.table_ct + t0, nt) : T.ct_const[a];
-            const double radicand = 1 - Aro * ct / (D * h);
-            if (radicand < 0) err = 1;
-            const double alpha = .5 * (1 + sqrt(fmax(radicand, 0.)));
-            const double incident = sp / alpha;
-            double cp;
-            if (T.control[a] == 1) cp = interp_table(incident, T.table_speed + t0, T.table_cp + t0, nt);
-            else cp = T.cp_const[a] > 0 ? T.cp_const[a] : .5 * ct * (1 + sqrt(1 - ct));
-            const double K = Aro * ct / (2 * (alpha * alpha));
-            const double density = T.foot_density[k];
-            kk = K * density / (double)m.area[c];
-            F += K * density * (sp * sp);
-            work += K * density * pow(sp, 3.0);
-            P += .5 * Aro * cp * pow(incident, 3.0) * density;
-            inc += T.foot_norm[k] * incident;
-            ctm += T.foot_norm[k] * ct;
-            cpm += T.foot_norm[k] * cp;
-        }
-        T.kslot[k] = kk;
-    }
-    if (err) atomicOr(T.error, 1 << slot);
-    F = block_sum<TRACE_BLOCK>(F, sh);
-    work = block_sum<TRACE_BLOCK>(work, sh);
-    P = block_sum<TRACE_BLOCK>(P, sh);
-    inc = block_sum<TRACE_BLOCK>(inc, sh);
-    ctm = block_sum<TRACE_BLOCK>(ctm, sh);
-    cpm = block_sum<TRACE_BLOCK>(cpm, sh);
-    if (threadIdx.x == 0) {
-        double *d = T.diag + ((size_t)slot * T.nrot + a) * TURBINE_DIAG;
-        d[0] = reason == 0; d[1] = reason; d[2] = low; d[3] = high; d[4] = u1; d[5] = depth;
-        d[6] = F; d[7] = work; d[8] = P; d[9] = reason == 0 ? inc : u1; d[10] = ctm; d[11] = cpm;
-        d[12] = margin; d[13] = clearance; d[14] = 0.; d[15] = 0.;
-    }
-}
-
-// ThetisArray.apply (river_turbine_thetis.py:350-362): the momentum sink
-// k |u| (u, v) of every rotor covering the cell, in rotor order.
-template <class R>
-__global__ void __launch_bounds__(256) k_turbine_apply(TurbineView T, StateView<R> q, int nc)
-{
-    const int u = blockIdx.x * blockDim.x + threadIdx.x;
-    if (u >= T.nunion) return;
-    const int c = T.union_cell[u];
-    const double hs = fmax(q.h[c], 1e-6);
-    const double uu = q.hu[c] / hs, vv = q.hv[c] / hs, sp = hypot(uu, vv);
-    double sx = 0., sy = 0.;
-    for (int e = T.union_off[u]; e < T.union_off[u + 1]; ++e) {
-        const double k = T.kslot[T.union_slot[e]];
-        sx = sx + k * sp * uu;
-        sy = sy + k * sp * vv;
-    }
-    T.tsink[c] = sx; T.tsink[nc + c] = sy;
 }
 
 // Footprint weights (river_turbine_geometry.py:29-71): one workgroup per

@@ -8,6 +8,7 @@ template void run_stage<float>(const MeshView<float> &, StateView<float>, StateV
                                const LaunchCfg &, hipStream_t);
 template void run_rates<float>(const MeshView<float> &, StateView<float>, const double *, Work<float>, Control *,
                                double *, double *, double, const LaunchCfg &, hipStream_t);
+template void run_finalize<float>(Work<float>, Control *, hipStream_t);
 template void run_river_substep<float>(const MeshView<float> &, StateView<float>, StateView<float>, StateView<float>,
                                         StateView<float>, const double *, Work<float>, Control *, const RiverBuffers &,
                                         const TurbineView *, double, double, const LaunchCfg &, hipStream_t);

@@ -43,9 +43,11 @@ __device__ __forceinline__ int wave_or(int v)
 }
 
 // Block-level max of a non-negative double, then one device atomic per
-// workgroup (every thread of the block must call it).
-__device__ __forceinline__ void block_max_to(double v, unsigned long long *target)
+// workgroup into the block's slot of a partial-maximum array (every thread of
+// the block must call it).
+__device__ __forceinline__ void block_max_to(double v, unsigned long long *slots)
 {
+    unsigned long long *target = slots + (blockIdx.x % RSLOTS) * RSTRIDE;
     __shared__ double part[16];
     v = wave_max(v);
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
@@ -73,6 +75,27 @@ __device__ __forceinline__ void block_or_to(int v, int *target)
     }
 }
 
+// Maximum over the slots of a partial-maximum array.
+__device__ __forceinline__ double slot_max(const unsigned long long *slots)
+{
+    unsigned long long m = 0;
+    for (int i = 0; i < RSLOTS; ++i) { const unsigned long long v = slots[i * RSTRIDE]; m = v > m ? v : m; }
+    return __longlong_as_double((long long)m);
+}
+
+// Reduce the partial maxima into the Control block (one thread per quantity
+// and stage).
+static __global__ void __launch_bounds__(64) k_finalize(const unsigned long long *rpart, Control *ctl)
+{
+    const int t = threadIdx.x;
+    if (t >= 8) return;
+    const int q = t >> 2, s = t & 3;
+    const unsigned long long *p = rpart + rpart_index(q, s);
+    unsigned long long m = 0;
+    for (int i = 0; i < RSLOTS; ++i) { const unsigned long long v = p[i * RSTRIDE]; m = v > m ? v : m; }
+    if (q == RQ_HMAX) ctl->hmax_bits[s] = m; else ctl->peak_bits[s] = m;
+}
+
 // XCD-aware block remap: consecutive logical blocks (neighbouring mesh
 // ranges in Hilbert order) are kept on one XCD so they share its L2.
 __device__ __forceinline__ int logical_block(int remap)
@@ -88,7 +111,7 @@ __device__ __forceinline__ int logical_block(int remap)
 // ---------------------------------------------------------------------------
 template <class R, int SCHEME>
 __global__ void __launch_bounds__(256) k_prep(MeshView<R> m, StateView<R> q, const double *__restrict__ z,
-                                              Work<R> w, Control *ctl, int stage, int remap)
+                                              Work<R> w, Control *ctl, int stage, int remap, int want_hmax)
 {
     const int nc = m.nc;
     int c = logical_block(remap) * blockDim.x + threadIdx.x;
@@ -188,13 +211,14 @@ __global__ void __launch_bounds__(256) k_prep(MeshView<R> m, StateView<R> q, con
         }
         w.flags[c] = (uint8_t)flags;
     }
-    block_max_to(hmx > 0.0 ? hmx : 0.0, &ctl->hmax_bits[stage]);
+    (void)ctl;
+    if (want_hmax) block_max_to(hmx > 0.0 ? hmx : 0.0, w.rpart + rpart_index(RQ_HMAX, stage));
 }
 
 // FP64-exact form of the LS-MUSCL preparation (same operation order as the
 // reference: primitives, then adjacent - centre differences in FP64).
 static __global__ void __launch_bounds__(256) k_prep_ls64(MeshView<double> m, StateView<double> q, const double *__restrict__ z,
-                                                        Work<double> w, Control *ctl, int stage, int remap)
+                                                        Work<double> w, Control *ctl, int stage, int remap, int want_hmax)
 {
     const int nc = m.nc;
     int c = logical_block(remap) * blockDim.x + threadIdx.x;
@@ -277,7 +301,8 @@ static __global__ void __launch_bounds__(256) k_prep_ls64(MeshView<double> m, St
         for (int j = 0; j < 4; ++j) { w.grad[(2 * j) * nc + c] = gx[j]; w.grad[(2 * j + 1) * nc + c] = gy[j]; }
         w.flags[c] = (uint8_t)flags;
     }
-    block_max_to(hmx > 0.0 ? hmx : 0.0, &ctl->hmax_bits[stage]);
+    (void)ctl;
+    if (want_hmax) block_max_to(hmx > 0.0 ? hmx : 0.0, w.rpart + rpart_index(RQ_HMAX, stage));
 }
 
 // face_value (cutcell_solver.f90:8999-9051) for schemes 1 and 5.
@@ -810,7 +835,7 @@ __global__ void __launch_bounds__(256) k_gather(MeshView<R> m, StateView<R> in, 
             nhu = in.hu[c] - dt * s1;
             nhv = in.hv[c] - dt * s2;
             if (nh < 0) {
-                const double hmax = __longlong_as_double((long long)ctl->hmax_bits[STAGE]);
+                const double hmax = slot_max(w.rpart + rpart_index(RQ_HMAX, STAGE));
                 if (-nh <= 64 * 2.2204460492503131e-16 * vmax(hmax, 1.0)) { nh = 0; nhu = 0; nhv = 0; }
                 else fail = fail > 3 ? fail : 3;
             }
@@ -847,7 +872,7 @@ __global__ void __launch_bounds__(256) k_gather(MeshView<R> m, StateView<R> in, 
         }
         if (STAGE != 3) { out.h[c] = nh; out.hu[c] = nhu; out.hv[c] = nhv; }
     }
-    block_max_to(ratio > 0.0 ? ratio : 0.0, &ctl->peak_bits[STAGE]);
+    block_max_to(ratio > 0.0 ? ratio : 0.0, w.rpart + rpart_index(RQ_PEAK, STAGE));
     if (fail) { atomicMax(&ctl->failure[STAGE], fail); atomicMax(&ctl->fail_cell[STAGE], c + 1); }
 }
 

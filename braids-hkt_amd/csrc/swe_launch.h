@@ -28,6 +28,10 @@ void run_stage(const MeshView<R> &m, StateView<R> in, StateView<R> base, StateVi
                const double *z, Work<R> w, Control *ctl, double *ledger_stage, int mode,
                double dt, double t, const LaunchCfg &cfg, hipStream_t stream);
 
+// Reduce the partial maxima of the stages into the Control block.
+template <class R>
+void run_finalize(Work<R> w, Control *ctl, hipStream_t stream);
+
 // Spatial rate without draining or CFL rejection (bend_native_rates):
 // rate = -sum (L/A)(F - 1/2 g h^2 n); water = fl(1,:).
 template <class R>

@@ -28,13 +28,15 @@ static inline void timed_launch(const LaunchCfg &cfg, int slot, int kind, K kern
 
 template <class R, int SCHEME>
 static void launch_prep(const MeshView<R> &m, StateView<R> in, const double *z, Work<R> w, Control *ctl,
-                        int slot, const LaunchCfg &cfg, hipStream_t s)
+                        int slot, const LaunchCfg &cfg, hipStream_t s, int want_hmax = 1)
 {
     const int B = cfg.block;
     if constexpr (std::is_same<R, double>::value && SCHEME == 5)
-        timed_launch(cfg, slot, KK_PREP, k_prep_ls64, dim3(grid_for(m.nc, B)), dim3(B), s, m, in, z, w, ctl, slot, cfg.remap);
+        timed_launch(cfg, slot, KK_PREP, k_prep_ls64, dim3(grid_for(m.nc, B)), dim3(B), s, m, in, z, w, ctl, slot,
+                     cfg.remap, want_hmax);
     else
-        timed_launch(cfg, slot, KK_PREP, k_prep<R, SCHEME>, dim3(grid_for(m.nc, B)), dim3(B), s, m, in, z, w, ctl, slot, cfg.remap);
+        timed_launch(cfg, slot, KK_PREP, k_prep<R, SCHEME>, dim3(grid_for(m.nc, B)), dim3(B), s, m, in, z, w, ctl,
+                     slot, cfg.remap, want_hmax);
 }
 
 template <class R, int SCHEME>
@@ -67,6 +69,13 @@ static void stage_kernels(const MeshView<R> &m, StateView<R> in, StateView<R> ba
     }
     if (m.nb > 0 && ledger_stage)
         timed_launch(cfg, mode, KK_LEDGER, k_ledger<R>, gb, tb, s, m, w, ledger_stage, dt);
+}
+
+// The maxima of the partial slots into the Control block.
+template <class R>
+void run_finalize(Work<R> w, Control *ctl, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_finalize, dim3(1), dim3(64), 0, s, (const unsigned long long *)w.rpart, ctl);
 }
 
 template <class R>
@@ -106,25 +115,20 @@ static void river_stage(const MeshView<R> &m, StateView<R> in, StateView<R> in0,
 {
     const int B = cfg.block;
     const dim3 gc(grid_for(m.nc, B)), blk(B), gb(m.nb), tb(TRACE_BLOCK);
-    launch_prep<R, SCHEME>(m, in, z, w, ctl, slot, cfg, s);
+    launch_prep<R, SCHEME>(m, in, z, w, ctl, slot, cfg, s, 0);     // the river path uses no max depth
     if (m.nb > 0) timed_launch(cfg, slot, KK_TRACES, k_traces<R, SCHEME>, gb, tb, s, m, in, z, w, ctl, slot, t);
     launch_face<R, SCHEME>(m, in, z, w, slot, cfg, s);
-    timed_launch(cfg, slot, KK_RTHETA, k_river_theta<R>, gc, blk, s, m, in, w, rb.theta, ctl, slot, dt, cfg.remap);
-    const double *tsink = nullptr;
-    const uint8_t *tmask = nullptr;
-    if (tv && tv->nrot > 0) {
-        TurbineView T = *tv;
-        T.error = &ctl->turbine_error;
-        timed_launch(cfg, slot, KK_TURBINE, k_turbine_sample<R>, dim3(T.nrot), tb, s, T, m, in, z, slot);
-        hipLaunchKernelGGL((k_turbine_apply<R>), dim3(grid_for(T.nunion, 256)), dim3(256), 0, s, T, in, m.nc);
-        tsink = T.tsink; tmask = T.tmask;
-    }
+    TurbineView T{};
+    if (tv && tv->nrot > 0) { T = *tv; T.error = &ctl->turbine_error; }
+    const int ncb = grid_for(m.nc, B), nrb = (T.nrot + (B / 64) - 1) / (B / 64);
+    timed_launch(cfg, slot, KK_RTHETA, k_river_theta<R>, dim3(ncb + nrb), blk, s, m, in, w, rb.theta, ctl, slot, dt,
+                 cfg.remap, T, z, ncb);
     if (slot == 0)
         timed_launch(cfg, slot, KK_RUPDATE, k_river_update<R, 0>, gc, blk, s, m, in, in0, out, w,
-                     (const double *)rb.theta, tsink, tmask, rb.wlim[0], rb.drag, ctl, slot, dt, 0.0, cfg.remap);
+                     (const double *)rb.theta, T, rb.wlim[0], rb.drag, ctl, slot, dt, 0.0, cfg.remap);
     else
         timed_launch(cfg, slot, KK_RUPDATE, k_river_update<R, 1>, gc, blk, s, m, in, in0, out, w,
-                     (const double *)rb.theta, tsink, tmask, rb.wlim[1], rb.drag, ctl, slot, dt, .5 * dt, cfg.remap);
+                     (const double *)rb.theta, T, rb.wlim[1], rb.drag, ctl, slot, dt, .5 * dt, cfg.remap);
 }
 
 template <class R>
@@ -146,6 +150,7 @@ void run_river_substep(const MeshView<R> &m, StateView<R> base, StateView<R> wor
     if (m.nb > 0)
         hipLaunchKernelGGL((k_river_ledger<R>), dim3(m.nb), dim3(TRACE_BLOCK), 0, s, m,
                            (const double *)rb.wlim[0], (const double *)rb.wlim[1], rb.ledger, dt);
+    run_finalize<R>(w, ctl, s);
 }
 
 }  // namespace braids

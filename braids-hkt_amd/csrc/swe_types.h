@@ -43,7 +43,16 @@ template <class R> struct StateView {
     double *hu, *hv;
 };
 
+// Partial maxima of the block reductions: RSLOTS slots per quantity and
+// stage, one 128-byte line apart, so that the per-workgroup atomics of one
+// launch are spread over many addresses instead of serialising on one.
+constexpr int RSLOTS = 32, RSTRIDE = 16;
+constexpr int RPART = RSLOTS * RSTRIDE;      // u64 per (quantity, stage)
+enum : int { RQ_HMAX = 0, RQ_PEAK = 1 };
+__host__ __device__ inline int rpart_index(int quantity, int stage) { return (quantity * 4 + stage) * RPART; }
+
 template <class R> struct Work {
+    unsigned long long *rpart;   // [2][4][RPART] partial maxima (bits of non-negative doubles)
     R *grad;       // [8*nc]
     uint8_t *flags;// [nc]
     R *trace;      // [3*nf]
@@ -56,7 +65,7 @@ template <class R> struct Work {
 // Per-slot reductions; slots 0-2 are the SSPRK(3,3) stages, slot 3 a
 // single plain Euler stage.
 struct Control {
-    unsigned long long hmax_bits[4];   // max depth of the stage input (non-negative)
+    unsigned long long hmax_bits[4];   // max depth of the stage input (non-negative), see k_finalize
     unsigned long long peak_bits[4];   // max CFL ratio over active cells
     int failure[4];                    // encoded failure (max), see FAIL_*
     int drain_any[4];                  // any draining cell
@@ -70,7 +79,7 @@ struct DragView {
     const double *values;   // [nc] roughness per cell
 };
 
-constexpr int TURBINE_DIAG = 16;   // per rotor and stage, see k_turbine_sample
+constexpr int TURBINE_DIAG = 16;   // per rotor and stage, see turbine_sample_wave
 
 struct TurbineView {
     int nrot, nslot, nunion;
@@ -86,8 +95,8 @@ struct TurbineView {
     const double *xy;                     // [2*nc] centroids
     double *kslot;                        // [nslot] K density / area (0 when the rotor is off)
     double *diag;                         // [2][nrot][TURBINE_DIAG]
-    double *tsink;                        // [2*nc] (entries of union cells only)
-    const uint8_t *tmask;                 // [nc] 1 on union cells
+    const uint8_t *tmask;                 // [nc] 1 on cells covered by a rotor footprint
+    const int *tuidx;                     // [nc] index into union_cell of such a cell
     int *error;                           // induction domain error flag
 };
 

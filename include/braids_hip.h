@@ -141,6 +141,12 @@ int braids_river_step(braids_solver *s, double dt, double t, braids_river_result
 int braids_river_advance(braids_solver *s, double dt, double time, double *step_hint, int *substeps,
                          int *rejections, double *accepted_cfl_max);
 
+/* Exactly nsteps accepted substeps with the same hint controller and no
+ * outer interval (no reference counterpart: the throughput loop of bench and
+ * serving callers); *time and *step_hint (> 0) in/out. */
+int braids_river_run(braids_solver *s, int nsteps, double *time, double *step_hint, int *accepted,
+                     int *rejections);
+
 /* Thetis rotor array (ThetisArray, river_turbine_thetis.py:300-362) in CSR
  * form: footprint cells with normalized weights and density integrals
  * (weights / ((D/2)^2 1.45661), :253), rotor-plane sample rows (cell,

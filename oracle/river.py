@@ -123,10 +123,10 @@ def plane_rows(mesh, X, Y, D):
     v = mesh.verts
     y0, y1 = Y - D / 2, Y + D / 2
     rows = []
-    for i in range(mesh.nc):
+    near = np.flatnonzero((v[[0, 2, 4]].min(0) <= X) & (v[[0, 2, 4]].max(0) >= X) &
+                          (v[[1, 3, 5]].max(0) >= y0) & (v[[1, 3, 5]].min(0) <= y1))
+    for i in near:
         P = v[:, i].reshape(3, 2)
-        if not (P[:, 0].min() <= X <= P[:, 0].max()):
-            continue
         ys = []
         for e in range(3):
             a, b = P[e], P[(e + 1) % 3]
