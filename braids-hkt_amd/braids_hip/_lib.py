@@ -77,6 +77,12 @@ SIGNATURES = {
     'braids_turbines_set': (_I, [_P, ctypes.POINTER(BraidsTurbines)]),
     'braids_turbines_read': (_I, [_P, _P, _P]),
     'braids_turbine_footprint': (_I, [_I, _P, _P, _P, _P, _P, _P, _P]),
+    'braids_batch_create': (_I, [_I, _P, _P]),
+    'braids_batch_destroy': (_I, [_P]),
+    'braids_batch_add': (_I, [_P, _P, _I, _D, _D, _P]),
+    'braids_batch_remove': (_I, [_P, _I, _P]),
+    'braids_batch_run': (_I, [_P, _I]),
+    'braids_batch_poll': (_I, [_P, _P, _I, _P]),
     'braids_profile': (_I, [_P, _I]),
     'braids_profile_read': (_I, [_P, _P, _P]),
     'braids_stream_copy': (_I, [_P, _P, ctypes.c_longlong, _P, _I, _P]),

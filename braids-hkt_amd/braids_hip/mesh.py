@@ -386,18 +386,20 @@ def turbine_sites(n=12, length=5000.0, width=400.0, amplitude=60.0, wavelength=8
     return [(float(x), float(y)) for x, y in zip(xs, ys)]
 
 
-def braided_turbine_reach(nturbines=12, D=5.0, velocity=1.0, **kw):
+def braided_turbine_reach(nturbines=12, D=5.0, velocity=1.0, spacing=200.0, **kw):
     """The braided reach with nturbines D-diameter rotors (tabulated Ct/Cp
     vs speed) sited in scour pools on the thalweg, and an initial stream of
     `velocity` m/s (in water deeper than 0.5 m) so the rotors start covered
     and running.  Returns the mesh
     and the turbine rows (rotor bottom 1 m above the highest bed cell
     around the site)."""
-    sites = turbine_sites(nturbines, **{k: kw[k] for k in ('length', 'width', 'amplitude', 'wavelength') if k in kw})
+    sites = turbine_sites(nturbines, spacing=spacing,
+                          **{k: kw[k] for k in ('length', 'width', 'amplitude', 'wavelength') if k in kw})
     m = braided_reach(pools=sites, velocity=velocity, **kw)
     rows = []
     for k, (x, y) in enumerate(sites):
-        near = (np.abs(m.xy[0] - x) < D) & (np.abs(m.xy[1] - y) < D)
+        r = max(D, 1.5 * max(m.dx, m.dy))
+        near = (np.abs(m.xy[0] - x) < r) & (np.abs(m.xy[1] - y) < r)
         zbed = float(m.z[near].max())
         rows.append(dict(name=f'T{k + 1:02d}', X=x, Y=y, D=D, hub_elevation_m=zbed + 1.0 + D / 2,
                          minimum_bed_clearance_m=0.5, minimum_surface_clearance_m=0.5, control=THETIS_TABLE))

@@ -1,0 +1,324 @@
+"""Continuous-batching serving of river scenario solves on one MI355X.
+
+Requests (scenario solves of uneven mesh sizes, discharges and turbine
+layouts) arrive over time.  The scheduler admits each one as soon as a slot
+and HBM budget are free, adds it to a running batch, and keeps enqueueing
+batched substeps.  Every kernel of a substep is one launch over all admitted
+scenarios, each scenario's step controller runs on the device, and the
+substep is a replayed graph, so admitting or retiring a scenario never
+stalls the others beyond one table rebuild.  Each scenario runs a fixed
+number of accepted SSPRK(2,2) substeps; its latency is completion minus
+arrival.
+
+The baseline ("plain per-solve GPU runs") solves the same requests one at a
+time in arrival order with the single-scenario path (HydraulicSolver +
+braids_river_run).
+
+    python -m braids_hip.serve [--requests 256] [--rate 4] [--steps 2000] ...
+"""
+import argparse
+import ctypes
+import json
+import math
+import sys
+import time
+from dataclasses import dataclass, field
+
+import numpy as np
+
+from . import _lib
+from . import mesh as M
+from .solver import HydraulicSolver
+
+
+class SlotStatus(ctypes.Structure):
+    _fields_ = [('id', ctypes.c_int), ('accepted', ctypes.c_int), ('target', ctypes.c_int),
+                ('rejections', ctypes.c_int), ('done', ctypes.c_int), ('reason', ctypes.c_int),
+                ('time', ctypes.c_double), ('hint', ctypes.c_double), ('cfl_max', ctypes.c_double)]
+
+
+class Batch:
+    """ctypes owner of a braids_batch."""
+
+    def __init__(self, precision=32, stream=None):
+        self.L = _lib.load()
+        h = ctypes.c_void_p()
+        raw = getattr(stream, 'cuda_stream', stream)
+        _lib.check(self.L.braids_batch_create(int(precision), ctypes.c_void_p(int(raw) if raw else 0),
+                                              ctypes.byref(h)), 'braids_batch_create')
+        self.h = h
+        self.members = {}
+
+    def close(self):
+        if getattr(self, 'h', None):
+            self.L.braids_batch_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def add(self, solver, steps, time=0.0, hint=0.01):
+        i = ctypes.c_int()
+        _lib.check(self.L.braids_batch_add(self.h, solver.h, int(steps), float(time), float(hint), ctypes.byref(i)),
+                   'braids_batch_add')
+        self.members[i.value] = solver
+        return i.value
+
+    def remove(self, sid):
+        st = SlotStatus()
+        _lib.check(self.L.braids_batch_remove(self.h, int(sid), ctypes.byref(st)), 'braids_batch_remove')
+        solver = self.members.pop(sid)
+        solver.time = st.time
+        solver.river_hint = st.hint
+        return st
+
+    def run(self, substeps):
+        _lib.check(self.L.braids_batch_run(self.h, int(substeps)), 'braids_batch_run')
+
+    def poll(self):
+        n = len(self.members)
+        arr = (SlotStatus * max(n, 1))()
+        cnt = ctypes.c_int()
+        _lib.check(self.L.braids_batch_poll(self.h, arr, n, ctypes.byref(cnt)), 'braids_batch_poll')
+        return [arr[i] for i in range(cnt.value)]
+
+
+# ---------------------------------------------------------------------------
+# Config-4 requests
+@dataclass
+class Request:
+    rid: int
+    arrival: float
+    geometry: str
+    cells: int
+    q: float
+    turbines: int
+    steps: int
+    mesh: object = None
+    rows: list = field(default_factory=list)
+
+
+def _channel(nx, ny, q, depth=7.0):
+    m = M.plain_channel(nx=nx, ny=ny, length=2000.0, width=100.0, depth=depth, q=q)
+    m.q0[1] = m.q0[0] * (q / (100.0 * depth))
+    return m
+
+
+def pooled_meshes(classes=8, lo=20_000, hi=400_000, seed=7):
+    """Mesh pool: `classes` log-spaced sizes in [lo, hi] for each geometry.
+    Channels are 2000 m x 100 m at 7 m depth; braided reaches are the 5 km x
+    400 m reach with 16 rotor pools on the thalweg."""
+    pool = {}
+    sizes = np.geomspace(lo, hi, classes)
+    for k, n in enumerate(sizes):
+        # channel: 2 nx ny = n with nx/ny = 20
+        ny = max(2, int(round(math.sqrt(n / 40.0)))); nx = max(4, int(round(n / (2 * ny))))
+        pool[('channel', k)] = ('channel', nx, ny)
+        # braided: aspect 12.5
+        ny = max(4, int(round(math.sqrt(n / 25.0)))); nx = max(8, int(round(n / (2 * ny))))
+        pool[('braided', k)] = ('braided', nx, ny)
+    return sizes, pool
+
+
+def make_requests(n=256, rate=4.0, steps=2000, seed=4, classes=8, lo=20_000, hi=400_000):
+    """Poisson arrivals (seeded exponential gaps), log-uniform mesh sizes,
+    channel or braided geometry, Q ~ U[200, 2500] m^3/s, 0-16 rotors."""
+    rng = np.random.default_rng(seed)
+    t = np.cumsum(rng.exponential(1.0 / rate, n))
+    t -= t[0]
+    cells = np.exp(rng.uniform(np.log(lo), np.log(hi), n))
+    geo = np.where(rng.random(n) < 0.5, 'channel', 'braided')
+    q = rng.uniform(200.0, 2500.0, n)
+    nt = rng.integers(0, 17, n)
+    return [Request(i, float(t[i]), str(geo[i]), int(cells[i]), float(q[i]), int(nt[i]), steps) for i in range(n)]
+
+
+class MeshCache:
+    """Builds each pooled geometry once; a request gets a shallow copy with
+    its own discharge and rotor rows."""
+
+    def __init__(self, classes=8, lo=20_000, hi=400_000):
+        self.sizes, self.spec = pooled_meshes(classes, lo, hi)
+        self.cache = {}
+
+    def base(self, geometry, k):
+        key = (geometry, k)
+        if key not in self.cache:
+            g, nx, ny = self.spec[key]
+            if g == 'channel':
+                m = _channel(nx, ny, 1000.0)
+                rows = []
+                for j in range(16):
+                    x = 300.0 + 90.0 * j + 7.3; y = 30.0 + 40.0 * (j % 2) + 3.1
+                    r = max(5.0, 1.5 * max(m.dx, m.dy))
+                    near = (np.abs(m.xy[0] - x) < r) & (np.abs(m.xy[1] - y) < r)
+                    rows.append(dict(name=f'T{j + 1:02d}', X=x, Y=y, D=5.0, hub_elevation_m=float(m.z[near].max()) + 3.5,
+                                     minimum_bed_clearance_m=0.5, minimum_surface_clearance_m=0.5,
+                                     control=M.THETIS_TABLE))
+            else:
+                m, rows = M.braided_turbine_reach(nturbines=16, nx=nx, ny=ny, spacing=150.0)
+            self.cache[key] = (m.renumbered('hilbert'), rows)
+        return self.cache[key]
+
+    def realise(self, req):
+        k = int(np.argmin(np.abs(np.log(self.sizes) - np.log(req.cells))))
+        m, rows = self.base(req.geometry, k)
+        mm = M.Mesh(**{f.name: getattr(m, f.name) for f in __import__('dataclasses').fields(m)})
+        mm.forcing_q = m.forcing_q.copy()
+        mm.forcing_q[0] = req.q
+        req.mesh = mm
+        # rotors need cells narrower than the rotor across the footprint
+        req.rows = rows[:req.turbines] if max(m.dx, m.dy) < 0.9 * 5.0 else []
+        return req
+
+
+def scenario_bytes(m, nrot=0, precision=32):
+    """HBM footprint of one scenario's solver (state slots, geometry, work)."""
+    r = 4 if precision == 32 else 8
+    cell = 5 * 24 + 8 + 8 * r + 4 * r + 16 + 1 + 8 + 8 + 8
+    face = 4 * 3 + 6 * r + 3 * r + 4 * r + 3 * r + 16
+    inc = 4 * 3 + 4 * r + 8
+    return m.nc * cell + m.nf * face + m.nadj * inc + 64 * 1024 + nrot * 4096
+
+
+def build_solver(req, precision=32, stream=None):
+    from .river import RiverExplicit
+    from .turbines import ThetisArray
+    s = HydraulicSolver(req.mesh, precision=precision, scheme=5, stream=stream)
+    s.set_state(req.mesh.q0, prepare=False)
+    RiverExplicit(s, cfl=.5)
+    if req.rows:
+        ThetisArray(req.mesh, req.rows).install(s)
+    return s
+
+
+def serve(requests, cache, precision=32, budget_bytes=64 << 30, max_slots=64, chunk=8, log=None):
+    """Continuous batching: returns per-request dicts (arrival, admitted,
+    done, latency) and the makespan."""
+    import torch
+    stream = torch.cuda.Stream()
+    batch = Batch(precision, stream)
+    pending = sorted(requests, key=lambda r: r.arrival)
+    active = {}                 # slot id -> (req, solver, bytes)
+    used = 0
+    out = {}
+    t0 = time.perf_counter()
+    i = 0
+    while i < len(pending) or active:
+        now = time.perf_counter() - t0
+        admitted = False
+        while i < len(pending) and pending[i].arrival <= now and len(active) < max_slots:
+            req = cache.realise(pending[i])
+            need = scenario_bytes(req.mesh, len(req.rows), precision)
+            if used + need > budget_bytes and active:
+                break
+            s = build_solver(req, precision, stream)
+            sid = batch.add(s, req.steps, 0.0, 0.01)
+            active[sid] = (req, s, need)
+            used += need
+            out[req.rid] = dict(arrival=req.arrival, admitted=time.perf_counter() - t0, cells=req.mesh.nc,
+                                rotors=len(req.rows), geometry=req.geometry)
+            i += 1; admitted = True
+        if not active:
+            # idle until the next arrival
+            time.sleep(max(0.0, pending[i].arrival - (time.perf_counter() - t0)))
+            continue
+        batch.run(chunk)
+        for st in batch.poll():
+            if st.done:
+                req, s, need = active.pop(st.id)
+                fin = batch.remove(st.id)
+                now = time.perf_counter() - t0
+                out[req.rid].update(done=now, latency=now - req.arrival, accepted=fin.accepted,
+                                    rejections=fin.rejections, reason=fin.reason)
+                used -= need
+                s.close()
+                req.mesh = None
+                if log:
+                    log(f'[serve] request {req.rid} done: {out[req.rid]}')
+    torch.cuda.synchronize()
+    makespan = time.perf_counter() - t0
+    batch.close()
+    return out, makespan
+
+
+def plain(requests, cache, precision=32, log=None):
+    """The same requests solved one at a time in arrival order with the
+    single-scenario path; latency includes queueing behind earlier requests."""
+    import torch
+    stream = torch.cuda.Stream()
+    out = {}
+    t0 = time.perf_counter()
+    for req in sorted(requests, key=lambda r: r.arrival):
+        now = time.perf_counter() - t0
+        if req.arrival > now:
+            time.sleep(req.arrival - now)
+        cache.realise(req)
+        start = time.perf_counter() - t0
+        s = build_solver(req, precision, stream)
+        acc, rej = s.river_run(req.steps, hint=0.01)
+        torch.cuda.synchronize()
+        now = time.perf_counter() - t0
+        out[req.rid] = dict(arrival=req.arrival, admitted=start, done=now, latency=now - req.arrival,
+                            accepted=acc, rejections=rej, cells=req.mesh.nc, rotors=len(req.rows))
+        s.close()
+        req.mesh = None
+        if log:
+            log(f'[plain] request {req.rid} done: {out[req.rid]}')
+    makespan = time.perf_counter() - t0
+    return out, makespan
+
+
+def summarize(out, makespan, steps):
+    lat = np.array([v['latency'] for v in out.values()])
+    cu = sum(v['cells'] * v['accepted'] for v in out.values())
+    return dict(scenarios=len(out), makespan_s=makespan, scenarios_per_hour=len(out) / makespan * 3600.0,
+                p50_latency_s=float(np.percentile(lat, 50)), p99_latency_s=float(np.percentile(lat, 99)),
+                cell_updates_per_s=cu / makespan,
+                all_completed=all(v.get('accepted', 0) == steps for v in out.values()))
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument('--requests', type=int, default=256)
+    ap.add_argument('--rate', type=float, default=4.0)
+    ap.add_argument('--steps', type=int, default=2000)
+    ap.add_argument('--precision', type=int, default=32)
+    ap.add_argument('--classes', type=int, default=8)
+    ap.add_argument('--saturated', action='store_true', help='all requests arrive at t=0 (capacity)')
+    ap.add_argument('--skip-plain', action='store_true')
+    ap.add_argument('--chunk', type=int, default=8)
+    args = ap.parse_args()
+    import torch
+    torch.cuda.init()
+    log = lambda s: print(s, file=sys.stderr, flush=True)
+    cache = MeshCache(args.classes)
+    reqs = make_requests(args.requests, args.rate, args.steps, classes=args.classes)
+    if args.saturated:
+        for r in reqs:
+            r.arrival = 0.0
+    for g in ('channel', 'braided'):
+        for k in range(args.classes):
+            cache.base(g, k)
+    log(f'[serve] {len(reqs)} requests, mesh pool built')
+    res, mk = serve(reqs, cache, args.precision, chunk=args.chunk)
+    line = dict(batched=summarize(res, mk, args.steps))
+    log(f'[serve] batched: {line["batched"]}')
+    if not args.skip_plain:
+        reqs = make_requests(args.requests, args.rate, args.steps, classes=args.classes)
+        if args.saturated:
+            for r in reqs:
+                r.arrival = 0.0
+        res2, mk2 = plain(reqs, cache, args.precision)
+        line['plain'] = summarize(res2, mk2, args.steps)
+        line['speedup_scenarios_per_hour'] = line['batched']['scenarios_per_hour'] / line['plain']['scenarios_per_hour']
+    line['config'] = dict(requests=args.requests, rate_per_s=args.rate, steps=args.steps, precision=args.precision,
+                          mesh_classes=args.classes, saturated=args.saturated)
+    print(json.dumps(line), flush=True)
+
+
+if __name__ == '__main__':
+    main()

@@ -1,0 +1,200 @@
+// Batched river substeps: every kernel kind of a substep is ONE launch over
+// all scenarios of a batch.  Workgroups map to scenarios ("slots") through
+// per-kind block prefix tables; each slot's arrays are reached through a
+// device-resident slot table, and each slot carries its own step controller
+// (hint, rejections, accepted count) that k_batch_finalize advances on the
+// device.  A substep therefore needs no host synchronisation and the whole
+// sequence can be captured once as a graph and replayed.  The per-cell
+// arithmetic is the same device bodies as the single-scenario launches, so a
+// scenario's result does not depend on which other scenarios share the batch.
+#pragma once
+#include "river_kernels.h"
+
+namespace braids {
+
+__device__ __forceinline__ int find_slot(const int *prefix, int n, int b)
+{
+    int lo = 0, hi = n;         // prefix[lo] <= b < prefix[hi]
+    while (hi - lo > 1) {
+        const int mid = (lo + hi) >> 1;
+        if (prefix[mid] <= b) lo = mid; else hi = mid;
+    }
+    return lo;
+}
+
+static __global__ void __launch_bounds__(256) kb_zero(BatchPlan P, const int *nzero, unsigned int *const *blocks,
+                                               const SlotRun *run)
+{
+    const int s = find_slot(P.zero, P.nslot, blockIdx.x);
+    if (!run[s].active) return;
+    const int i = (blockIdx.x - P.zero[s]) * blockDim.x + threadIdx.x;
+    if (i < nzero[s]) blocks[s][i] = 0u;
+}
+
+template <class R>
+__global__ void __launch_bounds__(256) kb_drag(BatchPlan P, const SlotView<R> *V, SlotRun *run)
+{
+    const int s = find_slot(P.cell, P.nslot, blockIdx.x);
+    if (!run[s].active) return;
+    const SlotView<R> &v = V[s];
+    const int lb = blockIdx.x - P.cell[s];
+    if (lb >= v.ncb) return;
+    k_river_drag_body<R>(v.m.nc, v.q[run[s].parity], v.work, v.rb.drag, .5 * run[s].hint, (double)v.m.grav,
+                         (double)v.m.dry, 1, xcd_order(lb, v.ncb));
+}
+
+template <class R, int SCHEME>
+__global__ void __launch_bounds__(256) kb_prep(BatchPlan P, const SlotView<R> *V, const SlotRun *run, int stage)
+{
+    const int s = find_slot(P.cell, P.nslot, blockIdx.x);
+    if (!run[s].active) return;
+    const SlotView<R> &v = V[s];
+    const int lb = blockIdx.x - P.cell[s];
+    if (lb >= v.ncb) return;
+    const StateView<R> in = stage == 0 ? v.work : v.q1;
+    if constexpr (std::is_same<R, double>::value && SCHEME == 5)
+        k_prep_ls64_body(v.m, in, v.z, v.w, v.ctl, stage, 1, 0, xcd_order(lb, v.ncb));
+    else
+        k_prep_body<R, SCHEME>(v.m, in, v.z, v.w, v.ctl, stage, 1, 0, xcd_order(lb, v.ncb));
+}
+
+template <class R, int SCHEME>
+__global__ void __launch_bounds__(TRACE_BLOCK) kb_traces(BatchPlan P, const SlotView<R> *V, const SlotRun *run,
+                                                         int stage)
+{
+    const int s = find_slot(P.bnd, P.nslot, blockIdx.x);
+    if (!run[s].active) return;
+    const SlotView<R> &v = V[s];
+    const StateView<R> in = stage == 0 ? v.work : v.q1;
+    const double t = run[s].time + (stage ? run[s].hint : 0.0);
+    k_traces_body<R, SCHEME>(v.m, in, v.z, v.w, v.ctl, stage, t, blockIdx.x - P.bnd[s]);
+}
+
+template <class R, int SCHEME>
+__global__ void __launch_bounds__(256) kb_face(BatchPlan P, const SlotView<R> *V, const SlotRun *run, int stage)
+{
+    const int s = find_slot(P.face, P.nslot, blockIdx.x);
+    if (!run[s].active) return;
+    const SlotView<R> &v = V[s];
+    const int nfb = (v.m.nf + 255) >> 8, lb = blockIdx.x - P.face[s];
+    if (lb >= nfb) return;
+    const StateView<R> in = stage == 0 ? v.work : v.q1;
+    if constexpr (std::is_same<R, float>::value)
+        k_face_f32_body<SCHEME>(v.m, in, v.z, v.w, 1, xcd_order(lb, nfb));
+    else
+        k_face_body<R, SCHEME>(v.m, in, v.z, v.w, 1, xcd_order(lb, nfb));
+}
+
+template <class R>
+__global__ void __launch_bounds__(256) kb_theta(BatchPlan P, const SlotView<R> *V, const SlotRun *run, int stage)
+{
+    const int s = find_slot(P.theta, P.nslot, blockIdx.x);
+    if (!run[s].active) return;
+    const SlotView<R> &v = V[s];
+    const int lb = blockIdx.x - P.theta[s];
+    const StateView<R> in = stage == 0 ? v.work : v.q1;
+    if (lb >= v.ncb) {
+        const int a = (lb - v.ncb) * (blockDim.x >> 6) + (threadIdx.x >> 6);
+        if (a < v.T.nrot) turbine_sample_wave<R>(v.T, v.m, in, v.z, stage, a, threadIdx.x & 63);
+        return;
+    }
+    k_river_theta_body<R>(v.m, in, v.w, v.rb.theta, stage, run[s].hint, xcd_order(lb, v.ncb));
+}
+
+template <class R, int STAGE>
+__global__ void __launch_bounds__(256) kb_update(BatchPlan P, const SlotView<R> *V, const SlotRun *run)
+{
+    const int s = find_slot(P.cell, P.nslot, blockIdx.x);
+    if (!run[s].active) return;
+    const SlotView<R> &v = V[s];
+    const int lb = blockIdx.x - P.cell[s];
+    if (lb >= v.ncb) return;
+    const double dt = run[s].hint;
+    if (STAGE == 0)
+        k_river_update_body<R, 0>(v.m, v.work, v.work, v.q1, v.w, v.rb.theta, v.T, v.rb.wlim[0], v.rb.drag, v.ctl,
+                                  0, dt, 0.0, 1, xcd_order(lb, v.ncb));
+    else
+        k_river_update_body<R, 1>(v.m, v.q1, v.work, v.q[1 - run[s].parity], v.w, v.rb.theta, v.T, v.rb.wlim[1],
+                                  v.rb.drag, v.ctl, 1, dt, .5 * dt, 1, xcd_order(lb, v.ncb));
+}
+
+template <class R>
+__global__ void __launch_bounds__(TRACE_BLOCK) kb_ledger(BatchPlan P, const SlotView<R> *V, const SlotRun *run)
+{
+    const int s = find_slot(P.bnd, P.nslot, blockIdx.x);
+    if (!run[s].active) return;
+    const SlotView<R> &v = V[s];
+    k_river_ledger_body<R>(v.m, v.rb.wlim[0], v.rb.wlim[1], v.rb.ledger, run[s].hint, blockIdx.x - P.bnd[s]);
+}
+
+// One workgroup per slot: reduce the stage maxima, then take the substep's
+// accept/reject decision in the order of RiverExplicit.rate (spatial, boundary
+// status, CFL, non-finite/negative) and advance the slot's controller.
+template <class R>
+__global__ void __launch_bounds__(64) kb_finalize(const SlotView<R> *V, SlotRun *run, int nslot)
+{
+    const int s = blockIdx.x;
+    if (s >= nslot || !run[s].active) return;
+    const SlotView<R> &v = V[s];
+    __shared__ double peaks[2];
+    if (threadIdx.x < 2) {
+        const unsigned long long *p = v.w.rpart + rpart_index(RQ_PEAK, threadIdx.x);
+        unsigned long long mx = 0;
+        for (int i = 0; i < RSLOTS; ++i) { const unsigned long long x = p[i * RSTRIDE]; mx = x > mx ? x : mx; }
+        peaks[threadIdx.x] = __longlong_as_double((long long)mx);
+    }
+    __syncthreads();
+    const Control *c = v.ctl;
+    SlotRun &r = run[s];
+    const double dt = r.hint;
+    int reason = 0, stage = -1;
+    double peak = 0.;
+    for (int k = 0; k < 2 && !reason; ++k) {
+        const int code = c->failure[k];
+        if (code >= FAIL_TRACE) reason = 1;
+        else if (code == RFAIL_BOUNDARY) reason = 3;
+        else {
+            peak = fmax(peak, peaks[k]);
+            if ((c->turbine_error >> k) & 1) reason = 6;
+            else if (!(dt * peaks[k] <= v.cfl * (1 + 1e-12))) reason = 2;
+            else if (code == RFAIL_NONFINITE) reason = 4;
+            else if (code == RFAIL_NEGATIVE) reason = 5;
+        }
+        if (reason) stage = k;
+    }
+    const double safety = .98;
+    if (reason == 6) {                       // an error, not a retry
+        if (threadIdx.x == 0) { r.reason = 6; r.fail_stage = stage; r.done = 1; r.active = 0; }
+        return;
+    }
+    if (reason) {
+        if (threadIdx.x == 0) {
+            r.rejections += 1;
+            r.hint = peak > 0 ? fmin(.5 * dt, safety * v.cfl / peak) : .5 * dt;
+            r.reason = reason; r.fail_stage = stage;
+            if (r.rejections > 100 * (r.accepted + 1)) { r.done = 1; r.active = 0; }
+        }
+        return;
+    }
+    // accepted: boundary ledger and turbine energy (compensated), then the hint
+    for (int b = threadIdx.x; b < v.m.nb; b += blockDim.x) v.ledger_acc[b] = v.ledger_acc[b] + v.rb.ledger[b];
+    for (int a = threadIdx.x; a < v.T.nrot; a += blockDim.x) {
+        const double p0 = v.T.diag[(size_t)a * TURBINE_DIAG + 8];
+        const double p1 = v.T.diag[((size_t)v.T.nrot + a) * TURBINE_DIAG + 8];
+        const double add = dt * (.5 * (p0 + p1)), e = v.energy[2 * a], sum = e + add;
+        v.energy[2 * a + 1] += fabs(e) >= fabs(add) ? (e - sum) + add : (add - sum) + e;
+        v.energy[2 * a] = sum;
+    }
+    if (threadIdx.x == 0) {
+        r.parity ^= 1;
+        r.accepted += 1;
+        r.time = r.time + dt;
+        r.cflmax = fmax(r.cflmax, dt * peak);
+        r.peak = peak;
+        r.hint = peak > 0 ? fmin(2 * dt, safety * v.cfl / peak) : 2 * dt;
+        r.reason = 0;
+        if (r.accepted >= r.target) { r.done = 1; r.active = 0; }
+    }
+}
+
+}  // namespace braids

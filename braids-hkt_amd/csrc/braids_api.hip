@@ -579,10 +579,210 @@ struct Solver : SolverBase {
     }
 };
 
+
+// ---------------------------------------------------------------------------
+// Batched serving: many scenarios advanced together on one stream.
+struct BatchBase {
+    virtual ~BatchBase() {}
+    virtual int add(SolverBase *s, int target, double time, double hint, int *id) = 0;
+    virtual int remove(int id, braids_slot_status *st) = 0;
+    virtual int run(int substeps) = 0;
+    virtual int poll(braids_slot_status *st, int cap, int *n) = 0;
+    hipStream_t stream = nullptr;
+    bool own_stream = false;
+};
+
+template <class R>
+struct Batch : BatchBase {
+    struct Slot {
+        int id;
+        Solver<R> *s;
+        SlotRun run;
+        double *ledger_acc, *energy;
+    };
+    std::vector<Slot> slots;
+    int next_id = 0, scheme = -1;
+    bool dirty = true;
+    size_t cap = 0;
+    SlotView<R> *views = nullptr;
+    SlotRun *runs = nullptr, *runs_host = nullptr;
+    int *prefix = nullptr, *nzero = nullptr;
+    unsigned int **blocks = nullptr;
+    BatchPlan plan{};
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+
+    ~Batch() override
+    {
+        if (exec) (void)hipGraphExecDestroy(exec);
+        if (graph) (void)hipGraphDestroy(graph);
+        for (auto &sl : slots) { (void)hipFree(sl.ledger_acc); (void)hipFree(sl.energy); }
+        for (void *p : {(void *)views, (void *)runs, (void *)prefix, (void *)nzero, (void *)blocks})
+            if (p) (void)hipFree(p);
+        if (runs_host) (void)hipHostFree(runs_host);
+        if (own_stream && stream) (void)hipStreamDestroy(stream);
+    }
+
+    int sync_runs()
+    {
+        if (!runs || slots.empty() || dirty_layout) return BRAIDS_OK;
+        HIP_TRY(hipMemcpyAsync(runs_host, runs, slots.size() * sizeof(SlotRun), hipMemcpyDeviceToHost, stream));
+        HIP_TRY(hipStreamSynchronize(stream));
+        for (size_t i = 0; i < slots.size(); ++i) slots[i].run = runs_host[i];
+        return BRAIDS_OK;
+    }
+    bool dirty_layout = false;     // device run table does not match `slots` order (set by add/remove)
+
+    int rebuild()
+    {
+        if (exec) { (void)hipGraphExecDestroy(exec); exec = nullptr; }
+        if (graph) { (void)hipGraphDestroy(graph); graph = nullptr; }
+        const size_t n = slots.size();
+        if (n > cap) {
+            for (void *p : {(void *)views, (void *)runs, (void *)prefix, (void *)nzero, (void *)blocks})
+                if (p) (void)hipFree(p);
+            if (runs_host) (void)hipHostFree(runs_host);
+            cap = std::max<size_t>(n, 2 * cap);
+            HIP_TRY(hipMalloc(&views, cap * sizeof(SlotView<R>)));
+            HIP_TRY(hipMalloc(&runs, cap * sizeof(SlotRun)));
+            HIP_TRY(hipMalloc(&prefix, 5 * (cap + 1) * sizeof(int)));
+            HIP_TRY(hipMalloc(&nzero, cap * sizeof(int)));
+            HIP_TRY(hipMalloc(&blocks, cap * sizeof(unsigned int *)));
+            HIP_TRY(hipHostMalloc(&runs_host, cap * sizeof(SlotRun), hipHostMallocDefault));
+        }
+        std::vector<SlotView<R>> hv(n);
+        std::vector<SlotRun> hr(n);
+        std::vector<int> pre(5 * (n + 1), 0), nz(n);
+        std::vector<unsigned int *> bl(n);
+        auto pad8 = [](int x) { return (x + 7) & ~7; };
+        for (size_t i = 0; i < n; ++i) {
+            Solver<R> *x = slots[i].s;
+            SlotView<R> v{};
+            v.m = x->m; v.w = x->w;
+            v.q[0] = x->st[0]; v.q[1] = x->st[3]; v.work = x->st[1]; v.q1 = x->st[2];
+            v.z = x->z; v.ctl = x->ctl; v.rb = x->rb;
+            v.T = x->tv;
+            if (x->nrot > 0) v.T.error = &x->ctl->turbine_error; else v.T.nrot = 0;
+            v.cfl = x->river_cfl;
+            v.ncb = (x->nc + 255) / 256;
+            v.nzero = (int)(x->block_bytes / 4);
+            v.ledger_acc = slots[i].ledger_acc; v.energy = slots[i].energy;
+            hv[i] = v;
+            hr[i] = slots[i].run;
+            nz[i] = v.nzero;
+            bl[i] = (unsigned int *)x->ctl;
+            const int nrb = (v.T.nrot + 3) / 4;
+            pre[0 * (n + 1) + i + 1] = pre[0 * (n + 1) + i] + pad8(v.ncb);
+            pre[1 * (n + 1) + i + 1] = pre[1 * (n + 1) + i] + pad8(v.ncb + nrb);
+            pre[2 * (n + 1) + i + 1] = pre[2 * (n + 1) + i] + pad8((x->nf + 255) / 256);
+            pre[3 * (n + 1) + i + 1] = pre[3 * (n + 1) + i] + x->nb;
+            pre[4 * (n + 1) + i + 1] = pre[4 * (n + 1) + i] + (v.nzero + 255) / 256;
+        }
+        if (n) {
+            HIP_TRY(hipMemcpy(views, hv.data(), n * sizeof(SlotView<R>), hipMemcpyHostToDevice));
+            HIP_TRY(hipMemcpy(runs, hr.data(), n * sizeof(SlotRun), hipMemcpyHostToDevice));
+            HIP_TRY(hipMemcpy(prefix, pre.data(), pre.size() * sizeof(int), hipMemcpyHostToDevice));
+            HIP_TRY(hipMemcpy(nzero, nz.data(), n * sizeof(int), hipMemcpyHostToDevice));
+            HIP_TRY(hipMemcpy(blocks, bl.data(), n * sizeof(unsigned int *), hipMemcpyHostToDevice));
+        }
+        plan = BatchPlan{};
+        plan.nslot = (int)n; plan.scheme = scheme;
+        plan.cell = prefix; plan.theta = prefix + (n + 1); plan.face = prefix + 2 * (n + 1);
+        plan.bnd = prefix + 3 * (n + 1); plan.zero = prefix + 4 * (n + 1);
+        plan.ncell = pre[n]; plan.ntheta = pre[(n + 1) + n]; plan.nface = pre[2 * (n + 1) + n];
+        plan.nbnd = pre[3 * (n + 1) + n]; plan.nzero = pre[4 * (n + 1) + n];
+        dirty = false; dirty_layout = false;
+        if (n == 0) return BRAIDS_OK;
+        HIP_TRY(hipStreamBeginCapture(stream, hipStreamCaptureModeThreadLocal));
+        run_batch_substep<R>(plan, views, runs, nzero, blocks, stream);
+        HIP_TRY(hipStreamEndCapture(stream, &graph));
+        HIP_TRY(hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0));
+        return BRAIDS_OK;
+    }
+
+    int add(SolverBase *base, int target, double time, double hint, int *id) override
+    {
+        Solver<R> *x = dynamic_cast<Solver<R> *>(base);
+        if (!x) { set_error("solver precision differs from the batch"); return BRAIDS_EINVAL; }
+        for (auto &sl : slots) if (sl.s == x) { set_error("solver is already in the batch"); return BRAIDS_EINVAL; }
+        if (scheme < 0) scheme = x->m.scheme;
+        if (x->m.scheme != scheme) { set_error("all scenarios of a batch share one scheme"); return BRAIDS_EINVAL; }
+        int rc = sync_runs();
+        if (rc) return rc;
+        if (!x->rb.theta && (rc = x->river_buffers())) return rc;
+        Slot sl{};
+        sl.id = next_id++;
+        sl.s = x;
+        sl.run = SlotRun{};
+        sl.run.hint = hint; sl.run.time = time; sl.run.target = target; sl.run.active = target > 0 ? 1 : 0;
+        sl.run.done = target > 0 ? 0 : 1; sl.run.fail_stage = -1;
+        HIP_TRY(hipMalloc(&sl.ledger_acc, std::max(1, x->nb) * sizeof(double)));
+        HIP_TRY(hipMalloc(&sl.energy, std::max(1, 2 * x->nrot) * sizeof(double)));
+        HIP_TRY(hipMemset(sl.ledger_acc, 0, std::max(1, x->nb) * sizeof(double)));
+        HIP_TRY(hipMemset(sl.energy, 0, std::max(1, 2 * x->nrot) * sizeof(double)));
+        slots.push_back(sl);
+        dirty = true; dirty_layout = true;
+        *id = sl.id;
+        return BRAIDS_OK;
+    }
+
+    static void status_of(const Slot &sl, braids_slot_status *st)
+    {
+        st->id = sl.id; st->accepted = sl.run.accepted; st->target = sl.run.target;
+        st->rejections = sl.run.rejections; st->done = sl.run.done; st->reason = sl.run.reason;
+        st->time = sl.run.time; st->hint = sl.run.hint; st->cfl_max = sl.run.cflmax;
+    }
+
+    int remove(int id, braids_slot_status *st) override
+    {
+        int rc = sync_runs();
+        if (rc) return rc;
+        for (size_t i = 0; i < slots.size(); ++i) {
+            if (slots[i].id != id) continue;
+            Slot sl = slots[i];
+            if (st) status_of(sl, st);
+            Solver<R> *x = sl.s;
+            if (sl.run.parity) std::swap(x->st[0], x->st[3]);
+            std::vector<double> led(std::max(1, x->nb)), en(std::max(1, 2 * x->nrot));
+            HIP_TRY(hipMemcpy(led.data(), sl.ledger_acc, led.size() * sizeof(double), hipMemcpyDeviceToHost));
+            HIP_TRY(hipMemcpy(en.data(), sl.energy, en.size() * sizeof(double), hipMemcpyDeviceToHost));
+            for (int b = 0; b < x->nb; ++b) x->ledger[b] = x->ledger[b] + led[b];
+            for (int a = 0; a < x->nrot; ++a) { x->energy[a] += en[2 * a]; x->energy_comp[a] += en[2 * a + 1]; }
+            (void)hipFree(sl.ledger_acc); (void)hipFree(sl.energy);
+            slots.erase(slots.begin() + i);
+            dirty = true; dirty_layout = true;
+            return BRAIDS_OK;
+        }
+        set_error("no such batch slot"); return BRAIDS_EINVAL;
+    }
+
+    int run(int substeps) override
+    {
+        if (dirty) { int rc = rebuild(); if (rc) return rc; }
+        if (slots.empty()) return BRAIDS_OK;
+        for (int k = 0; k < substeps; ++k) HIP_TRY(hipGraphLaunch(exec, stream));
+        return BRAIDS_OK;
+    }
+
+    int poll(braids_slot_status *st, int capn, int *n) override
+    {
+        int rc = sync_runs();
+        if (rc) return rc;
+        *n = (int)slots.size();
+        for (int i = 0; i < (int)slots.size() && i < capn; ++i) status_of(slots[i], &st[i]);
+        return BRAIDS_OK;
+    }
+};
+
 }  // namespace
 
 struct braids_solver {
     SolverBase *impl;
+};
+
+struct braids_batch {
+    BatchBase *impl;
+    int precision;
 };
 
 static int check_mesh(const braids_mesh *m)
@@ -901,6 +1101,53 @@ extern "C" int braids_turbine_footprint(int npairs, const double *tri, const dou
     cleanup();
     if (e != hipSuccess) { set_error(std::string("footprint: ") + hipGetErrorString(e)); return BRAIDS_EHIP; }
     return BRAIDS_OK;
+}
+
+// -- batched serving -------------------------------------------------------------
+extern "C" int braids_batch_create(int precision, void *stream, braids_batch **out)
+{
+    if (!out || (precision != 32 && precision != 64)) { set_error("invalid batch arguments"); return BRAIDS_EINVAL; }
+    BatchBase *b = precision == 64 ? (BatchBase *)new Batch<double>() : (BatchBase *)new Batch<float>();
+    if (stream) b->stream = (hipStream_t)stream;
+    else {
+        if (hipStreamCreate(&b->stream) != hipSuccess) { delete b; set_error("hipStreamCreate failed"); return BRAIDS_EHIP; }
+        b->own_stream = true;
+    }
+    *out = new braids_batch{b, precision};
+    return BRAIDS_OK;
+}
+
+extern "C" int braids_batch_destroy(braids_batch *b)
+{
+    if (!b) return BRAIDS_OK;
+    delete b->impl;
+    delete b;
+    return BRAIDS_OK;
+}
+
+extern "C" int braids_batch_add(braids_batch *b, braids_solver *s, int target_steps, double time, double step_hint,
+                                int *id)
+{
+    if (!b || !s || !id || target_steps < 0 || !(step_hint > 0)) { set_error("invalid batch add arguments"); return BRAIDS_EINVAL; }
+    return b->impl->add(s->impl, target_steps, time, step_hint, id);
+}
+
+extern "C" int braids_batch_remove(braids_batch *b, int id, braids_slot_status *status)
+{
+    if (!b) { set_error("null batch"); return BRAIDS_EINVAL; }
+    return b->impl->remove(id, status);
+}
+
+extern "C" int braids_batch_run(braids_batch *b, int substeps)
+{
+    if (!b || substeps < 0) { set_error("invalid batch run arguments"); return BRAIDS_EINVAL; }
+    return b->impl->run(substeps);
+}
+
+extern "C" int braids_batch_poll(braids_batch *b, braids_slot_status *status, int capacity, int *count)
+{
+    if (!b || !count || (capacity > 0 && !status)) { set_error("invalid batch poll arguments"); return BRAIDS_EINVAL; }
+    return b->impl->poll(status, capacity, count);
 }
 
 extern "C" int braids_profile(braids_solver *s, int enable)

@@ -49,15 +49,22 @@ __device__ __forceinline__ void cell_drag(double h, double &hu, double &hv, doub
 }
 
 template <class R>
-__global__ void __launch_bounds__(256) k_river_drag(int nc, StateView<R> in, StateView<R> out, DragView d,
-                                                    double dt, double grav, double dry, int remap)
+__device__ __forceinline__ void k_river_drag_body(int nc, StateView<R> in, StateView<R> out, DragView d,
+                                                    double dt, double grav, double dry, int remap, int lb)
 {
-    const int c = logical_block(remap) * blockDim.x + threadIdx.x;
+    const int c = lb * blockDim.x + threadIdx.x;
     if (c >= nc) return;
     const double h = in.h[c];
     double hu = in.hu[c], hv = in.hv[c];
     cell_drag(h, hu, hv, d.mode > 0 ? d.values[c] : 0., d.mode, dt, grav, dry);
     out.h[c] = h; out.hu[c] = hu; out.hv[c] = hv;
+}
+
+template <class R>
+__global__ void __launch_bounds__(256) k_river_drag(int nc, StateView<R> in, StateView<R> out, DragView d,
+                                                    double dt, double grav, double dry, int remap)
+{
+    k_river_drag_body<R>(nc, in, out, d, dt, grav, dry, remap, logical_block(remap));
 }
 
 // ---------------------------------------------------------------------------
@@ -217,23 +224,11 @@ __device__ __forceinline__ void turbine_sink(const TurbineView &T, int u, double
 // midpoint_inventory (river_midpoint.h:13-21) with the stage input as the
 // available inventory, and explicit_frequency (river_explicit.h:13-25, no
 // exterior owner) max-reduced into the stage's CFL slot.
-// Workgroups past the ncb cell blocks sample the rotors, one wavefront each.
 template <class R>
-__global__ void __launch_bounds__(256) k_river_theta(MeshView<R> m, StateView<R> q, Work<R> w, double *theta,
-                                                     Control *ctl, int slot, double dt, int remap,
-                                                     TurbineView T, const double *__restrict__ z, int ncb)
+__device__ __forceinline__ void k_river_theta_body(MeshView<R> m, StateView<R> q, Work<R> w, double *theta, int slot,
+                                                   double dt, int lb)
 {
     const int nf = m.nf;
-    if ((int)blockIdx.x >= ncb) {
-        const int a = ((int)blockIdx.x - ncb) * (blockDim.x >> 6) + (threadIdx.x >> 6);
-        if (a < T.nrot) turbine_sample_wave<R>(T, m, q, z, slot, a, threadIdx.x & 63);
-        return;
-    }
-    int lb = blockIdx.x;
-    if (remap) {          // XCD-aware order over the cell blocks only
-        const int n = ncb, qq = n >> 3, r8 = n & 7, x = lb & 7, k = lb >> 3;
-        lb = x * qq + (x < r8 ? x : r8) + k;
-    }
     const int c = lb * blockDim.x + threadIdx.x;
     double freq = 0.;
     if (c < m.nc) {
@@ -252,20 +247,42 @@ __global__ void __launch_bounds__(256) k_river_theta(MeshView<R> m, StateView<R>
         freq = sum / A;
         if (!(freq == freq)) freq = 1e300;     // a NaN frequency rejects the trial
     }
-    (void)ctl;
     block_max_to(freq, w.rpart + rpart_index(RQ_PEAK, slot));
+}
+
+// XCD-aware order of block b among n blocks (the logical_block mapping for a
+// sub-range whose first block is a multiple of 8).
+__device__ __forceinline__ int xcd_order(int b, int n)
+{
+    const int qq = n >> 3, r8 = n & 7, x = b & 7, k = b >> 3;
+    return x * qq + (x < r8 ? x : r8) + k;
+}
+
+// Workgroups past the ncb cell blocks sample the rotors, one wavefront each.
+template <class R>
+__global__ void __launch_bounds__(256) k_river_theta(MeshView<R> m, StateView<R> q, Work<R> w, double *theta,
+                                                     Control *ctl, int slot, double dt, int remap,
+                                                     TurbineView T, const double *__restrict__ z, int ncb)
+{
+    (void)ctl;
+    if ((int)blockIdx.x >= ncb) {
+        const int a = ((int)blockIdx.x - ncb) * (blockDim.x >> 6) + (threadIdx.x >> 6);
+        if (a < T.nrot) turbine_sample_wave<R>(T, m, q, z, slot, a, threadIdx.x & 63);
+        return;
+    }
+    k_river_theta_body<R>(m, q, w, theta, slot, dt, remap ? xcd_order(blockIdx.x, ncb) : (int)blockIdx.x);
 }
 
 // STAGE 0: out = in + dt r (in is the dragged base);  STAGE 1: out = in0/2 +
 // (in + dt r)/2 with in0 the dragged base, then the closing drag half-step.
 template <class R, int STAGE>
-__global__ void __launch_bounds__(256) k_river_update(MeshView<R> m, StateView<R> in, StateView<R> in0,
+__device__ __forceinline__ void k_river_update_body(MeshView<R> m, StateView<R> in, StateView<R> in0,
                                                       StateView<R> out, Work<R> w, const double *__restrict__ theta,
                                                       TurbineView T, double *wlim, DragView d, Control *ctl,
-                                                      int slot, double dt, double drag_dt, int remap)
+                                                      int slot, double dt, double drag_dt, int remap, int lb)
 {
     const int nc = m.nc, nf = m.nf;
-    const int c = logical_block(remap) * blockDim.x + threadIdx.x;
+    const int c = lb * blockDim.x + threadIdx.x;
     int fail = 0;
     if (c < nc) {
         const double h = in.h[c], dry = (double)m.dry;
@@ -337,13 +354,22 @@ __global__ void __launch_bounds__(256) k_river_update(MeshView<R> m, StateView<R
     if (fail) { atomicMax(&ctl->failure[slot], fail); atomicMax(&ctl->fail_cell[slot], c + 1); }
 }
 
+template <class R, int STAGE>
+__global__ void __launch_bounds__(256) k_river_update(MeshView<R> m, StateView<R> in, StateView<R> in0,
+                                                      StateView<R> out, Work<R> w, const double *__restrict__ theta,
+                                                      TurbineView T, double *wlim, DragView d, Control *ctl,
+                                                      int slot, double dt, double drag_dt, int remap)
+{
+    k_river_update_body<R, STAGE>(m, in, in0, out, w, theta, T, wlim, d, ctl, slot, dt, drag_dt, remap, logical_block(remap));
+}
+
 // Boundary ledger of one substep: ledger[b] = -dt sum_f L (w1 + w2)/2 over
 // the faces of boundary b (inward water; the substep's mean limited flux).
 template <class R>
-__global__ void __launch_bounds__(TRACE_BLOCK) k_river_ledger(MeshView<R> m, const double *w1, const double *w2,
-                                                              double *ledger, double dt)
+__device__ __forceinline__ void k_river_ledger_body(MeshView<R> m, const double *w1, const double *w2,
+                                                              double *ledger, double dt, int lb)
 {
-    const int b = blockIdx.x;
+    const int b = lb;
     if (b >= m.nb) return;
     __shared__ double sh[TRACE_BLOCK];
     double s = 0;
@@ -353,6 +379,13 @@ __global__ void __launch_bounds__(TRACE_BLOCK) k_river_ledger(MeshView<R> m, con
     }
     s = block_sum<TRACE_BLOCK>(s, sh);
     if (threadIdx.x == 0) ledger[b] = s;
+}
+
+template <class R>
+__global__ void __launch_bounds__(TRACE_BLOCK) k_river_ledger(MeshView<R> m, const double *w1, const double *w2,
+                                                              double *ledger, double dt)
+{
+    k_river_ledger_body<R>(m, w1, w2, ledger, dt, blockIdx.x);
 }
 
 // Footprint weights (river_turbine_geometry.py:29-71): one workgroup per

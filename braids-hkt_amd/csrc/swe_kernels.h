@@ -110,11 +110,11 @@ __device__ __forceinline__ int logical_block(int remap)
 
 // ---------------------------------------------------------------------------
 template <class R, int SCHEME>
-__global__ void __launch_bounds__(256) k_prep(MeshView<R> m, StateView<R> q, const double *__restrict__ z,
-                                              Work<R> w, Control *ctl, int stage, int remap, int want_hmax)
+__device__ __forceinline__ void k_prep_body(MeshView<R> m, StateView<R> q, const double *__restrict__ z,
+                                              Work<R> w, Control *ctl, int stage, int remap, int want_hmax, int lb)
 {
     const int nc = m.nc;
-    int c = logical_block(remap) * blockDim.x + threadIdx.x;
+    int c = lb * blockDim.x + threadIdx.x;
     double hmx = 0.0;
     if (c < nc) {
         const R dry = m.dry;
@@ -215,13 +215,20 @@ __global__ void __launch_bounds__(256) k_prep(MeshView<R> m, StateView<R> q, con
     if (want_hmax) block_max_to(hmx > 0.0 ? hmx : 0.0, w.rpart + rpart_index(RQ_HMAX, stage));
 }
 
+template <class R, int SCHEME>
+__global__ void __launch_bounds__(256) k_prep(MeshView<R> m, StateView<R> q, const double *__restrict__ z,
+                                              Work<R> w, Control *ctl, int stage, int remap, int want_hmax)
+{
+    k_prep_body<R, SCHEME>(m, q, z, w, ctl, stage, remap, want_hmax, logical_block(remap));
+}
+
 // FP64-exact form of the LS-MUSCL preparation (same operation order as the
 // reference: primitives, then adjacent - centre differences in FP64).
-static __global__ void __launch_bounds__(256) k_prep_ls64(MeshView<double> m, StateView<double> q, const double *__restrict__ z,
-                                                        Work<double> w, Control *ctl, int stage, int remap, int want_hmax)
+static __device__ __forceinline__ void k_prep_ls64_body(MeshView<double> m, StateView<double> q, const double *__restrict__ z,
+                                                        Work<double> w, Control *ctl, int stage, int remap, int want_hmax, int lb)
 {
     const int nc = m.nc;
-    int c = logical_block(remap) * blockDim.x + threadIdx.x;
+    int c = lb * blockDim.x + threadIdx.x;
     double hmx = 0.0;
     if (c < nc) {
         const double dry = m.dry;
@@ -305,6 +312,12 @@ static __global__ void __launch_bounds__(256) k_prep_ls64(MeshView<double> m, St
     if (want_hmax) block_max_to(hmx > 0.0 ? hmx : 0.0, w.rpart + rpart_index(RQ_HMAX, stage));
 }
 
+static __global__ void __launch_bounds__(256) k_prep_ls64(MeshView<double> m, StateView<double> q, const double *__restrict__ z,
+                                                        Work<double> w, Control *ctl, int stage, int remap, int want_hmax)
+{
+    k_prep_ls64_body(m, q, z, w, ctl, stage, remap, want_hmax, logical_block(remap));
+}
+
 // face_value (cutcell_solver.f90:8999-9051) for schemes 1 and 5.
 // Returns eta and z in FP64, velocities in R.
 template <class R, int SCHEME>
@@ -359,10 +372,10 @@ constexpr int TRACE_BLOCK = 256;
 // Boundary traces: one workgroup per open boundary.  The sums of the
 // total-discharge distribution are strided per lane, then tree-reduced.
 template <class R, int SCHEME>
-__global__ void __launch_bounds__(TRACE_BLOCK) k_traces(MeshView<R> m, StateView<R> q, const double *__restrict__ z,
-                                                        Work<R> w, Control *ctl, int stage, double t)
+__device__ __forceinline__ void k_traces_body(MeshView<R> m, StateView<R> q, const double *__restrict__ z,
+                                                        Work<R> w, Control *ctl, int stage, double t, int lb)
 {
-    const int b = blockIdx.x;
+    const int b = lb;
     if (b >= m.nb) return;
     const int k0 = m.bnd_offset[b], k1 = m.bnd_offset[b + 1];
     const double dry = (double)m.dry, g = (double)m.grav;
@@ -490,13 +503,20 @@ __global__ void __launch_bounds__(TRACE_BLOCK) k_traces(MeshView<R> m, StateView
     if (fail) atomicMax(&ctl->failure[stage], FAIL_TRACE + fail);
 }
 
+template <class R, int SCHEME>
+__global__ void __launch_bounds__(TRACE_BLOCK) k_traces(MeshView<R> m, StateView<R> q, const double *__restrict__ z,
+                                                        Work<R> w, Control *ctl, int stage, double t)
+{
+    k_traces_body<R, SCHEME>(m, q, z, w, ctl, stage, t, blockIdx.x);
+}
+
 // ---------------------------------------------------------------------------
 template <class R, int SCHEME>
-__global__ void __launch_bounds__(256) k_face(MeshView<R> m, StateView<R> q, const double *__restrict__ z,
-                                              Work<R> w, int remap)
+__device__ __forceinline__ void k_face_body(MeshView<R> m, StateView<R> q, const double *__restrict__ z,
+                                              Work<R> w, int remap, int lb)
 {
     const int nf = m.nf;
-    const int f = logical_block(remap) * blockDim.x + threadIdx.x;
+    const int f = lb * blockDim.x + threadIdx.x;
     if (f >= nf) return;
     const R g = m.grav, dry = m.dry;
     const int c = m.owner[f], n = m.neighbour[f], b = m.bid[f];
@@ -616,6 +636,13 @@ __global__ void __launch_bounds__(256) k_face(MeshView<R> m, StateView<R> q, con
     w.F[3 * nf + f] = fr[1]; w.F[4 * nf + f] = fr[2]; w.F[5 * nf + f] = speed;
 }
 
+template <class R, int SCHEME>
+__global__ void __launch_bounds__(256) k_face(MeshView<R> m, StateView<R> q, const double *__restrict__ z,
+                                              Work<R> w, int remap)
+{
+    k_face_body<R, SCHEME>(m, q, z, w, remap, logical_block(remap));
+}
+
 // FP32 face kernel.  Same face classification as k_face, but the stored
 // momentum fluxes are net of the adjacent cell-centre hydrostatic pressure
 // (G = F - 1/2 g h_c^2 n), and every pressure term is summed in FP64:
@@ -623,12 +650,12 @@ __global__ void __launch_bounds__(256) k_face(MeshView<R> m, StateView<R> q, con
 // which for the wet-column correction equals 1/2 g (h_c + h_f)(eta_f - eta_c),
 // a free-surface difference.  A lake at rest therefore stays at rest in FP32.
 template <int SCHEME>
-__global__ void __launch_bounds__(256) k_face_f32(MeshView<float> m, StateView<float> q, const double *__restrict__ z,
-                                                  Work<float> w, int remap)
+__device__ __forceinline__ void k_face_f32_body(MeshView<float> m, StateView<float> q, const double *__restrict__ z,
+                                                  Work<float> w, int remap, int lb)
 {
     typedef float R;
     const int nf = m.nf;
-    const int f = logical_block(remap) * blockDim.x + threadIdx.x;
+    const int f = lb * blockDim.x + threadIdx.x;
     if (f >= nf) return;
     const R g = m.grav, dry = m.dry;
     const double gg = (double)g, dd = (double)dry;
@@ -726,12 +753,19 @@ __global__ void __launch_bounds__(256) k_face_f32(MeshView<float> m, StateView<f
     w.F[5 * nf + f] = speed;
 }
 
+template <int SCHEME>
+__global__ void __launch_bounds__(256) k_face_f32(MeshView<float> m, StateView<float> q, const double *__restrict__ z,
+                                                  Work<float> w, int remap)
+{
+    k_face_f32_body<SCHEME>(m, q, z, w, remap, logical_block(remap));
+}
+
 // ---------------------------------------------------------------------------
 template <class R>
-__global__ void __launch_bounds__(256) k_drain(MeshView<R> m, StateView<R> q, Work<R> w, Control *ctl,
-                                               int stage, double dt, int remap)
+__device__ __forceinline__ void k_drain_body(MeshView<R> m, StateView<R> q, Work<R> w, Control *ctl,
+                                               int stage, double dt, int remap, int lb)
 {
-    const int c = logical_block(remap) * blockDim.x + threadIdx.x;
+    const int c = lb * blockDim.x + threadIdx.x;
     int drains = 0, fail = 0;
     if (c < m.nc) {
         R theta = R(1);
@@ -759,17 +793,24 @@ __global__ void __launch_bounds__(256) k_drain(MeshView<R> m, StateView<R> q, Wo
     if (fail) atomicMax(&ctl->failure[stage], FAIL_DRAIN);
 }
 
+template <class R>
+__global__ void __launch_bounds__(256) k_drain(MeshView<R> m, StateView<R> q, Work<R> w, Control *ctl,
+                                               int stage, double dt, int remap)
+{
+    k_drain_body<R>(m, q, w, ctl, stage, dt, remap, logical_block(remap));
+}
+
 // Gather + update.  STAGE 0/1/2 select the SSPRK(3,3) combination
 // (advance_ssp33, cutcell_solver.f90:15858-15986): a = E(q); b = 3/4 q + 1/4 E(a);
 // c = 1/3 q + 2/3 E(b).  Each output is prepared (dry momentum cleared, speed
 // capped); the last one also receives the semi-implicit Manning source.
 template <class R, int STAGE>
-__global__ void __launch_bounds__(256) k_gather(MeshView<R> m, StateView<R> in, StateView<R> base,
+__device__ __forceinline__ void k_gather_body(MeshView<R> m, StateView<R> in, StateView<R> base,
                                                 StateView<R> out, Work<R> w, Control *ctl,
-                                                double dt, int remap)
+                                                double dt, int remap, int lb)
 {
     const int nc = m.nc, nf = m.nf;
-    const int c = logical_block(remap) * blockDim.x + threadIdx.x;
+    const int c = lb * blockDim.x + threadIdx.x;
     double ratio = 0.0;
     int fail = 0;
     if (c < nc) {
@@ -876,14 +917,22 @@ __global__ void __launch_bounds__(256) k_gather(MeshView<R> m, StateView<R> in, 
     if (fail) { atomicMax(&ctl->failure[STAGE], fail); atomicMax(&ctl->fail_cell[STAGE], c + 1); }
 }
 
+template <class R, int STAGE>
+__global__ void __launch_bounds__(256) k_gather(MeshView<R> m, StateView<R> in, StateView<R> base,
+                                                StateView<R> out, Work<R> w, Control *ctl,
+                                                double dt, int remap)
+{
+    k_gather_body<R, STAGE>(m, in, base, out, w, ctl, dt, remap, logical_block(remap));
+}
+
 // Spatial rate without draining or CFL (bend_native_rates_tail.inc:4-20):
 // rate = -sum_k s_k (L/A)(F - 1/2 g h^2 n); plus the shared face water flux.
 template <class R>
-__global__ void __launch_bounds__(256) k_rates(MeshView<R> m, StateView<R> in, Work<R> w,
-                                               double *rate, double *water, int remap)
+__device__ __forceinline__ void k_rates_body(MeshView<R> m, StateView<R> in, Work<R> w,
+                                               double *rate, double *water, int remap, int lb)
 {
     const int nc = m.nc, nf = m.nf;
-    const int c = logical_block(remap) * blockDim.x + threadIdx.x;
+    const int c = lb * blockDim.x + threadIdx.x;
     if (c < nf) water[c] = (double)w.F[c];
     if (c < nc) {
         const double h = in.h[c];
@@ -906,12 +955,19 @@ __global__ void __launch_bounds__(256) k_rates(MeshView<R> m, StateView<R> in, W
     }
 }
 
+template <class R>
+__global__ void __launch_bounds__(256) k_rates(MeshView<R> m, StateView<R> in, Work<R> w,
+                                               double *rate, double *water, int remap)
+{
+    k_rates_body<R>(m, in, w, rate, water, remap, logical_block(remap));
+}
+
 // Boundary ledger of one stage: accepted(1,b) -= dt L fl(1,f) (:12058-12066),
 // one workgroup per boundary (strided partial sums, fixed-order tree).
 template <class R>
-__global__ void __launch_bounds__(TRACE_BLOCK) k_ledger(MeshView<R> m, Work<R> w, double *ledger_stage, double dt)
+__device__ __forceinline__ void k_ledger_body(MeshView<R> m, Work<R> w, double *ledger_stage, double dt, int lb)
 {
-    const int b = blockIdx.x;
+    const int b = lb;
     if (b >= m.nb) return;
     __shared__ double sh[TRACE_BLOCK];
     double s = 0;
@@ -921,6 +977,12 @@ __global__ void __launch_bounds__(TRACE_BLOCK) k_ledger(MeshView<R> m, Work<R> w
     }
     s = block_sum<TRACE_BLOCK>(s, sh);
     if (threadIdx.x == 0) ledger_stage[b] = s;
+}
+
+template <class R>
+__global__ void __launch_bounds__(TRACE_BLOCK) k_ledger(MeshView<R> m, Work<R> w, double *ledger_stage, double dt)
+{
+    k_ledger_body<R>(m, w, ledger_stage, dt, blockIdx.x);
 }
 
 }  // namespace braids

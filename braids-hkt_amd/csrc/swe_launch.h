@@ -53,6 +53,45 @@ void run_river_substep(const MeshView<R> &m, StateView<R> base, StateView<R> wor
                        StateView<R> out, const double *z, Work<R> w, Control *ctl, const RiverBuffers &rb,
                        const TurbineView *tv, double dt, double t, const LaunchCfg &cfg, hipStream_t stream);
 
+// ---- batched river substeps over many scenarios (slots) ----
+template <class R> struct SlotView {
+    MeshView<R> m;
+    Work<R> w;
+    StateView<R> q[2];          // accepted state ping-pong (run.parity selects the current one)
+    StateView<R> work, q1;      // dragged base, stage-1 output
+    const double *z;
+    Control *ctl;
+    RiverBuffers rb;
+    TurbineView T;
+    double cfl;
+    int ncb;                    // cell blocks (256 threads)
+    int nzero;                  // 32-bit words of the control block to clear per substep
+    double *ledger_acc;         // [nb] accepted inward water
+    double *energy;             // [2*nrot] compensated energy sums (value, compensation)
+};
+
+// Per-slot controller state (device), RiverExplicit.advance's hint rules
+// (river_explicit.py:121-155) for a fixed number of accepted substeps.
+struct SlotRun {
+    double hint, time, trial, peak, cflmax;
+    int parity, accepted, target, rejections, active, done, reason, fail_stage;
+};
+
+struct BatchPlan {
+    int nslot, scheme;
+    int ncell, ntheta, nface, nbnd, nzero;     // grid sizes (block totals)
+    const int *cell;            // [nslot+1] prefix of cell blocks (each padded to a multiple of 8)
+    const int *theta;           // cell + rotor blocks
+    const int *face;            // face blocks
+    const int *bnd;             // boundaries (one block each)
+    const int *zero;            // control-clearing blocks
+};
+
+// One batched substep for every active slot (no host synchronisation).
+template <class R>
+void run_batch_substep(const BatchPlan &P, const SlotView<R> *V, SlotRun *run, const int *nzero,
+                       unsigned int *const *blocks, hipStream_t stream);
+
 // Footprint weights of (rotor, cell) pairs (device arrays).
 void run_footprint(int npairs, const double *tri, const double *D, const double *nodes, const double *weights,
                    double *value, double *clipped, int *order, hipStream_t stream);

@@ -4,6 +4,7 @@
 #include "swe_launch.h"
 #include "swe_kernels.h"
 #include "river_kernels.h"
+#include "batch_kernels.h"
 #include <hip/hip_ext.h>
 #include <type_traits>
 
@@ -151,6 +152,34 @@ void run_river_substep(const MeshView<R> &m, StateView<R> base, StateView<R> wor
         hipLaunchKernelGGL((k_river_ledger<R>), dim3(m.nb), dim3(TRACE_BLOCK), 0, s, m,
                            (const double *)rb.wlim[0], (const double *)rb.wlim[1], rb.ledger, dt);
     run_finalize<R>(w, ctl, s);
+}
+
+template <class R, int SCHEME>
+static void batch_stage(const BatchPlan &P, const SlotView<R> *V, SlotRun *run, int stage, hipStream_t s)
+{
+    hipLaunchKernelGGL((kb_prep<R, SCHEME>), dim3(P.ncell), dim3(256), 0, s, P, V, (const SlotRun *)run, stage);
+    if (P.nbnd > 0)
+        hipLaunchKernelGGL((kb_traces<R, SCHEME>), dim3(P.nbnd), dim3(TRACE_BLOCK), 0, s, P, V, (const SlotRun *)run, stage);
+    hipLaunchKernelGGL((kb_face<R, SCHEME>), dim3(P.nface), dim3(256), 0, s, P, V, (const SlotRun *)run, stage);
+    hipLaunchKernelGGL((kb_theta<R>), dim3(P.ntheta), dim3(256), 0, s, P, V, (const SlotRun *)run, stage);
+    if (stage == 0) hipLaunchKernelGGL((kb_update<R, 0>), dim3(P.ncell), dim3(256), 0, s, P, V, (const SlotRun *)run);
+    else hipLaunchKernelGGL((kb_update<R, 1>), dim3(P.ncell), dim3(256), 0, s, P, V, (const SlotRun *)run);
+}
+
+template <class R>
+void run_batch_substep(const BatchPlan &P, const SlotView<R> *V, SlotRun *run, const int *nzero,
+                       unsigned int *const *blocks, hipStream_t s)
+{
+    if (P.nslot <= 0) return;
+    hipLaunchKernelGGL(kb_zero, dim3(P.nzero), dim3(256), 0, s, P, nzero, blocks, (const SlotRun *)run);
+    hipLaunchKernelGGL((kb_drag<R>), dim3(P.ncell), dim3(256), 0, s, P, V, run);
+    for (int stage = 0; stage < 2; ++stage) {
+        if (P.scheme == 5) batch_stage<R, 5>(P, V, run, stage, s);
+        else batch_stage<R, 1>(P, V, run, stage, s);
+    }
+    if (P.nbnd > 0)
+        hipLaunchKernelGGL((kb_ledger<R>), dim3(P.nbnd), dim3(TRACE_BLOCK), 0, s, P, V, (const SlotRun *)run);
+    hipLaunchKernelGGL((kb_finalize<R>), dim3(P.nslot), dim3(64), 0, s, V, run, P.nslot);
 }
 
 }  // namespace braids

@@ -186,6 +186,34 @@ int braids_turbines_read(braids_solver *s, double *diag, double *energy);
 int braids_turbine_footprint(int npairs, const double *tri, const double *D, const double *nodes,
                              const double *weights, double *value, double *clipped, int *order);
 
+/* ---- Batched serving (no reference counterpart: the reference runs one
+ * scenario per owner, river_worker.py) ----
+ * A batch advances many solver handles (scenarios) together: every kernel of
+ * a river substep is one launch over all of them, each scenario keeps its own
+ * step controller on the device (the RiverExplicit hint rules for a fixed
+ * number of accepted substeps), and a substep is replayed as a captured
+ * graph with no host synchronisation.  A solver in a batch must have been
+ * configured for the river path (braids_river_configure, optional turbines)
+ * and must not be stepped by other entries until it is removed. */
+typedef struct braids_batch braids_batch;
+typedef struct braids_slot_status {
+    int id, accepted, target, rejections, done, reason;
+    double time, hint, cfl_max;
+} braids_slot_status;
+/* stream: hipStream_t as void* (NULL: the batch creates its own). */
+int braids_batch_create(int precision, void *stream, braids_batch **out);
+int braids_batch_destroy(braids_batch *b);
+/* Admit a scenario for target_steps accepted substeps starting at time with
+ * the given first substep hint; *id identifies its slot. */
+int braids_batch_add(braids_batch *b, braids_solver *s, int target_steps, double time, double step_hint, int *id);
+/* Retire a slot: the solver's resident state, ledger and turbine energy are
+ * the scenario's results (status may be NULL). */
+int braids_batch_remove(braids_batch *b, int id, braids_slot_status *status);
+/* Enqueue `substeps` batched substeps (asynchronous). */
+int braids_batch_run(braids_batch *b, int substeps);
+/* Wait for the enqueued work and report every slot (in slot order). */
+int braids_batch_poll(braids_batch *b, braids_slot_status *status, int capacity, int *count);
+
 /* Mass inventory sum(A h) of the resident state (compensated FP64 sum). */
 int braids_mass(braids_solver *s, double *mass);
 
