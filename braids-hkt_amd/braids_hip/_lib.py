@@ -44,6 +44,15 @@ class BraidsRiverResult(ctypes.Structure):
                 ('frequency_peak', _D)]
 
 
+class BraidsExnerOptions(ctypes.Structure):
+    _fields_ = [('d50', _D), ('relative_density', _D), ('porosity', _D), ('critical_shields', _D),
+                ('morfac', _D), ('sediment_depth', _D), ('rock', _P)]
+
+
+class BraidsExnerResult(ctypes.Structure):
+    _fields_ = [('accepted', _I), ('residual', _D), ('boundary', _D), ('change', _D), ('donor_min', _D)]
+
+
 class BraidsTurbines(ctypes.Structure):
     _fields_ = [('nrot', _I),
                 ('D', _P), ('top', _P), ('bottom', _P), ('tol', _P), ('min_surface', _P), ('min_bed', _P),
@@ -77,6 +86,11 @@ SIGNATURES = {
     'braids_turbines_set': (_I, [_P, ctypes.POINTER(BraidsTurbines)]),
     'braids_turbines_read': (_I, [_P, _P, _P]),
     'braids_turbine_footprint': (_I, [_I, _P, _P, _P, _P, _P, _P, _P]),
+    'braids_exner_configure': (_I, [_P, ctypes.POINTER(BraidsExnerOptions)]),
+    'braids_exner_finish': (_I, [_P, _D, ctypes.POINTER(BraidsExnerResult)]),
+    'braids_snapshot': (_I, [_P]),
+    'braids_restore': (_I, [_P]),
+    'braids_get_bed': (_I, [_P, _P]),
     'braids_batch_create': (_I, [_I, _P, _P]),
     'braids_batch_destroy': (_I, [_P]),
     'braids_batch_add': (_I, [_P, _P, _I, _D, _D, _P]),

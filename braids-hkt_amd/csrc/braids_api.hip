@@ -96,15 +96,19 @@ struct SolverBase {
     virtual int river_configure(double cfl, int drag_mode, const double *drag_values) = 0;
     virtual int river_step(double dt, double t, braids_river_result *res) = 0;
     virtual int turbines_set(const braids_turbines *t) = 0;
+    virtual int exner_configure(const braids_exner_options *o) = 0;
+    virtual int exner_finish(double dt, braids_exner_result *r) = 0;
+    virtual int snapshot(int restore) = 0;
+    virtual int get_bed(double *z) = 0;
     hipStream_t stream = nullptr;
     // river path
     double river_cfl = 0.5;
-    std::vector<double> manning_host, xy_host;
+    std::vector<double> manning_host, xy_host, length_host, normal_host;
     int nrot = 0;
     std::vector<double> energy, energy_comp;   // per rotor, compensated (Neumaier)
     std::vector<double> diag_last;             // [2][nrot][TURBINE_DIAG] of the last accepted substep
     int nc = 0, nf = 0, nb = 0;
-    double cfl_target = 0.9, dtmax = 1.0;
+    double cfl_target = 0.9, dtmax = 1.0, grav64 = 9.81;
     std::vector<double> ledger;        // accepted inward water per boundary
     std::vector<double> area;          // host copy for the mass inventory
     std::vector<int> series_offset;
@@ -164,6 +168,9 @@ struct Solver : SolverBase {
     double *qscratch = nullptr;        // [3*nc] double for conversions
     RiverBuffers rb;
     TurbineView tv{};
+    ExnerView ex{};
+    double *zsnap = nullptr;             // bed snapshot (paired with st[4])
+    double *ex_host = nullptr;           // pinned [8] result readback
     double *diag_host = nullptr;       // pinned [2][nrot][TURBINE_DIAG]
     void set_events(hipEvent_t *ev, int mask) override { cfg.ev = ev; cfg.ev_mask = mask; profiling = ev != nullptr; }
 
@@ -171,6 +178,7 @@ struct Solver : SolverBase {
     {
         if (host_block) (void)hipHostFree(host_block);
         if (diag_host) (void)hipHostFree(diag_host);
+        if (ex_host) (void)hipHostFree(ex_host);
         for (void *p : allocs) (void)hipFree(p);
         for (auto e : events) (void)hipEventDestroy(e);
     }
@@ -200,7 +208,7 @@ struct Solver : SolverBase {
     {
         nc = mesh->nc; nf = mesh->nf; nb = mesh->nb;
         const int nadj = mesh->nadj;
-        cfl_target = opt->cfl_target; dtmax = opt->dtmax;
+        cfl_target = opt->cfl_target; dtmax = opt->dtmax; grav64 = opt->grav;
         cfg.block = opt->block > 0 ? opt->block : 256;
         cfg.remap = opt->remap;
         m.nc = nc; m.nf = nf; m.nb = nb; m.nadj = nadj; m.scheme = opt->scheme;
@@ -277,6 +285,8 @@ struct Solver : SolverBase {
         area.assign(mesh->area, mesh->area + nc);
         manning_host.assign(mesh->manning, mesh->manning + nc);
         xy_host.assign(mesh->xy, mesh->xy + 2 * (size_t)nc);
+        length_host.assign(mesh->length, mesh->length + nf);
+        normal_host.assign(mesh->normal, mesh->normal + 2 * (size_t)nf);
         for (auto &s : st) {
             if ((rc = alloc(s.h, nc))) return rc;
             if ((rc = alloc(s.hu, nc))) return rc;
@@ -540,6 +550,75 @@ struct Solver : SolverBase {
             }
         }
         res->accepted = 1;
+        return BRAIDS_OK;
+    }
+
+    // -- Exner bed update ----------------------------------------------------
+    int exner_configure(const braids_exner_options *o) override
+    {
+        int rc = 0;
+        if (!ex.qs) {
+            const int nblk = (nc + 255) / 256;
+            ex.nblk = nblk;
+            if ((rc = alloc(ex.qs, 2 * (size_t)nc))) return rc;
+            if ((rc = alloc(ex.flux, nf))) return rc;
+            if ((rc = alloc(ex.fac, nc))) return rc;
+            if ((rc = alloc(ex.ztrial, nc))) return rc;
+            if ((rc = alloc(ex.partial, 4 * (size_t)nblk))) return rc;
+            if ((rc = alloc(ex.fail, 1))) return rc;
+            if ((rc = alloc(ex.result, 8))) return rc;
+            double *rk = nullptr;
+            if ((rc = alloc(rk, nc))) return rc;
+            ex.rock = rk;
+            if ((rc = upload(ex.length, length_host))) return rc;
+            if ((rc = upload(ex.normal, normal_host))) return rc;
+            if ((rc = upload(ex.area, area))) return rc;
+            if ((rc = upload(ex.manning, manning_host))) return rc;
+            HIP_TRY(hipHostMalloc(&ex_host, 8 * sizeof(double), hipHostMallocDefault));
+        }
+        ex.d50 = o->d50; ex.rd = o->relative_density; ex.porosity = o->porosity; ex.theta_c = o->critical_shields;
+        ex.morfac = o->morfac; ex.sed_depth = o->sediment_depth; ex.grav = grav64;
+        HIP_TRY(hipMemcpy(const_cast<double *>(ex.rock), o->rock, nc * sizeof(double), hipMemcpyHostToDevice));
+        return BRAIDS_OK;
+    }
+    int exner_finish(double dt, braids_exner_result *r) override
+    {
+        if (!ex.qs) { set_error("Exner update is not configured"); return BRAIDS_EINVAL; }
+        run_exner<R>(m, st[0], z, ex, dt * ex.morfac, cfg, stream);
+        HIP_TRY(hipGetLastError());
+        HIP_TRY(hipMemcpyAsync(ex_host, ex.result, 8 * sizeof(double), hipMemcpyDeviceToHost, stream));
+        HIP_TRY(hipStreamSynchronize(stream));
+        r->accepted = ex_host[0] != 0.; r->residual = ex_host[1]; r->boundary = ex_host[2];
+        r->change = ex_host[3]; r->donor_min = ex_host[4];
+        if (r->accepted) std::swap(z, ex.ztrial);
+        return r->accepted ? BRAIDS_OK : BRAIDS_REJECTED;
+    }
+    int snapshot(int restore) override
+    {
+        int rc = 0;
+        if (!zsnap && (rc = alloc(zsnap, nc))) return rc;
+        const size_t b = nc * sizeof(double);
+        if (!restore) {
+            HIP_TRY(hipMemcpyAsync(st[4].h, st[0].h, b, hipMemcpyDeviceToDevice, stream));
+            HIP_TRY(hipMemcpyAsync(st[4].hu, st[0].hu, b, hipMemcpyDeviceToDevice, stream));
+            HIP_TRY(hipMemcpyAsync(st[4].hv, st[0].hv, b, hipMemcpyDeviceToDevice, stream));
+            HIP_TRY(hipMemcpyAsync(zsnap, z, b, hipMemcpyDeviceToDevice, stream));
+            snap_ledger = ledger; snap_energy = energy; snap_comp = energy_comp;
+        } else {
+            HIP_TRY(hipMemcpyAsync(st[0].h, st[4].h, b, hipMemcpyDeviceToDevice, stream));
+            HIP_TRY(hipMemcpyAsync(st[0].hu, st[4].hu, b, hipMemcpyDeviceToDevice, stream));
+            HIP_TRY(hipMemcpyAsync(st[0].hv, st[4].hv, b, hipMemcpyDeviceToDevice, stream));
+            HIP_TRY(hipMemcpyAsync(z, zsnap, b, hipMemcpyDeviceToDevice, stream));
+            ledger = snap_ledger; energy = snap_energy; energy_comp = snap_comp;
+        }
+        HIP_TRY(hipStreamSynchronize(stream));
+        return BRAIDS_OK;
+    }
+    std::vector<double> snap_ledger, snap_energy, snap_comp;
+    int get_bed(double *zh) override
+    {
+        HIP_TRY(hipMemcpyAsync(zh, z, nc * sizeof(double), hipMemcpyDeviceToHost, stream));
+        HIP_TRY(hipStreamSynchronize(stream));
         return BRAIDS_OK;
     }
 
@@ -1101,6 +1180,36 @@ extern "C" int braids_turbine_footprint(int npairs, const double *tri, const dou
     cleanup();
     if (e != hipSuccess) { set_error(std::string("footprint: ") + hipGetErrorString(e)); return BRAIDS_EHIP; }
     return BRAIDS_OK;
+}
+
+// -- Exner bed update ---------------------------------------------------------------
+extern "C" int braids_exner_configure(braids_solver *s, const braids_exner_options *o)
+{
+    if (!s || !o || !o->rock || !(o->d50 > 0) || !(o->relative_density > 0) || !(o->porosity >= 0) ||
+        !(o->porosity < 1) || !(o->critical_shields >= 0) || !(o->morfac > 0) || !(o->sediment_depth > 0)) {
+        set_error("invalid Exner options"); return BRAIDS_EINVAL;
+    }
+    return s->impl->exner_configure(o);
+}
+extern "C" int braids_exner_finish(braids_solver *s, double dt, braids_exner_result *r)
+{
+    if (!s || !r || !(dt > 0)) { set_error("invalid Exner arguments"); return BRAIDS_EINVAL; }
+    return s->impl->exner_finish(dt, r);
+}
+extern "C" int braids_snapshot(braids_solver *s)
+{
+    if (!s) { set_error("null solver"); return BRAIDS_EINVAL; }
+    return s->impl->snapshot(0);
+}
+extern "C" int braids_restore(braids_solver *s)
+{
+    if (!s) { set_error("null solver"); return BRAIDS_EINVAL; }
+    return s->impl->snapshot(1);
+}
+extern "C" int braids_get_bed(braids_solver *s, double *z)
+{
+    if (!s || !z) { set_error("null argument"); return BRAIDS_EINVAL; }
+    return s->impl->get_bed(z);
 }
 
 // -- batched serving -------------------------------------------------------------

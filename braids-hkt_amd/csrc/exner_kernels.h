@@ -1,0 +1,148 @@
+// Exner bed update (ShorelinePhysics.finish, shoreline_solver.py:218-241) on
+// the device, FP64 throughout (the bed is the FP64 accumulator in both
+// precision modes):
+//   k_exner_cells    cells: Meyer-Peter--Mueller bedload vector
+//                    (braids_bedload_mpm_module.f90:15-41) with the Manning
+//                    skin friction cf = g n^2 / h^(1/3)
+//   k_exner_faces    faces: the face-normal bedload flux (wet-side average,
+//                    owner's vector on open faces, zero on walls)
+//   k_exner_donors   cells: donor factors of the erodible stock
+//                    phi (z - rock) (donor_factors, bend_methods_v1_kernels.h:394-400)
+//   k_exner_update   cells: limited fluxes (limit_flux, :401-403) applied per
+//                    incident face, the divergence (:389-392), the trial bed,
+//                    the rock / increment checks and per-block partial sums of
+//                    the solid residual and boundary volume
+//   k_exner_reduce   one workgroup: fixed-order sums of the partials and the
+//                    accept decision
+#pragma once
+#include "swe_kernels.h"
+
+namespace braids {
+
+// braids_mpm_bedload (Zhu et al. 2018, Eq. 12)
+__device__ __forceinline__ void mpm_bedload(double u, double v, double cf, double d50, double rd, double g,
+                                            double thc, double &qx, double &qy)
+{
+    qx = 0.; qy = 0.;
+    const double speed = sqrt(u * u + v * v);
+    if (speed == 0.) return;
+    double theta = cf * (speed * speed) / (g * rd * d50);
+    theta = fmax(0., theta - thc);
+    const double magnitude = 8. * sqrt(g * rd * (d50 * d50 * d50)) * pow(theta, 1.5);
+    qx = magnitude * u / speed; qy = magnitude * v / speed;
+}
+
+template <class R>
+__global__ void __launch_bounds__(256) k_exner_cells(MeshView<R> m, StateView<R> q, ExnerView E, int remap)
+{
+    const int c = logical_block(remap) * blockDim.x + threadIdx.x;
+    if (c >= m.nc) return;
+    const double h = q.h[c];
+    double qx = 0., qy = 0.;
+    if (h > E.sed_depth) {
+        const double n = E.manning[c], g = E.grav;     // FP64 gravity in both modes
+        const double cf = g * (n * n) / pow(h, 1. / 3.);
+        mpm_bedload(q.hu[c] / h, q.hv[c] / h, cf, E.d50, E.rd, g, E.theta_c, qx, qy);
+    }
+    E.qs[c] = qx; E.qs[m.nc + c] = qy;
+}
+
+template <class R>
+__global__ void __launch_bounds__(256) k_exner_faces(MeshView<R> m, StateView<R> q, ExnerView E, int remap)
+{
+    const int f = logical_block(remap) * blockDim.x + threadIdx.x;
+    if (f >= m.nf) return;
+    const int nc = m.nc, o = m.owner[f], j = m.neighbour[f];
+    const double nx = E.normal[f], ny = E.normal[m.nf + f];
+    double F = 0.;
+    if (j >= 0) {
+        if (q.h[o] > E.sed_depth && q.h[j] > E.sed_depth)
+            F = .5 * ((E.qs[o] + E.qs[j]) * nx + (E.qs[nc + o] + E.qs[nc + j]) * ny);
+    } else if (m.bid[f] >= 0 && q.h[o] > E.sed_depth) {
+        F = E.qs[o] * nx + E.qs[nc + o] * ny;
+    }
+    E.flux[f] = F;
+}
+
+template <class R>
+__global__ void __launch_bounds__(256) k_exner_donors(MeshView<R> m, const double *__restrict__ z0, ExnerView E,
+                                                      double dt, int remap)
+{
+    const int c = logical_block(remap) * blockDim.x + threadIdx.x;
+    if (c >= m.nc) return;
+    const double phi = 1 - E.porosity;
+    const double available = phi * (z0[c] - E.rock[c]);
+    double outgoing = 0.;
+    for (int k = m.offset[c]; k < m.offset[c + 1]; ++k) {
+        const int f = m.adj_face[k];
+        const double sg = (m.adj_flags[k] & ADJ_OWNER) ? 1. : -1.;
+        outgoing += fmax(0., sg * E.flux[f]) * E.length[f];
+    }
+    E.fac[c] = outgoing > 0. ? fmin(1., fmax(0., available) * E.area[c] / (dt * outgoing)) : 1.;
+}
+
+template <class R>
+__global__ void __launch_bounds__(256) k_exner_update(MeshView<R> m, StateView<R> q, const double *__restrict__ z0,
+                                                      ExnerView E, double dt, int remap)
+{
+    __shared__ double sh[256];
+    const int c = logical_block(remap) * blockDim.x + threadIdx.x;
+    const double phi = 1 - E.porosity, scale = dt / phi;
+    double res = 0., bnd = 0., chg = 0., fmn = 1.;
+    int fail = 0;
+    if (c < m.nc) {
+        double r = 0.;
+        for (int k = m.offset[c]; k < m.offset[c + 1]; ++k) {
+            const int f = m.adj_face[k];
+            const bool own = (m.adj_flags[k] & ADJ_OWNER) != 0;
+            double F = E.flux[f];
+            const int donor = F >= 0. ? m.owner[f] : m.neighbour[f];
+            if (donor >= 0) F = F * E.fac[donor];
+            const double L = E.length[f];
+            r -= (own ? 1. : -1.) * L * F;
+            if (m.neighbour[f] < 0 && m.bid[f] >= 0) bnd += dt * F * L;
+        }
+        const double A = E.area[c];
+        const double zt = z0[c] + scale * (r / A);
+        E.ztrial[c] = zt;
+        if (!isfinite(zt) || zt < E.rock[c] - 1e-13) fail = 1;
+        res = phi * (zt - z0[c]) * A;
+        chg = fabs(zt - z0[c]) / fmax(q.h[c], E.sed_depth);
+        fmn = E.fac[c];
+    }
+    res = block_sum<256>(res, sh);
+    bnd = block_sum<256>(bnd, sh);
+    chg = -block_min<256>(-chg, sh);
+    fmn = block_min<256>(fmn, sh);
+    if (threadIdx.x == 0) {
+        E.partial[blockIdx.x] = res; E.partial[E.nblk + blockIdx.x] = bnd;
+        E.partial[2 * E.nblk + blockIdx.x] = chg; E.partial[3 * E.nblk + blockIdx.x] = fmn;
+    }
+    if (fail) atomicOr(E.fail, 1);
+}
+
+static __global__ void __launch_bounds__(256) k_exner_reduce(ExnerView E)
+{
+    __shared__ double sh[256];
+    double res = 0., bnd = 0., chg = 0., fmn = 1.;
+    for (int b = threadIdx.x; b < E.nblk; b += 256) {
+        res += E.partial[b]; bnd += E.partial[E.nblk + b];
+        chg = fmax(chg, E.partial[2 * E.nblk + b]); fmn = fmin(fmn, E.partial[3 * E.nblk + b]);
+    }
+    res = block_sum<256>(res, sh);
+    bnd = block_sum<256>(bnd, sh);
+    chg = -block_min<256>(-chg, sh);
+    fmn = block_min<256>(fmn, sh);
+    if (threadIdx.x == 0) {
+        const double boundary = -bnd, residual = res - boundary;
+        const int ok = !*E.fail && fabs(residual) <= 1e-10 && chg <= .05;
+        E.result[0] = ok; E.result[1] = residual; E.result[2] = boundary; E.result[3] = chg; E.result[4] = fmn;
+    }
+}
+
+static __global__ void k_exner_clear(ExnerView E)
+{
+    if (threadIdx.x == 0 && blockIdx.x == 0) *E.fail = 0;
+}
+
+}  // namespace braids

@@ -11,6 +11,8 @@ template void run_rates<double>(const MeshView<double> &, StateView<double>, con
 template void run_finalize<double>(Work<double>, Control *, hipStream_t);
 template void run_batch_substep<double>(const BatchPlan &, const SlotView<double> *, SlotRun *, const int *,
                                         unsigned int *const *, hipStream_t);
+template void run_exner<double>(const MeshView<double> &, StateView<double>, const double *, ExnerView, double,
+                                const LaunchCfg &, hipStream_t);
 template void run_river_substep<double>(const MeshView<double> &, StateView<double>, StateView<double>, StateView<double>,
                                         StateView<double>, const double *, Work<double>, Control *, const RiverBuffers &,
                                         const TurbineView *, double, double, const LaunchCfg &, hipStream_t);

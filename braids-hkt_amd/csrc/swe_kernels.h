@@ -367,6 +367,21 @@ __device__ __forceinline__ double block_sum(double v, double *sh)
     return r;
 }
 
+template <int BLOCK>
+__device__ __forceinline__ double block_min(double v, double *sh)
+{
+    sh[threadIdx.x] = v;
+    __syncthreads();
+#pragma unroll
+    for (int s = BLOCK / 2; s > 0; s >>= 1) {
+        if ((int)threadIdx.x < s) sh[threadIdx.x] = fmin(sh[threadIdx.x], sh[threadIdx.x + s]);
+        __syncthreads();
+    }
+    const double r = sh[0];
+    __syncthreads();
+    return r;
+}
+
 constexpr int TRACE_BLOCK = 256;
 
 // Boundary traces: one workgroup per open boundary.  The sums of the

@@ -92,6 +92,11 @@ template <class R>
 void run_batch_substep(const BatchPlan &P, const SlotView<R> *V, SlotRun *run, const int *nzero,
                        unsigned int *const *blocks, hipStream_t stream);
 
+// Exner bed update from the state q and bed z0 into E.ztrial and E.result.
+template <class R>
+void run_exner(const MeshView<R> &m, StateView<R> q, const double *z0, ExnerView E, double dt, const LaunchCfg &cfg,
+               hipStream_t stream);
+
 // Footprint weights of (rotor, cell) pairs (device arrays).
 void run_footprint(int npairs, const double *tri, const double *D, const double *nodes, const double *weights,
                    double *value, double *clipped, int *order, hipStream_t stream);

@@ -5,6 +5,7 @@
 #include "swe_kernels.h"
 #include "river_kernels.h"
 #include "batch_kernels.h"
+#include "exner_kernels.h"
 #include <hip/hip_ext.h>
 #include <type_traits>
 
@@ -180,6 +181,19 @@ void run_batch_substep(const BatchPlan &P, const SlotView<R> *V, SlotRun *run, c
     if (P.nbnd > 0)
         hipLaunchKernelGGL((kb_ledger<R>), dim3(P.nbnd), dim3(TRACE_BLOCK), 0, s, P, V, (const SlotRun *)run);
     hipLaunchKernelGGL((kb_finalize<R>), dim3(P.nslot), dim3(64), 0, s, V, run, P.nslot);
+}
+
+template <class R>
+void run_exner(const MeshView<R> &m, StateView<R> q, const double *z0, ExnerView E, double dt, const LaunchCfg &cfg,
+               hipStream_t s)
+{
+    const dim3 gc(grid_for(m.nc, 256)), gf(grid_for(m.nf, 256)), blk(256);
+    hipLaunchKernelGGL(k_exner_clear, dim3(1), dim3(64), 0, s, E);
+    hipLaunchKernelGGL((k_exner_cells<R>), gc, blk, 0, s, m, q, E, cfg.remap);
+    hipLaunchKernelGGL((k_exner_faces<R>), gf, blk, 0, s, m, q, E, cfg.remap);
+    hipLaunchKernelGGL((k_exner_donors<R>), gc, blk, 0, s, m, z0, E, dt, cfg.remap);
+    hipLaunchKernelGGL((k_exner_update<R>), gc, blk, 0, s, m, q, z0, E, dt, cfg.remap);
+    hipLaunchKernelGGL(k_exner_reduce, dim3(1), dim3(256), 0, s, E);
 }
 
 }  // namespace braids

@@ -102,4 +102,19 @@ struct TurbineView {
 
 
 
+// Exner bed update buffers and sediment parameters (FP64).
+struct ExnerView {
+    double d50, rd, porosity, theta_c, morfac, sed_depth, grav;
+    const double *rock;
+    const double *length, *normal, *area, *manning;   // FP64 geometry in both precision modes
+    double *qs;          // [2*nc] cell bedload vectors
+    double *flux;        // [nf] face-normal bedload flux (raw)
+    double *fac;         // [nc] donor factors
+    double *ztrial;      // [nc]
+    double *partial;     // [4*nblk]: residual, boundary, max change, min factor
+    int *fail;
+    double *result;      // [8]: ok, residual, boundary, change, donor min
+    int nblk;
+};
+
 }  // namespace braids

@@ -186,6 +186,32 @@ int braids_turbines_read(braids_solver *s, double *diag, double *energy);
 int braids_turbine_footprint(int npairs, const double *tri, const double *D, const double *nodes,
                              const double *weights, double *value, double *clipped, int *order);
 
+/* ---- Exner bed update (ShorelinePhysics.finish, shoreline_solver.py:218-241) ----
+ * Meyer-Peter--Mueller bedload (braids_bedload_mpm_module.f90:15-41) with the
+ * Manning skin friction, face-normal flux from the wet sides, donor limiting
+ * of the erodible stock phi (z - rock) (donor_factors / limit_flux,
+ * bend_methods_v1_kernels.h:394-403) and z += (dt morfac / phi) div
+ * (divergence, :389-392), all FP64.  Rejected (BRAIDS_REJECTED, bed kept) if
+ * the bed drops below the rock, the solid-volume residual exceeds 1e-10 or
+ * the increment exceeds 5 % of the depth; accepted updates the bed. */
+typedef struct braids_exner_options {
+    double d50, relative_density, porosity, critical_shields, morfac, sediment_depth;
+    const double *rock;          /* [nc] host: non-erodible elevation */
+} braids_exner_options;
+typedef struct braids_exner_result {
+    int accepted;
+    double residual, boundary, change, donor_min;
+} braids_exner_result;
+int braids_exner_configure(braids_solver *s, const braids_exner_options *o);
+/* Bed update over the interval dt from the resident state. */
+int braids_exner_finish(braids_solver *s, double dt, braids_exner_result *r);
+/* Save / restore the resident state, bed, ledger and turbine energy (the
+ * owner's rollback of a rejected outer step). */
+int braids_snapshot(braids_solver *s);
+int braids_restore(braids_solver *s);
+/* Host copy of the resident bed [nc]. */
+int braids_get_bed(braids_solver *s, double *z);
+
 /* ---- Batched serving (no reference counterpart: the reference runs one
  * scenario per owner, river_worker.py) ----
  * A batch advances many solver handles (scenarios) together: every kernel of

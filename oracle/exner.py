@@ -1,0 +1,95 @@
+"""Oracle loader for the Exner bed update (test infrastructure only): the
+plain-C restatement (exner_oracle.c in liboracle.so) and, where built, the
+reference's Meyer-Peter--Mueller subroutine (amdflang, _ref/libbraids_ref.so)
+and its bed operator header (g++, _ref/libbed_kernels_ref.so)."""
+import ctypes
+
+import numpy as np
+
+from . import swe
+
+HERE = swe.HERE
+REF_BED = HERE / '_ref' / 'libbed_kernels_ref.so'
+_D, _P, _I = ctypes.c_double, ctypes.c_void_p, ctypes.c_int
+_BED = {'divergence': [_I, _I, _I] + [_P] * 7,
+        'donor_factors': [_I, _I, _I, _D] + [_P] * 8,
+        'limit_flux': [_I, _I, _P, _P, _P, _P]}
+
+
+def _ptr(a):
+    return None if a is None else ctypes.c_void_p(a.ctypes.data)
+
+
+def lib():
+    L = swe.lib()
+    if not getattr(L, '_exner_bound', False):
+        L.oex_mpm.argtypes = [_D] * 7 + [_P, _P]; L.oex_mpm.restype = _I
+        for n, a in _BED.items():
+            fn = getattr(L, 'oex_' + n); fn.argtypes = a; fn.restype = None
+        L.oex_face_flux.argtypes = [_P, _P, _D, _D, _D, _D, _P, _P]; L.oex_face_flux.restype = None
+        L.oex_finish.argtypes = [_P, _P, _P, _D, _D, _D, _D, _D, _D, _P, _P, _D, _P, _P, _P, _P, _P]
+        L.oex_finish.restype = _I
+        L._exner_bound = True
+    return L
+
+
+def mpm(u, v, cf, d50=0.02, rd=1.65, g=9.81, thc=0.047, fn=None):
+    qx = np.zeros(1); qy = np.zeros(1)
+    f = fn or lib().oex_mpm
+    ok = f(float(u), float(v), float(cf), float(d50), float(rd), float(g), float(thc), _ptr(qx), _ptr(qy))
+    return float(qx[0]), float(qy[0]), int(ok)
+
+
+def reference_mpm():
+    if not swe.REF_LIB.exists():
+        return None
+    L = ctypes.CDLL(str(swe.REF_LIB))
+    fn = L.ref_mpm_bedload
+    fn.argtypes = [_D] * 7 + [_P, _P, _P]; fn.restype = None
+
+    def call(u, v, cf, d50, rd, g, thc, pqx, pqy):
+        ok = np.zeros(1, np.int32)
+        fn(u, v, cf, d50, rd, g, thc, pqx, pqy, _ptr(ok))
+        return int(ok[0])
+    return call
+
+
+def bed_ops(reference=False):
+    if reference:
+        if not REF_BED.exists():
+            return None
+        L = ctypes.CDLL(str(REF_BED))
+        L.bend_threads.argtypes = [_I]; L.bend_threads(1)
+        out = {}
+        for n, a in _BED.items():
+            fn = getattr(L, n); fn.argtypes = a; fn.restype = None; out[n] = fn
+        return out
+    L = lib()
+    return {n: getattr(L, 'oex_' + n) for n in _BED}
+
+
+class ExnerOracle:
+    """ShorelinePhysics.finish (bed part) on one mesh."""
+
+    def __init__(self, mesh, rock, d50=0.02, rd=1.65, porosity=0.4, theta_c=0.047, morfac=1.0, sediment_depth=0.001):
+        self.L = lib()
+        self.o = swe.Oracle(mesh, scheme=5)
+        self.mesh = mesh
+        self.rock = np.ascontiguousarray(rock, dtype=np.float64)
+        self.man = np.ascontiguousarray(mesh.manning, dtype=np.float64)
+        self.p = (d50, rd, porosity, theta_c, morfac, sediment_depth)
+
+    def face_flux(self, q):
+        q = np.ascontiguousarray(q, dtype=np.float64); out = np.zeros(self.mesh.nf)
+        d50, rd, por, thc, mf, sd = self.p
+        self.L.oex_face_flux(ctypes.byref(self.o._mesh), _ptr(self.man), d50, rd, thc, sd, _ptr(q), _ptr(out))
+        return out
+
+    def finish(self, q, z0, dt):
+        q = np.ascontiguousarray(q, dtype=np.float64); z0 = np.ascontiguousarray(z0, dtype=np.float64)
+        zt = np.zeros_like(z0); r = np.zeros(4)
+        d50, rd, por, thc, mf, sd = self.p
+        ok = self.L.oex_finish(ctypes.byref(self.o._mesh), _ptr(self.man), _ptr(self.rock), d50, rd, por, thc, mf, sd,
+                               _ptr(q), _ptr(z0), float(dt), _ptr(zt), _ptr(r[0:]), _ptr(r[1:]), _ptr(r[2:]),
+                               _ptr(r[3:]))
+        return dict(ok=bool(ok), z=zt, residual=r[0], boundary=r[1], change=r[2], donor=r[3])

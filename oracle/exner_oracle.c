@@ -1,0 +1,130 @@
+/*
+ * Oracle (test infrastructure only): FP64 restatement of the Exner bed
+ * update of the river path.
+ *   braids_mpm_bedload            braids_bedload_mpm_module.f90:15-41
+ *                                 (Meyer-Peter--Mueller, Zhu et al. 2018 Eq. 12)
+ *   divergence, donor_factors,
+ *   limit_flux                    bend_methods_v1_kernels.h:389-405
+ *   ShorelinePhysics.finish       shoreline_solver.py:218-241 (bed part)
+ * The face bedload flux is this build's closure (the reference's face chain
+ * bed_faces -> matrix_bed_faces is outside this restatement): the cell MPM
+ * vectors, with the Manning skin friction cf = g n^2 / h^(1/3), averaged over
+ * the two sides of an interior face whose cells are both wet (h > sediment
+ * depth, the active-face rule of ShorelinePhysics.bed_faces,
+ * shoreline_solver.py:209-216), the owner's vector on an open boundary face,
+ * zero on walls; projected on the unit normal.
+ */
+#include "river_oracle.h"
+#include <math.h>
+#include <stdlib.h>
+#include <string.h>
+
+int oex_mpm(double u, double v, double cf, double d50, double rd, double g, double thc, double *qx, double *qy)
+{
+    *qx = 0.; *qy = 0.;
+    if (!(isfinite(u) && isfinite(v) && isfinite(cf) && isfinite(d50) && isfinite(rd) && isfinite(g) && isfinite(thc)))
+        return 0;
+    if (!(fmin(fmin(d50, rd), g) > 0. && fmin(cf, thc) >= 0.)) return 0;
+    double speed = sqrt(u * u + v * v);
+    if (speed == 0.) return 1;
+    double theta = cf * (speed * speed) / (g * rd * d50);
+    theta = fmax(0., theta - thc);
+    double magnitude = 8. * sqrt(g * rd * (d50 * d50 * d50)) * pow(theta, 1.5);
+    *qx = magnitude * u / speed; *qy = magnitude * v / speed;
+    return isfinite(*qx) && isfinite(*qy);
+}
+
+void oex_divergence(int nc, int nf, int component, const double *flux, const double *length, const double *area,
+                    const int *off, const int *faces, const int *signs, double *out)
+{
+    for (int i = 0; i < nc; ++i) {
+        double r = 0.;
+        for (int k = off[i]; k < off[i + 1]; ++k) r -= signs[k] * length[faces[k]] * flux[component * nf + faces[k]];
+        out[i] = r / area[i];
+    }
+}
+
+void oex_donor_factors(int nc, int nf, int component, double dt, const double *available, const double *flux,
+                       const double *length, const double *area, const int *off, const int *faces, const int *signs,
+                       double *out)
+{
+    for (int i = 0; i < nc; ++i) {
+        double outgoing = 0.;
+        for (int k = off[i]; k < off[i + 1]; ++k) {
+            int f = faces[k];
+            outgoing += fmax(0., signs[k] * flux[component * nf + f]) * length[f];
+        }
+        out[i] = outgoing > 0. ? fmin(1., fmax(0., available[i]) * area[i] / (dt * outgoing)) : 1.;
+    }
+}
+
+void oex_limit_flux(int nf, int component, const int *owner, const int *nb, const double *fac, double *flux)
+{
+    for (int i = 0; i < nf; ++i) {
+        int j = flux[component * nf + i] >= 0. ? owner[i] : nb[i];
+        if (j >= 0) flux[component * nf + i] *= fac[j];
+    }
+}
+
+void oex_face_flux(const om_mesh *m, const double *manning, double d50, double rd, double thc, double sed_depth,
+                   const double *q, double *flux)
+{
+    const int nc = m->nc, nf = m->nf;
+    const double g = m->grav;
+    double *qs = malloc(sizeof(double) * 2 * nc);
+    for (int c = 0; c < nc; ++c) {
+        double h = q[c];
+        qs[c] = qs[nc + c] = 0.;
+        if (h <= sed_depth) continue;
+        double u = q[nc + c] / h, v = q[2 * nc + c] / h;
+        double cf = g * (manning[c] * manning[c]) / pow(h, 1. / 3.);
+        oex_mpm(u, v, cf, d50, rd, g, thc, &qs[c], &qs[nc + c]);
+    }
+    for (int f = 0; f < nf; ++f) {
+        int o = m->owner[f], j = m->neighbour[f];
+        double nx = m->normal[f], ny = m->normal[nf + f], F = 0.;
+        if (j >= 0) {
+            if (q[o] > sed_depth && q[j] > sed_depth)
+                F = .5 * ((qs[o] + qs[j]) * nx + (qs[nc + o] + qs[nc + j]) * ny);
+        } else if (m->bid[f] >= 0 && q[o] > sed_depth) {
+            F = qs[o] * nx + qs[nc + o] * ny;
+        }
+        flux[f] = F;
+    }
+    free(qs);
+}
+
+/* The bed part of ShorelinePhysics.finish: returns 1 if accepted. */
+int oex_finish(const om_mesh *m, const double *manning, const double *rock, double d50, double rd, double porosity,
+               double thc, double morfac, double sed_depth, const double *q, const double *z0, double step_dt,
+               double *ztrial, double *residual_out, double *boundary_out, double *change_out, double *donor_out)
+{
+    const int nc = m->nc, nf = m->nf;
+    const double dt = step_dt * morfac, phi = 1 - porosity;
+    double *flux = malloc(sizeof(double) * nf), *available = malloc(sizeof(double) * nc);
+    double *fac = malloc(sizeof(double) * nc), *div = malloc(sizeof(double) * nc);
+    oex_face_flux(m, manning, d50, rd, thc, sed_depth, q, flux);
+    for (int c = 0; c < nc; ++c) available[c] = phi * (z0[c] - rock[c]);
+    oex_donor_factors(nc, nf, 0, dt, available, flux, m->length, m->area, m->offset, m->faces, m->signs, fac);
+    oex_limit_flux(nf, 0, m->owner, m->neighbour, fac, flux);
+    double donor = INFINITY;
+    for (int c = 0; c < nc; ++c) donor = fmin(donor, fac[c]);
+    oex_divergence(nc, nf, 0, flux, m->length, m->area, m->offset, m->faces, m->signs, div);
+    for (int c = 0; c < nc; ++c) ztrial[c] = z0[c] + (dt / phi) * div[c];
+    double boundary = 0., residual = 0., change = 0.;
+    for (int f = 0; f < nf; ++f)
+        if (m->neighbour[f] < 0 && m->bid[f] >= 0) boundary += dt * flux[f] * m->length[f];
+    boundary = -boundary;
+    for (int c = 0; c < nc; ++c) {
+        residual += phi * (ztrial[c] - z0[c]) * m->area[c];
+        change = fmax(change, fabs(ztrial[c] - z0[c]) / fmax(q[c], sed_depth));
+    }
+    residual -= boundary;
+    int ok = 1;
+    for (int c = 0; c < nc; ++c)
+        if (!isfinite(ztrial[c]) || ztrial[c] < rock[c] - 1e-13) ok = 0;
+    if (fabs(residual) > 1e-10 || change > .05) ok = 0;
+    *residual_out = residual; *boundary_out = boundary; *change_out = change; *donor_out = donor;
+    free(flux); free(available); free(fac); free(div);
+    return ok;
+}

@@ -6,6 +6,7 @@ module braids_ref_driver
  use iso_c_binding
  use geoclaw_module, only: grav,dry_tolerance
  use cutcell_solver, only: wave_flux
+ use braids_bedload_mpm_module, only: braids_mpm_bedload
  implicit none
 contains
  subroutine ref_set_constants(g,dry) bind(C,name='ref_set_constants')
@@ -30,5 +31,15 @@ contains
   external riemanntype
   call riemanntype(hl,hr,ul,ur,hm,s1m,s2m,r1,r2,1,dry_tolerance,grav)
   rare1=merge(1,0,r1);rare2=merge(1,0,r2)
+ end subroutine
+
+ ! Meyer-Peter--Mueller bedload per unit width, braids_bedload_mpm_module.f90:15-41
+ subroutine ref_mpm_bedload(u,v,cf,d50,rd,g,thc,qx,qy,valid) bind(C,name='ref_mpm_bedload')
+  real(c_double),value :: u,v,cf,d50,rd,g,thc
+  real(c_double),intent(out) :: qx,qy
+  integer(c_int),intent(out) :: valid
+  logical :: ok
+  call braids_mpm_bedload(u,v,cf,d50,rd,g,thc,qx,qy,ok)
+  valid=merge(1,0,ok)
  end subroutine
 end module

@@ -1,0 +1,118 @@
+"""Pinning of the Exner oracle (oracle/exner_oracle.c).
+
+* braids_mpm_bedload (the reference Fortran, amdflang build) and the bed
+  operators divergence / donor_factors / limit_flux (the reference header,
+  g++ build) are matched bit for bit: on committed golden vectors always, and
+  live when oracle/_ref is built.
+* MPM reduces to 8 sqrt(g D d^3) (theta - theta_c)^1.5 on a uniform stream.
+* The bed update of ShorelinePhysics.finish conserves solids exactly against
+  its boundary ledger, never digs below the rock, keeps a uniform stream's
+  bed unchanged in the interior, and rejects a too-large increment.
+"""
+from pathlib import Path
+
+import numpy as np
+import pytest
+
+from braids_hip import mesh as M
+from oracle import exner as OX
+
+GOLD = Path(__file__).resolve().parent / 'golden' / 'exner_ref.npz'
+
+
+def test_mpm_matches_reference_golden():
+    g = np.load(GOLD)
+    got = np.array([OX.mpm(u, v, cf, d50) for u, v, cf, d50 in g['mpm_in']])
+    assert np.array_equal(got, g['mpm_out'])
+    assert (got[:, 0] != 0).sum() > 1000            # the cases transport sediment
+
+
+def test_mpm_matches_live_reference():
+    ref = OX.reference_mpm()
+    if ref is None:
+        pytest.skip('oracle/_ref not built here')
+    rng = np.random.default_rng(9)
+    for _ in range(5000):
+        u, v = rng.normal(0, 3, 2); cf = rng.uniform(0, 0.03); d50 = rng.uniform(1e-4, 0.1)
+        assert OX.mpm(u, v, cf, d50) == OX.mpm(u, v, cf, d50, fn=ref)
+
+
+@pytest.mark.parametrize('k', range(3))
+def test_bed_operators_match_reference_golden(k):
+    from tests.golden.make_golden_exner import case_mesh, run_ops
+    g = np.load(GOLD)
+    x = dict(flux=g[f'c{k}_flux'], available=g[f'c{k}_available'], dt=float(g[f'c{k}_dt']))
+    y = run_ops(OX.bed_ops(), case_mesh(), x)
+    for name in ('factors', 'limited', 'divergence'):
+        assert np.array_equal(y[name], g[f'c{k}_{name}']), name
+    assert g[f'c{k}_factors'].min() < 1.0
+
+
+def test_bed_operators_match_live_reference():
+    ops = OX.bed_ops(reference=True)
+    if ops is None:
+        pytest.skip('oracle/_ref not built here')
+    from tests.golden.make_golden_exner import op_inputs, run_ops
+    m = M.braided_reach(nx=20, ny=6, length=100.0, width=30.0, nbars=4)
+    for seed in range(10):
+        x = op_inputs(m, 50 + seed)
+        a, b = run_ops(ops, m, x), run_ops(OX.bed_ops(), m, x)
+        for n in a:
+            assert np.array_equal(a[n], b[n]), (seed, n)
+
+
+def test_mpm_closed_form():
+    g, rd, d, thc = 9.81, 1.65, 0.02, 0.047
+    for u, cf in ((2.0, 0.01), (1.2, 0.004), (3.0, 0.002)):
+        theta = cf * u * u / (g * rd * d)
+        want = 8 * np.sqrt(g * rd * d ** 3) * max(theta - thc, 0.0) ** 1.5
+        qx, qy, ok = OX.mpm(u, 0.0, cf, d, rd, g, thc)
+        assert ok and qy == 0.0 and np.isclose(qx, want, rtol=1e-14)
+    assert OX.mpm(0.1, 0.0, 0.004, d)[0] == 0.0        # below the threshold
+
+
+def channel(u=2.0, depth=2.0):
+    m = M.plain_channel(nx=60, ny=6, length=120.0, width=12.0, depth=depth, q=u * depth * 12.0, n=0.03)
+    m.z[:] = 0.0
+    m.q0[0] = depth; m.q0[1] = depth * u; m.q0[2] = 0.0
+    return m
+
+
+def test_finish_conserves_solids_and_respects_rock():
+    m = channel()
+    rng = np.random.default_rng(3)
+    q = m.q0.copy()
+    q[1] *= 1 + 0.3 * rng.standard_normal(m.nc)          # non-uniform stream: erosion and deposition
+    rock = m.z - rng.uniform(0, 2e-4, m.nc)                # a thin erodible layer
+    ex = OX.ExnerOracle(m, rock)
+    r = ex.finish(q, m.z, 0.5)
+    assert r['donor'] < 1.0                                # the donor limiter acts
+    assert np.all(r['z'] >= rock - 1e-13)
+    # solids: sum phi dz A equals the inward boundary volume
+    phi = 0.6
+    net = np.sum(phi * (r['z'] - m.z) * m.area)
+    assert abs(net - r['boundary']) <= 1e-12 * max(1e-9, abs(r['boundary']))
+    assert abs(r['residual']) <= 1e-10
+
+
+def test_finish_uniform_stream_keeps_interior_bed():
+    m = channel()
+    rock = m.z - 5.0
+    ex = OX.ExnerOracle(m, rock)
+    r = ex.finish(m.q0, m.z, 1.0)
+    interior = np.ones(m.nc, bool)
+    for f in np.flatnonzero(m.neighbour < 0):
+        interior[m.owner[f]] = False
+    assert r['ok']
+    assert np.abs(r['z'][interior] - m.z[interior]).max() <= 1e-15
+    # the uniform flux leaves through the outlet and enters through the inlet:
+    # only boundary cells change, by equal and opposite volumes
+    assert abs(r['boundary']) <= 1e-12
+
+
+def test_finish_rejects_large_increment():
+    m = channel(u=4.0, depth=0.5)
+    q = m.q0.copy(); q[1, : m.nc // 2] *= 0.1              # strong convergence of bedload
+    ex = OX.ExnerOracle(m, m.z - 5.0)
+    r = ex.finish(q, m.z, 500.0)
+    assert not r['ok'] and r['change'] > 0.05
