@@ -23,7 +23,8 @@ import re
 # kernel-name prefix -> kind used by the solver's event profiler
 KINDS = {'k_prep': 'prep', 'k_prep_ls64': 'prep', 'k_traces': 'traces', 'k_face': 'face',
          'k_face_f32': 'face', 'k_drain': 'drain', 'k_gather': 'gather', 'k_ledger': 'ledger',
-         'k_stream_copy': 'copy'}
+         'k_river_theta': 'rtheta', 'k_river_update': 'rupdate', 'k_river_drag': 'rdrag',
+         'k_river_ledger': 'ledger', 'k_stream_copy': 'copy'}
 
 
 def short_name(name):
