@@ -630,6 +630,19 @@ static int traces(om_ctx *x, const double *q, const double *z, int *failure)
     return 1;
 }
 
+/* Open-boundary traces of the state q at time t: boundary_hydraulic_at and
+ * traces (:9814-10533 for modes 2 and 6).  face_value is first order when the
+ * context's scheme is 1.  Returns 1, or 0 with the traces failure code. */
+int om_boundary_traces(om_ctx *x, const double *q, const double *z, double t, double *trace, int *failure)
+{
+    primitive_and_active(x, q, z);
+    boundary_hydraulic_at(x, t);
+    *failure = 0;
+    int ok = traces(x, q, z, failure);
+    memcpy(trace, x->trace, sizeof(double) * 3 * (size_t)x->m.nf);
+    return ok;
+}
+
 /* Face loop of euler (:10938-11110) for hydraulics only. */
 static void face_fluxes(om_ctx *x, const double *q, const double *z)
 {

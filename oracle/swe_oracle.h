@@ -63,6 +63,9 @@ int om_euler(om_ctx *ctx, const double *q, const double *z, double dt, double t,
 int om_rates(om_ctx *ctx, const double *q, const double *z, double dt, double t,
              double *rate, double *water);
 
+/* Open-boundary traces (h, hu, hv per face, zero elsewhere) of q at time t. */
+int om_boundary_traces(om_ctx *ctx, const double *q, const double *z, double t, double *trace, int *failure);
+
 /* prepare_hydraulic_stage, cutcell_solver.f90:8513-8534 (in place). */
 void om_prepare_stage(om_ctx *ctx, double *q);
 /* Manning part of source, cutcell_solver.f90:14580-14600 (in place). */
