@@ -13,9 +13,11 @@ _P = ctypes.c_void_p
 BRAIDS_OK = 0
 BRAIDS_REJECTED = 1
 KERNEL_KINDS = ('prep', 'traces', 'face', 'drain', 'gather', 'ledger',
-                'rdrag', 'rtheta', 'turbine', 'rupdate')
+                'rdrag', 'rtheta', 'turbine', 'rupdate',
+                'hrecon', 'hface', 'hbudget', 'hupdate')
 TURBINE_DIAG = 16
 RIVER_REASONS = {0: None, 1: 'spatial', 2: 'cfl', 3: 'boundary', 4: 'nonfinite', 5: 'negative', 6: 'turbine'}
+HLLC_REASONS = {0: None, 1: 'exterior', 2: 'cfl', 3: 'positivity', 4: 'momentum', 6: 'turbine'}
 
 
 class BraidsMesh(ctypes.Structure):
@@ -100,6 +102,10 @@ SIGNATURES = {
     'braids_profile': (_I, [_P, _I]),
     'braids_profile_read': (_I, [_P, _P, _P]),
     'braids_stream_copy': (_I, [_P, _P, ctypes.c_longlong, _P, _I, _P]),
+    'braids_hllc_configure': (_I, [_P, _D, _I, _P, _P, _I]),
+    'braids_hllc_step': (_I, [_P, _D, _D, _P]),
+    'braids_hllc_advance': (_I, [_P, _D, _D, _P, _P, _P, _P]),
+    'braids_hllc_run': (_I, [_P, _I, _P, _P, _P, _P]),
     'braids_engine_set_stream': (_I, [_P]),
     'explicit_frequency': (_I, [_I, _I, _D, _D, _P, _P, _P, _P, _P, _P, _P, _P, _P, _P, _P]),
     'midpoint_inventory': (_I, [_I, _D, _P, _P, _P, _P, _P, _P, _P, _P]),
@@ -107,6 +113,14 @@ SIGNATURES = {
     'midpoint_gather': (_I, [_I, _I, _P, _P, _P, _P, _P, _P, _P, _P]),
     'river_cell_drag': (_I, [_I, _D, _D, _D, _I, _D, _I, _P, _P, _P]),
     'river_cell_drag_spatial': (_I, [_I, _D, _D, _D, _I, _P, _I, _P, _P, _P]),
+    'hllc_reconstruct': (_I, [_I, _I, _D, _D, _D] + [_P] * 15),
+    'hllc_predict_source': (_I, [_I, _D, _D] + [_P] * 3),
+    'hllc_faces': (_I, [_I, _I, _D, _D] + [_P] * 13),
+    'hllc_budget': (_I, [_I, _I, _D] + [_P] * 12),
+    'hllc_limit': (_I, [_I, _I, _D] + [_P] * 8),
+    'hllc_update': (_I, [_I, _I, _D, _D] + [_P] * 11),
+    'hllc_turbine_pressure': (_I, [_I, _I, _D, _D] + [_P] * 9),
+    'hllc_advanced_frequency': (_I, [_I, _I, _D, _D] + [_P] * 9),
 }
 
 _lib = None

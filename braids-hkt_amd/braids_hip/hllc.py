@@ -1,0 +1,124 @@
+"""Python host side of the HLLC MUSCL-Hancock hydraulic owner, mirroring the
+reference's HLLCHydraulics (src/braids_core/hydraulics/hllc.py) and its river
+controller RiverHLLC (river_hllc.py).
+
+The reference drives six WaveEngine operators per substep with two host
+round-trips (the CFL frequency maximum and the update status).  Here a
+substep is one sequence of device launches (drag, reconstruction with the
+Hancock predictor, exterior traces, faces, budget, the update with the
+limiter applied on the fly) followed by one readback of the rejection flags;
+the substep and hint rules of HLLCHydraulics.advance run in
+braids_hllc_advance.  The rotor array installed on the solver (ThetisArray)
+is the momentum source, predicted at the start and evaluated at the half
+time, with the low-Froude face correction of river_turbines.py."""
+import numpy as np
+
+from .solver import HydraulicSolver, TrialRejected
+
+METHOD = 'hllc_muscl_hancock_polygon_v1'
+RIVER_METHOD = 'explicit_hllc_muscl_hancock_river_v1'
+CFL_SAFETY = .98
+
+
+def metric_inverse(mesh):
+    """[3, nc] (inv00, inv01, inv11) of the pseudo-inverse (rcond 1e-12) of each
+    cell's least-squares metric sum d d^T / |d|^2 over its neighbours and the
+    mirror images across its walls; open faces take no part (hllc.py:33-43)."""
+    nc = mesh.nc
+    off, faces, signs = mesh.offset, mesh.faces_csr, mesh.signs
+    cell = np.repeat(np.arange(nc), np.diff(off))
+    other = np.where(signs > 0, mesh.neighbour[faces], mesh.owner[faces])
+    open_face = (mesh.neighbour < 0) & (mesh.bid >= 0)
+    xy, mid, nrm = mesh.xy, mesh.mid, mesh.normal
+    d = np.zeros((2, faces.size))
+    inner = other >= 0
+    d[:, inner] = xy[:, other[inner]] - xy[:, cell[inner]]
+    wall = ~inner & ~open_face[faces]
+    fw = faces[wall]
+    dist = (mid[0, fw] - xy[0, cell[wall]]) * nrm[0, fw] + (mid[1, fw] - xy[1, cell[wall]]) * nrm[1, fw]
+    d[:, wall] = 2 * dist * nrm[:, fw]
+    nn = d[0] * d[0] + d[1] * d[1]
+    use = (inner | wall) & (nn > 0)
+    M = np.zeros((nc, 2, 2))
+    c = cell[use]; a, b, w = d[0, use], d[1, use], nn[use]
+    np.add.at(M[:, 0, 0], c, a * a / w)
+    np.add.at(M[:, 0, 1], c, a * b / w)
+    np.add.at(M[:, 1, 0], c, b * a / w)
+    np.add.at(M[:, 1, 1], c, b * b / w)
+    P = np.linalg.pinv(M, rcond=1e-12)
+    return np.ascontiguousarray(np.vstack([P[:, 0, 0], P[:, 0, 1], P[:, 1, 1]]))
+
+
+class HLLCHydraulics:
+    """HLLCHydraulics(solver, cfl=.9): Hancock integration with Strang drag
+    around each substep (river_hllc.py: the river owner's friction), the
+    stock-feasible exterior budget and the conservative donor limiter."""
+
+    def __init__(self, solver: HydraulicSolver, cfl=.9, drag_mode=2, roughness=None, pressure=None, inverse=None):
+        if isinstance(cfl, bool) or not np.isfinite(cfl) or not 0 < cfl <= 1:
+            raise ValueError('Hydraulic face-sum CFL must be in (0,1]')
+        if solver.precision != 64:
+            raise ValueError('The HLLC owner runs on FP64 solvers (precision 64)')
+        self.solver = solver
+        self.cfl = float(cfl)
+        self.inverse = metric_inverse(solver.mesh) if inverse is None else np.asarray(inverse, dtype=np.float64)
+        solver.hllc_configure(self.inverse, self.cfl, drag_mode, roughness,
+                              -1 if pressure is None else int(bool(pressure)))
+        self.step_hint = None
+        self.identity = dict(method=METHOD, order='piecewise-linear limited reconstruction; hancock',
+                             pressure='hydrostatic interface reconstruction; cell-reference pressure quadrature',
+                             riemann='HLLC; HLLE for dry faces or inadmissible contact states',
+                             cfl='dt max_cell(sum_face(length*signal_speed)/actual_area)', max_cfl=1.,
+                             friction='Strang drag around every actual Hancock substep',
+                             exterior='first-order boundary trace of the predictor at the half time; HLLC flux '
+                                      'between the owner state and the trace')
+
+    def state(self):
+        return dict(step_hint=self.step_hint)
+
+    def restore(self, state):
+        if set(state) != {'step_hint'}:
+            raise ValueError('HLLC clock state differs')
+        hint = state['step_hint']
+        if hint is not None and (not np.isfinite(hint) or hint <= 0):
+            raise ValueError('Invalid HLLC step hint')
+        self.step_hint = hint
+
+    def single(self, dt, time):
+        """One Hancock substep (_spatial_step); raises TrialRejected."""
+        if not np.isfinite(dt) or dt <= 0:
+            raise ValueError('Positive finite hydraulic interval required')
+        return self.solver.hllc_step(dt, time)
+
+    def advance(self, dt, time=0.):
+        """Advance the resident state over [time, time + dt]; returns the
+        reference's substep report (the state stays on the device)."""
+        if not np.isfinite(dt) or dt <= 0:
+            raise ValueError('Positive finite hydraulic interval required')
+        s = self.solver
+        s.hllc_hint = self.step_hint
+        n, rejections, cfl_max = s.hllc_advance(dt, time)
+        self.step_hint = s.hllc_hint
+        return dict(method=METHOD, hydraulic_substeps=n, hydraulic_retries=rejections, next_dt_s=self.step_hint,
+                    accepted_cfl_max=cfl_max)
+
+
+class RiverHLLC(HLLCHydraulics):
+    """The river controller (river_hllc.py:11-93) over the device owner."""
+
+    def advance(self, dt, time=0.):
+        info = super().advance(dt, time)
+        return dict(integrator=RIVER_METHOD, hydraulic_substeps=info['hydraulic_substeps'],
+                    hydraulic_rejections=info['hydraulic_retries'], accepted_cfl_max=info['accepted_cfl_max'],
+                    hydraulic_next_substep_hint_s=self.step_hint, explicit_fallback=False)
+
+
+def install(solver: HydraulicSolver, cfl=.9, **kw):
+    """river_hllc.install: the HLLC controller on a fresh solver."""
+    if solver.time:
+        raise ValueError('Fresh hydraulic owner required')
+    solver.midpoint = RiverHLLC(solver, cfl, **kw)
+    return solver
+
+
+__all__ = ['HLLCHydraulics', 'RiverHLLC', 'install', 'metric_inverse', 'TrialRejected']

@@ -196,6 +196,57 @@ class HydraulicSolver:
             raise TrialRejected(_lib.last_error())
         return acc.value, rej.value
 
+    # -- HLLC MUSCL-Hancock (FP64) --------------------------------------------
+    def hllc_configure(self, inverse, cfl=.9, drag_mode=2, roughness=None, pressure=-1):
+        """Hancock owner with face-sum CFL limit cfl, the Strang drag (as
+        river_configure), the inverse LS metric [3, nc] and the low-Froude face
+        correction (1 on, 0 off, -1 with an installed rotor array)."""
+        r = None if roughness is None else np.ascontiguousarray(roughness, dtype=np.float64).reshape(self.nc)
+        inv = np.ascontiguousarray(inverse, dtype=np.float64).reshape(3, self.nc)
+        _lib.check(self.L.braids_hllc_configure(self.h, float(cfl), int(drag_mode), _p(r), _p(inv), int(pressure)),
+                   'braids_hllc_configure')
+        self.hllc_cfl = float(cfl)
+
+    def hllc_step(self, dt, t=None):
+        """One Hancock substep; raises TrialRejected (state unchanged)."""
+        r = _lib.BraidsRiverResult()
+        rc = _lib.check(self.L.braids_hllc_step(self.h, float(dt), float(self.time if t is None else t),
+                                                ctypes.byref(r)), 'braids_hllc_step')
+        info = dict(accepted=bool(r.accepted), reason=_lib.HLLC_REASONS.get(r.reason, r.reason),
+                    fail_cell=r.fail_cell, frequency=r.frequency_peak)
+        if rc == _lib.BRAIDS_REJECTED:
+            err = TrialRejected(f'HLLC substep rejected ({info["reason"]}, cell {r.fail_cell}, '
+                                f'CFL {dt * r.frequency_peak:g})')
+            err.info = info
+            raise err
+        return info
+
+    def hllc_advance(self, dt, time=None):
+        """HLLCHydraulics.advance over the outer interval dt; returns
+        (substeps, rejections, accepted CFL max)."""
+        t0 = self.time if time is None else float(time)
+        hint = ctypes.c_double(self.hllc_hint if getattr(self, 'hllc_hint', None) else 0.0)
+        n = ctypes.c_int(); rej = ctypes.c_int(); cmax = ctypes.c_double()
+        rc = _lib.check(self.L.braids_hllc_advance(self.h, float(dt), t0, ctypes.byref(hint), ctypes.byref(n),
+                                                   ctypes.byref(rej), ctypes.byref(cmax)), 'braids_hllc_advance')
+        self.hllc_hint = hint.value
+        if rc == _lib.BRAIDS_REJECTED:
+            raise TrialRejected(_lib.last_error())
+        self.time = t0 + float(dt)
+        return n.value, rej.value, cmax.value
+
+    def hllc_run(self, nsteps, hint=None):
+        """Exactly nsteps accepted Hancock substeps from self.time; returns (accepted, rejections)."""
+        t = ctypes.c_double(self.time)
+        h = ctypes.c_double(hint or getattr(self, 'hllc_hint', None) or 0.01)
+        acc = ctypes.c_int(); rej = ctypes.c_int()
+        rc = _lib.check(self.L.braids_hllc_run(self.h, int(nsteps), ctypes.byref(t), ctypes.byref(h),
+                                               ctypes.byref(acc), ctypes.byref(rej)), 'braids_hllc_run')
+        self.time, self.hllc_hint = t.value, h.value
+        if rc == _lib.BRAIDS_REJECTED:
+            raise TrialRejected(_lib.last_error())
+        return acc.value, rej.value
+
     # -- measurement --------------------------------------------------------
     def profile(self, kinds=True):
         """kinds: True (all kernel kinds), False (off) or an iterable of kind names."""

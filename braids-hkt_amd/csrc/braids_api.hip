@@ -9,6 +9,7 @@
 #include <algorithm>
 #include <string>
 #include <vector>
+#include <type_traits>
 
 #include "braids_hip.h"
 #include "swe_launch.h"
@@ -96,6 +97,9 @@ struct SolverBase {
     virtual int river_configure(double cfl, int drag_mode, const double *drag_values) = 0;
     virtual int river_step(double dt, double t, braids_river_result *res) = 0;
     virtual int turbines_set(const braids_turbines *t) = 0;
+    virtual int hllc_configure(double cfl, int drag_mode, const double *drag_values, const double *inverse,
+                               int pressure) = 0;
+    virtual int hllc_step(double dt, double t, braids_river_result *res) = 0;
     virtual int exner_configure(const braids_exner_options *o) = 0;
     virtual int exner_finish(double dt, braids_exner_result *r) = 0;
     virtual int snapshot(int restore) = 0;
@@ -103,6 +107,7 @@ struct SolverBase {
     hipStream_t stream = nullptr;
     // river path
     double river_cfl = 0.5;
+    double hllc_cfl = 0.9;
     std::vector<double> manning_host, xy_host, length_host, normal_host;
     int nrot = 0;
     std::vector<double> energy, energy_comp;   // per rotor, compensated (Neumaier)
@@ -132,14 +137,18 @@ struct SolverBase {
         set_events(on ? events.data() : nullptr, prof_mask);
         return BRAIDS_OK;
     }
-    void collect(int slots, bool river = false)
+    // path: 0 SSPRK(3,3), 1 river SSPRK(2,2), 2 HLLC Hancock
+    void collect(int slots, int path = 0)
     {
         if (events.empty()) return;
-        static const bool ssp[KK_COUNT] = {true, true, true, true, true, true, false, false, false, false};
-        static const bool riv[KK_COUNT] = {true, true, true, false, false, false, true, true, false, true};
+        static const bool kinds[3][KK_COUNT] = {
+            {true, true, true, true, true, true, false, false, false, false, false, false, false, false},
+            {true, true, true, false, false, false, true, true, false, true, false, false, false, false},
+            {false, true, false, false, false, false, true, false, false, false, true, true, true, true}};
+        const bool river = path == 1;
         for (int slot = 0; slot < slots; ++slot)
             for (int k = 0; k < KK_COUNT; ++k) {
-                if (!(river ? riv[k] : ssp[k])) continue;
+                if (!kinds[path][k]) continue;
                 if (river && k == KK_RDRAG && slot > 0) continue;
                 if (river && k == KK_TURBINE && nrot == 0) continue;
                 if ((k == KK_TRACES || k == KK_LEDGER) && nb == 0) continue;
@@ -172,6 +181,10 @@ struct Solver : SolverBase {
     double *zsnap = nullptr;             // bed snapshot (paired with st[4])
     double *ex_host = nullptr;           // pinned [8] result readback
     double *diag_host = nullptr;       // pinned [2][nrot][TURBINE_DIAG]
+    HllcView hv{};                     // HLLC Hancock substep (FP64 solvers)
+    int hllc_pressure = -1;            // -1: the low-Froude correction follows the rotor array
+    std::vector<double> mid_host;
+    std::vector<int> signs_host, bid_host, neighbour_host;
     void set_events(hipEvent_t *ev, int mask) override { cfg.ev = ev; cfg.ev_mask = mask; profiling = ev != nullptr; }
 
     ~Solver() override
@@ -287,6 +300,10 @@ struct Solver : SolverBase {
         xy_host.assign(mesh->xy, mesh->xy + 2 * (size_t)nc);
         length_host.assign(mesh->length, mesh->length + nf);
         normal_host.assign(mesh->normal, mesh->normal + 2 * (size_t)nf);
+        mid_host.assign(mesh->mid, mesh->mid + 2 * (size_t)nf);
+        signs_host.assign(mesh->signs, mesh->signs + nadj);
+        bid_host.assign(mesh->bid, mesh->bid + nf);
+        neighbour_host.assign(mesh->neighbour, mesh->neighbour + nf);
         for (auto &s : st) {
             if ((rc = alloc(s.h, nc))) return rc;
             if ((rc = alloc(s.hu, nc))) return rc;
@@ -404,6 +421,10 @@ struct Solver : SolverBase {
     int river_configure(double cfl, int drag_mode, const double *drag_values) override
     {
         river_cfl = cfl;
+        return drag_configure(drag_mode, drag_values);
+    }
+    int drag_configure(int drag_mode, const double *drag_values)
+    {
         int rc = 0;
         if (!rb.theta && (rc = river_buffers())) return rc;
         double *v = const_cast<double *>(rb.drag.values);
@@ -506,7 +527,7 @@ struct Solver : SolverBase {
             HIP_TRY(hipMemcpyAsync(diag_host, tv.diag, 2 * (size_t)nrot * TURBINE_DIAG * sizeof(double),
                                    hipMemcpyDeviceToHost, stream));
         HIP_TRY(hipStreamSynchronize(stream));
-        if (profiling) collect(2, true);
+        if (profiling) collect(2, 1);
         const Control *hc = (const Control *)host_block;
         const double *hl = (const double *)((const char *)host_block + sizeof(Control));
         res->accepted = 0; res->reason = 0; res->failed_stage = -1; res->fail_cell = -1; res->frequency_peak = 0;
@@ -551,6 +572,105 @@ struct Solver : SolverBase {
         }
         res->accepted = 1;
         return BRAIDS_OK;
+    }
+
+    // -- HLLC MUSCL-Hancock (FP64) --------------------------------------------
+    int hllc_configure(double cfl, int drag_mode, const double *drag_values, const double *inverse,
+                       int pressure) override
+    {
+        if constexpr (!std::is_same<R, double>::value) {
+            (void)cfl; (void)drag_mode; (void)drag_values; (void)inverse; (void)pressure;
+            set_error("the HLLC path runs on FP64 solvers (precision 64)");
+            return BRAIDS_EINVAL;
+        } else {
+            int rc = 0;
+            if (!rb.theta && (rc = river_buffers())) return rc;
+            if ((rc = drag_configure(drag_mode, drag_values))) return rc;
+            hllc_cfl = cfl;
+            hllc_pressure = pressure;
+            if (!hv.grad) {
+                HllcGeo g{};
+                g.n = nc; g.nf = nf;
+                std::vector<int> sel(nf);
+                for (int f = 0; f < nf; ++f) sel[f] = neighbour_host[f] < 0 && bid_host[f] >= 0;
+                if ((rc = upload(g.xy, xy_host))) return rc;
+                if ((rc = upload(g.mid, mid_host))) return rc;
+                if ((rc = upload(g.selected, sel))) return rc;
+                if ((rc = upload(g.signs, signs_host))) return rc;
+                double *inv = nullptr;
+                if ((rc = alloc(inv, 3 * (size_t)nc))) return rc;
+                g.inverse = inv;
+                g.normal = m.normal; g.length = m.length; g.area = m.area;
+                g.owner = m.owner; g.neighbour = m.neighbour; g.off = m.offset; g.faces = m.adj_face;
+                hv.g = g;
+                if ((rc = alloc(hv.grad, 8 * (size_t)nc))) return rc;
+                if ((rc = alloc(hv.half, 4 * (size_t)nc))) return rc;
+                if ((rc = alloc(hv.pstat, nc))) return rc;
+                hv.F = w.F; hv.theta = rb.theta;
+            }
+            HIP_TRY(hipMemcpy(const_cast<double *>(hv.g.inverse), inverse, 3 * (size_t)nc * sizeof(double),
+                              hipMemcpyHostToDevice));
+            return BRAIDS_OK;
+        }
+    }
+    int hllc_step(double dt, double t, braids_river_result *res) override
+    {
+        if constexpr (!std::is_same<R, double>::value) {
+            (void)dt; (void)t; (void)res;
+            set_error("the HLLC path runs on FP64 solvers (precision 64)");
+            return BRAIDS_EINVAL;
+        } else {
+            if (!hv.grad) { set_error("HLLC is not configured (braids_hllc_configure)"); return BRAIDS_EINVAL; }
+            hv.momentum = nrot > 0;
+            hv.pressure = hllc_pressure < 0 ? nrot > 0 : hllc_pressure != 0;
+            zero_control();
+            run_hllc_substep<R>(m, st[0], st[1], st[2], st[3], z, w, ctl, rb, nrot > 0 ? &tv : nullptr, hv, dt, t,
+                                cfg, stream);
+            HIP_TRY(hipGetLastError());
+            HIP_TRY(hipMemcpyAsync(host_block, ctl, head_bytes, hipMemcpyDeviceToHost, stream));
+            if (nrot > 0)
+                HIP_TRY(hipMemcpyAsync(diag_host, tv.diag, 2 * (size_t)nrot * TURBINE_DIAG * sizeof(double),
+                                       hipMemcpyDeviceToHost, stream));
+            HIP_TRY(hipStreamSynchronize(stream));
+            if (profiling) collect(1, 2);
+            const Control *hc = (const Control *)host_block;
+            const double *hl = (const double *)((const char *)host_block + sizeof(Control));
+            res->accepted = 0; res->reason = 0; res->failed_stage = -1; res->fail_cell = -1; res->frequency_peak = 0;
+            const char *domain = "Thetis induction correction is outside its real domain: Ct*A exceeds rotor "
+                                 "diameter times local water depth";
+            // the order in which HLLCHydraulics._spatial_step raises
+            if (hc->turbine_error & 1) { set_error(domain); res->reason = BRAIDS_HLLC_TURBINE; return BRAIDS_EINVAL; }
+            const int code = hc->failure[0];
+            if (code >= FAIL_TRACE) {
+                res->reason = BRAIDS_HLLC_EXTERIOR; res->failed_stage = 0;
+                return BRAIDS_REJECTED;
+            }
+            double peak; unsigned long long b = hc->peak_bits[0];
+            memcpy(&peak, &b, 8);
+            res->frequency_peak = peak;
+            if (!std::isfinite(peak) || dt * peak > hllc_cfl * (1 + 1e-12)) res->reason = BRAIDS_HLLC_CFL;
+            else if (code == HFAIL_POSITIVE) res->reason = BRAIDS_HLLC_POSITIVITY;
+            else if (hc->turbine_error & 2) { set_error(domain); res->reason = BRAIDS_HLLC_TURBINE; return BRAIDS_EINVAL; }
+            else if (code == HFAIL_MOMENTUM) res->reason = BRAIDS_HLLC_MOMENTUM;
+            if (res->reason) {
+                res->failed_stage = 0;
+                res->fail_cell = res->reason == BRAIDS_HLLC_CFL ? -1 : hc->fail_cell[0] - 1;
+                return BRAIDS_REJECTED;
+            }
+            std::swap(st[0], st[3]);
+            for (int bb = 0; bb < nb; ++bb) ledger[bb] = ledger[bb] + hl[bb];
+            if (nrot > 0) {
+                std::copy(diag_host, diag_host + 2 * (size_t)nrot * TURBINE_DIAG, diag_last.begin());
+                for (int a = 0; a < nrot; ++a) {
+                    const double p1 = diag_host[((size_t)nrot + a) * TURBINE_DIAG + 8];
+                    const double v = dt * p1, sum = energy[a] + v;
+                    energy_comp[a] += std::fabs(energy[a]) >= std::fabs(v) ? (energy[a] - sum) + v : (v - sum) + energy[a];
+                    energy[a] = sum;
+                }
+            }
+            res->accepted = 1;
+            return BRAIDS_OK;
+        }
     }
 
     // -- Exner bed update ----------------------------------------------------
@@ -1121,6 +1241,98 @@ extern "C" int braids_river_run(braids_solver *s, int nsteps, double *time, doub
         *time = *time + trial;
         ++*accepted;
         hint = peak > 0 ? std::fmin(2 * trial, safety * x->river_cfl / peak) : 2 * trial;
+    }
+    *step_hint = hint;
+    if (rejections) *rejections = rejected;
+    return BRAIDS_OK;
+}
+
+// -- HLLC MUSCL-Hancock ------------------------------------------------------
+extern "C" int braids_hllc_configure(braids_solver *s, double cfl, int drag_mode, const double *drag_values,
+                                     const double *inverse, int pressure)
+{
+    if (!s || !(cfl > 0) || cfl > 1 || drag_mode < 0 || drag_mode > 4 || !inverse || pressure < -1 || pressure > 1) {
+        set_error("invalid HLLC options"); return BRAIDS_EINVAL;
+    }
+    return s->impl->hllc_configure(cfl, drag_mode, drag_values, inverse, pressure);
+}
+
+extern "C" int braids_hllc_step(braids_solver *s, double dt, double t, braids_river_result *res)
+{
+    if (!s || !res || !(dt > 0) || !std::isfinite(dt)) { set_error("invalid step arguments"); return BRAIDS_EINVAL; }
+    return s->impl->hllc_step(dt, t, res);
+}
+
+extern "C" int braids_hllc_advance(braids_solver *s, double dt, double time, double *step_hint, int *substeps,
+                                   int *rejections, double *accepted_cfl_max)
+{
+    if (!s || !step_hint || !substeps || !(dt > 0) || !std::isfinite(dt)) {
+        set_error("invalid advance arguments"); return BRAIDS_EINVAL;
+    }
+    SolverBase *x = s->impl;
+    const double floor_ = dt * std::ldexp(1.0, -20), safety = .98;
+    double elapsed = 0., hint = *step_hint > 0 ? *step_hint : dt, cflmax = 0.;
+    int used = 0, rejected = 0;
+    *substeps = 0;
+    if (rejections) *rejections = 0;
+    while (!used || elapsed < dt - 1e-13 * std::fmax(1., dt)) {
+        const double trial = std::fmin(hint, dt - elapsed);
+        const bool truncated = trial < hint * (1 - 1e-12);
+        braids_river_result r{};
+        int rc = x->hllc_step(trial, time + elapsed, &r);
+        if (rc < 0) { *step_hint = hint; return rc; }
+        const double peak = r.frequency_peak;
+        if (rc == BRAIDS_REJECTED) {
+            ++rejected;
+            if (rejections) *rejections = rejected;
+            hint = peak > 0 ? std::fmin(.5 * trial, safety * x->hllc_cfl / peak) : .5 * trial;
+            if (rejected > 100 || hint < floor_) {
+                *step_hint = hint;
+                set_error("HLLC substep rejected: more than 100 rejections or a step below dt/2^20");
+                return BRAIDS_REJECTED;
+            }
+            continue;
+        }
+        elapsed += trial; ++used;
+        cflmax = std::fmax(cflmax, trial * peak);
+        const double growth = truncated ? hint : 2 * trial;
+        hint = peak > 0 ? std::fmin(growth, safety * x->hllc_cfl / peak) : growth;
+        *substeps = used;
+    }
+    *step_hint = hint;
+    if (accepted_cfl_max) *accepted_cfl_max = cflmax;
+    return BRAIDS_OK;
+}
+
+extern "C" int braids_hllc_run(braids_solver *s, int nsteps, double *time, double *step_hint, int *accepted,
+                               int *rejections)
+{
+    if (!s || !time || !step_hint || !accepted || nsteps < 0 || !(*step_hint > 0)) {
+        set_error("invalid run arguments"); return BRAIDS_EINVAL;
+    }
+    SolverBase *x = s->impl;
+    const double safety = .98;
+    double hint = *step_hint;
+    int rejected = 0;
+    *accepted = 0;
+    while (*accepted < nsteps) {
+        const double trial = hint;
+        braids_river_result r{};
+        int rc = x->hllc_step(trial, *time, &r);
+        if (rc < 0) { *step_hint = hint; if (rejections) *rejections = rejected; return rc; }
+        const double peak = r.frequency_peak;
+        if (rc == BRAIDS_REJECTED) {
+            ++rejected;
+            hint = peak > 0 ? std::fmin(.5 * trial, safety * x->hllc_cfl / peak) : .5 * trial;
+            if (rejected > 100 * (*accepted + 1) || !(hint > 0)) {
+                *step_hint = hint; if (rejections) *rejections = rejected;
+                set_error("HLLC substeps keep being rejected"); return BRAIDS_REJECTED;
+            }
+            continue;
+        }
+        *time = *time + trial;
+        ++*accepted;
+        hint = peak > 0 ? std::fmin(2 * trial, safety * x->hllc_cfl / peak) : 2 * trial;
     }
     *step_hint = hint;
     if (rejections) *rejections = rejected;

@@ -6,11 +6,17 @@
 //   midpoint_inventory/faces/gather          river_midpoint.h:13-41
 //   river_cell_drag (+ river_chezy)          river_hydraulic_drag.h:3-16,
 //                                            river_roughness.py:42-47
+//   hllc_reconstruct, hllc_predict_source,   src/braids_core/hydraulics/hllc.h
+//   hllc_faces, hllc_budget, hllc_limit,
+//   hllc_update
+//   hllc_turbine_pressure                    hllc_turbines.h:13-30
+//   hllc_advanced_frequency                  hllc_advanced.h:13-22
 // with device pointers, launched on the stream set by braids_engine_set_stream.
 // FP64; built without FMA contraction.
 #include <hip/hip_runtime.h>
 #include "braids_hip.h"
 #include "braids_engine.h"
+#include "hllc_ops.h"
 
 namespace {
 
@@ -120,6 +126,144 @@ __global__ void k_river_cell_drag(int n, double dt, double gravity, double dry, 
     q[n + i] *= factor; q[2 * n + i] *= factor;
 }
 
+// ---------------------------------------------------------------------------
+// HLLC operators: one work-item per cell or face, bodies in hllc_ops.h.
+
+using braids::hl::Geo;
+
+__global__ void k_hllc_reconstruct(Geo g, double gravity, double dry, double dt, const double *q, const double *z,
+                                   double *grad, double *half, double *qhalf, int *status)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    const int n = g.n;
+    if (i >= n) return;
+    double G[8], eta, uh, wh;
+    braids::hl::reconstruct(g, i, gravity, dry, dt, q, q + n, q + 2 * n, z, G, eta, uh, wh);
+    for (int k = 0; k < 8; ++k) grad[k * n + i] = G[k];
+    const double zc = z[i];
+    half[i] = eta; half[n + i] = uh; half[2 * n + i] = wh; half[3 * n + i] = zc;
+    const double h = fmax(0., eta - zc);
+    qhalf[i] = h; qhalf[n + i] = h * uh; qhalf[2 * n + i] = h * wh;
+    status[i] = !isfinite(eta) || !isfinite(uh) || !isfinite(wh);
+}
+
+__global__ void k_hllc_predict_source(int n, double dry, double dt, const double *rate, double *half, double *qhalf)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const double h = qhalf[i];
+    qhalf[n + i] += .5 * dt * rate[n + i];
+    qhalf[2 * n + i] += .5 * dt * rate[2 * n + i];
+    half[n + i] = h > dry ? qhalf[n + i] / h : 0.;
+    half[2 * n + i] = h > dry ? qhalf[2 * n + i] / h : 0.;
+}
+
+__global__ void k_hllc_faces(Geo g, double gravity, double dry, const double *half, const double *grad,
+                             const double *external, const double *trace, double *fl, double *fr, double *speed)
+{
+    const int f = blockIdx.x * blockDim.x + threadIdx.x;
+    const int nf = g.nf;
+    if (f >= nf) return;
+    double a[3], b[3], s;
+    if (g.selected[f]) {
+        const double e[3] = {external[f], external[nf + f], external[2 * nf + f]};
+        const double t[3] = {trace[f], trace[nf + f], trace[2 * nf + f]};
+        braids::hl::face_selected(g, f, gravity, dry, half, e, t, a, s);
+        b[0] = a[0]; b[1] = a[1]; b[2] = a[2];
+    } else {
+        braids::hl::face_interior(g, f, gravity, dry, half, grad, a, b, s);
+    }
+    for (int m = 0; m < 3; ++m) { fl[m * nf + f] = a[m]; fr[m * nf + f] = b[m]; }
+    speed[f] = s;
+}
+
+__global__ void k_hllc_turbine_pressure(Geo g, double gravity, double dry, const double *half, const double *grad,
+                                        double *fl, double *fr)
+{
+    const int f = blockIdx.x * blockDim.x + threadIdx.x;
+    const int nf = g.nf;
+    if (f >= nf) return;
+    double a[3] = {fl[f], fl[nf + f], fl[2 * nf + f]}, b[3] = {fr[f], fr[nf + f], fr[2 * nf + f]};
+    braids::hl::turbine_pressure(g, f, gravity, dry, half, grad, a, b);
+    fl[nf + f] = a[1]; fl[2 * nf + f] = a[2]; fr[nf + f] = b[1]; fr[2 * nf + f] = b[2];
+}
+
+__global__ void k_hllc_budget(Geo g, double dt, const double *q, const double *fl, const double *speed,
+                              const int *status, double *theta, double *frequency)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= g.n) return;
+    double th, fr;
+    braids::hl::budget(g, i, dt, q[i], fl, speed, status[i], th, fr);
+    theta[i] = th; frequency[i] = fr;
+}
+
+__global__ void k_hllc_limit(Geo g, double dry, const double *q, const double *theta, double *fl, double *fr,
+                             int *status)
+{
+    const int f = blockIdx.x * blockDim.x + threadIdx.x;
+    const int n = g.n, nf = g.nf;
+    if (f >= nf) return;
+    double mass, d1, d2;
+    bool changed, corr;
+    status[f] = braids::hl::limit(g, f, dry, q, q + n, q + 2 * n, theta, fl[f], mass, d1, d2, changed, corr);
+    if (!changed) return;
+    fl[f] = fr[f] = mass;
+    if (corr) {
+        fl[nf + f] += d1; fr[nf + f] += d1;
+        fl[2 * nf + f] += d2; fr[2 * nf + f] += d2;
+    }
+}
+
+__global__ void k_hllc_update(Geo g, double dt, double dry, const double *q, const double *fl, const double *fr,
+                              const int *face_status, double *next, double *status)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    const int n = g.n, nf = g.nf;
+    if (i >= n) return;
+    double v[3] = {q[i], q[n + i], q[2 * n + i]};
+    bool bad = false;
+    for (int k = g.off[i]; k < g.off[i + 1]; ++k) {
+        const int f = g.faces[k], s = g.signs[k];
+        const double a = dt * s * g.length[f] / g.area[i];
+        const double *F = s > 0 ? fl : fr;
+        for (int m = 0; m < 3; ++m) v[m] -= a * F[m * nf + f];
+        bad = bad || face_status[f];
+    }
+    if (v[0] <= dry) { v[1] = 0.; v[2] = 0.; }
+    for (int m = 0; m < 3; ++m) {
+        next[m * n + i] = v[m];
+        bad = bad || !isfinite(v[m]);
+    }
+    status[i] = bad || v[0] < 0. ? 1. : 0.;
+}
+
+__global__ void k_hllc_advanced_frequency(int n, double dry, double nu, const double *q, const double *metric,
+                                          const int *owner, const int *nb, const int *off, const int *faces,
+                                          const double *length, const double *area, double *out)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    double sum = 0.;
+    if (q[i] > dry)
+        for (int k = off[i]; k < off[i + 1]; ++k) {
+            const int f = faces[k], j = nb[f], o = owner[f];
+            if (j >= 0 && q[o] > dry && q[j] > dry) sum += nu * length[f] * metric[f];
+        }
+    out[i] = sum / area[i];
+}
+
+Geo geo(int n, int nf, const double *xy, const double *mid, const double *normal, const int *owner,
+        const int *neighbour, const int *selected, const int *off, const int *faces, const int *signs,
+        const double *length, const double *area, const double *inverse)
+{
+    Geo g{};
+    g.n = n; g.nf = nf; g.xy = xy; g.mid = mid; g.normal = normal; g.owner = owner; g.neighbour = neighbour;
+    g.selected = selected; g.off = off; g.faces = faces; g.signs = signs; g.length = length; g.area = area;
+    g.inverse = inverse;
+    return g;
+}
+
 inline dim3 grid(int n) { return dim3(n > 0 ? (n + 255) / 256 : 1); }
 
 int finish()
@@ -191,5 +335,97 @@ extern "C" int river_cell_drag_spatial(int n, double dt, double gravity, double 
     if (n <= 0) return BRAIDS_OK;
     hipLaunchKernelGGL(k_river_cell_drag, grid(n), dim3(256), 0, g_stream, n, dt, gravity, dry, mode, 0.0, values,
                        feedback, components, q, chezy);
+    return finish();
+}
+
+// -- HLLC (src/braids_core/hydraulics/hllc.h, hllc_turbines.h, hllc_advanced.h) --
+
+extern "C" int hllc_reconstruct(int n, int nf, double gravity, double dry, double dt, const double *q, const double *z,
+                                const double *xy, const double *mid, const double *normal, const int *owner,
+                                const int *neighbour, const int *selected, const int *off, const int *faces,
+                                const double *inverse, double *grad, double *half, double *qhalf, int *status)
+{
+    if (n <= 0) return BRAIDS_OK;
+    const Geo g = geo(n, nf, xy, mid, normal, owner, neighbour, selected, off, faces, nullptr, nullptr, nullptr,
+                      inverse);
+    hipLaunchKernelGGL(k_hllc_reconstruct, grid(n), dim3(256), 0, g_stream, g, gravity, dry, dt, q, z, grad, half,
+                       qhalf, status);
+    return finish();
+}
+
+extern "C" int hllc_predict_source(int n, double dry, double dt, const double *rate, double *half, double *qhalf)
+{
+    if (n <= 0) return BRAIDS_OK;
+    hipLaunchKernelGGL(k_hllc_predict_source, grid(n), dim3(256), 0, g_stream, n, dry, dt, rate, half, qhalf);
+    return finish();
+}
+
+extern "C" int hllc_faces(int n, int nf, double gravity, double dry, const double *half, const double *grad,
+                          const double *xy, const double *mid, const double *normal, const int *owner,
+                          const int *neighbour, const int *selected, const double *external,
+                          const double *external_trace, double *fl, double *fr, double *speed)
+{
+    if (nf <= 0) return BRAIDS_OK;
+    const Geo g = geo(n, nf, xy, mid, normal, owner, neighbour, selected, nullptr, nullptr, nullptr, nullptr,
+                      nullptr, nullptr);
+    hipLaunchKernelGGL(k_hllc_faces, grid(nf), dim3(256), 0, g_stream, g, gravity, dry, half, grad, external,
+                       external_trace, fl, fr, speed);
+    return finish();
+}
+
+extern "C" int hllc_budget(int n, int nf, double dt, const double *q, const double *area, const double *length,
+                           const int *off, const int *faces, const int *signs, const double *fl, const double *speed,
+                           const int *selected, const int *status, double *theta, double *frequency)
+{
+    if (n <= 0) return BRAIDS_OK;
+    const Geo g = geo(n, nf, nullptr, nullptr, nullptr, nullptr, nullptr, selected, off, faces, signs, length, area,
+                      nullptr);
+    hipLaunchKernelGGL(k_hllc_budget, grid(n), dim3(256), 0, g_stream, g, dt, q, fl, speed, status, theta,
+                       frequency);
+    return finish();
+}
+
+extern "C" int hllc_limit(int n, int nf, double dry, const double *q, const double *theta, const int *owner,
+                          const int *neighbour, const int *selected, double *fl, double *fr, int *status)
+{
+    if (nf <= 0) return BRAIDS_OK;
+    const Geo g = geo(n, nf, nullptr, nullptr, nullptr, owner, neighbour, selected, nullptr, nullptr, nullptr,
+                      nullptr, nullptr, nullptr);
+    hipLaunchKernelGGL(k_hllc_limit, grid(nf), dim3(256), 0, g_stream, g, dry, q, theta, fl, fr, status);
+    return finish();
+}
+
+extern "C" int hllc_update(int n, int nf, double dt, double dry, const double *q, const double *area,
+                           const double *length, const int *off, const int *faces, const int *signs,
+                           const double *fl, const double *fr, const int *face_status, double *next, double *status)
+{
+    if (n <= 0) return BRAIDS_OK;
+    const Geo g = geo(n, nf, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, off, faces, signs, length, area,
+                      nullptr);
+    hipLaunchKernelGGL(k_hllc_update, grid(n), dim3(256), 0, g_stream, g, dt, dry, q, fl, fr, face_status, next,
+                       status);
+    return finish();
+}
+
+extern "C" int hllc_turbine_pressure(int n, int nf, double gravity, double dry, const double *half,
+                                     const double *grad, const double *xy, const double *mid, const double *normal,
+                                     const int *owner, const int *nb, double *fl, double *fr)
+{
+    if (nf <= 0) return BRAIDS_OK;
+    const Geo g = geo(n, nf, xy, mid, normal, owner, nb, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr,
+                      nullptr);
+    hipLaunchKernelGGL(k_hllc_turbine_pressure, grid(nf), dim3(256), 0, g_stream, g, gravity, dry, half, grad, fl,
+                       fr);
+    return finish();
+}
+
+extern "C" int hllc_advanced_frequency(int n, int nf, double dry, double nu, const double *q, const double *metric,
+                                       const int *owner, const int *nb, const int *off, const int *faces,
+                                       const double *length, const double *area, double *out)
+{
+    (void)nf;
+    if (n <= 0) return BRAIDS_OK;
+    hipLaunchKernelGGL(k_hllc_advanced_frequency, grid(n), dim3(256), 0, g_stream, n, dry, nu, q, metric, owner, nb,
+                       off, faces, length, area, out);
     return finish();
 }

@@ -1,0 +1,211 @@
+// HLLC MUSCL-Hancock substep on the device (HLLCHydraulics._spatial_step
+// with the Hancock predictor, src/braids_core/hydraulics/hllc.py:76-118),
+// FP64.  One substep of size dt is, with `work` the Strang-dragged base:
+//   k_river_drag    work = drag(base, dt/2)
+//   kh_reconstruct  cells: limited LS gradient, Hancock predictor -> half,
+//                   qhalf; extra workgroups: one wavefront per rotor samples
+//                   `work` (the predictor's momentum source)
+//   kh_predict      footprint cells: the rotor sink over dt/2 into qhalf and
+//                   the half-step velocities (hllc_predict_source)
+//   k_traces        open boundaries: first-order traces of qhalf at t + dt/2
+//   kh_faces        faces: exterior flux on open faces (HLLC between the
+//                   owner's predictor state and the trace), hydrostatic HLLC
+//                   elsewhere, plus the low-Froude correction of a rotor array
+//   kh_budget       cells: donor factor and face-sum frequency (max-reduced,
+//                   NaN kept); extra workgroups: the rotors sample qhalf
+//   kh_update       cells: hllc_limit applied per incident face on the fly,
+//                   hllc_update, the rotor sink at the half time, the closing
+//                   drag half-step
+//   k_river_ledger  inward water of the open boundaries
+// Rejections are flags in the Control block (slot 0), read once per substep.
+#pragma once
+#include "hllc_ops.h"
+#include "river_kernels.h"
+
+namespace braids {
+
+// Max of non-negative doubles (and NaN) as their bit patterns into the
+// workgroup's partial slot: NaN bits exceed +inf bits, which exceed every
+// finite value, so a NaN frequency anywhere wins as numpy.max lets it.
+__device__ __forceinline__ void block_bits_max_to(unsigned long long v, unsigned long long *slots)
+{
+    unsigned long long *target = slots + (blockIdx.x % RSLOTS) * RSTRIDE;
+    __shared__ unsigned long long part[16];
+    for (int o = 32; o > 0; o >>= 1) {
+        const unsigned long long x = (unsigned long long)__shfl_xor((long long)v, o, 64);
+        v = x > v ? x : v;
+    }
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    if (lane == 0) part[wave] = v;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        unsigned long long m = 0;
+        const int nw = (blockDim.x + 63) >> 6;
+        for (int i = 0; i < nw; ++i) m = part[i] > m ? part[i] : m;
+        if (m) atomicMax(target, m);
+    }
+}
+
+static __global__ void __launch_bounds__(256) kh_reconstruct(HllcView H, MeshView<double> m, StateView<double> work,
+                                                      const double *__restrict__ z, StateView<double> qh,
+                                                      TurbineView T, double gravity, double dry, double dt, int ncb,
+                                                      int remap)
+{
+    if ((int)blockIdx.x >= ncb) {
+        const int a = ((int)blockIdx.x - ncb) * (blockDim.x >> 6) + (threadIdx.x >> 6);
+        if (a < T.nrot) turbine_sample_wave<double>(T, m, work, z, 0, a, threadIdx.x & 63);
+        return;
+    }
+    const int lb = remap ? xcd_order(blockIdx.x, ncb) : (int)blockIdx.x;
+    const int i = lb * blockDim.x + threadIdx.x;
+    const int n = H.g.n;
+    if (i >= n) return;
+    double G[8], eta, uh, wh;
+    hl::reconstruct(H.g, i, gravity, dry, dt, work.h, work.hu, work.hv, z, G, eta, uh, wh);
+#pragma unroll
+    for (int k = 0; k < 8; ++k) H.grad[k * n + i] = G[k];
+    const double zc = z[i];
+    const double h = fmax(0., eta - zc);
+    double qu = h * uh, qv = h * wh, ur = uh, vr = wh;
+    if (H.momentum) {
+        // hllc_predict_source over every cell with a zero rate; the footprint
+        // cells add their rotor sink in kh_predict
+        qu = qu + .5 * dt * 0.;
+        qv = qv + .5 * dt * 0.;
+        ur = h > dry ? qu / h : 0.;
+        vr = h > dry ? qv / h : 0.;
+    }
+    H.half[i] = eta; H.half[n + i] = ur; H.half[2 * n + i] = vr; H.half[3 * n + i] = zc;
+    qh.h[i] = h; qh.hu[i] = qu; qh.hv[i] = qv;
+    H.pstat[i] = !isfinite(eta) || !isfinite(uh) || !isfinite(wh);
+}
+
+// hllc_predict_source on the footprint cells: the sink k |u| (u, v) of the
+// rotors sampled on `work`, over half the substep.
+static __global__ void __launch_bounds__(256) kh_predict(HllcView H, StateView<double> work, StateView<double> qh,
+                                                  TurbineView T, double dry, double dt)
+{
+    const int u = blockIdx.x * blockDim.x + threadIdx.x;
+    if (u >= T.nunion) return;
+    const int c = T.union_cell[u], n = H.g.n;
+    double sx, sy;
+    turbine_sink(T, u, work.h[c], work.hu[c], work.hv[c], sx, sy);
+    const double h = qh.h[c];
+    const double qu = qh.hu[c] + .5 * dt * (-sx), qv = qh.hv[c] + .5 * dt * (-sy);
+    qh.hu[c] = qu; qh.hv[c] = qv;
+    H.half[n + c] = h > dry ? qu / h : 0.;
+    H.half[2 * n + c] = h > dry ? qv / h : 0.;
+}
+
+// The exterior provider: HLLC flux between the owner's predictor state and
+// the open-boundary trace (both over the owner's bed), rotated to x/y.
+__device__ __forceinline__ void exterior_flux(const hl::Geo &g, int f, double gravity, double dry,
+                                              const StateView<double> &qh, const double tr[3], double e[3])
+{
+    const int o = g.owner[f];
+    const double nx = g.normal[f], ny = g.normal[g.nf + f];
+    const double hl_ = qh.h[o], hr = tr[0];
+    const double ul = hl_ > dry ? (qh.hu[o] * nx + qh.hv[o] * ny) / hl_ : 0.;
+    const double vl = hl_ > dry ? (-qh.hu[o] * ny + qh.hv[o] * nx) / hl_ : 0.;
+    const double ur = hr > dry ? (tr[1] * nx + tr[2] * ny) / hr : 0.;
+    const double vr = hr > dry ? (-tr[1] * ny + tr[2] * nx) / hr : 0.;
+    double out[3], s;
+    hl::flux(hl_, ul, vl, hr, ur, vr, gravity, dry, out, s);
+    e[0] = out[0];
+    e[1] = out[1] * nx - out[2] * ny;
+    e[2] = out[1] * ny + out[2] * nx;
+}
+
+static __global__ void __launch_bounds__(256) kh_faces(HllcView H, StateView<double> qh, const double *__restrict__ trace,
+                                                double gravity, double dry, int remap)
+{
+    const int f = logical_block(remap) * blockDim.x + threadIdx.x;
+    const int nf = H.g.nf;
+    if (f >= nf) return;
+    double a[3], b[3], s;
+    if (H.g.selected[f]) {
+        const double tr[3] = {trace[f], trace[nf + f], trace[2 * nf + f]};
+        double e[3];
+        exterior_flux(H.g, f, gravity, dry, qh, tr, e);
+        hl::face_selected(H.g, f, gravity, dry, H.half, e, tr, a, s);
+        b[1] = a[1]; b[2] = a[2];
+    } else {
+        hl::face_interior(H.g, f, gravity, dry, H.half, H.grad, a, b, s);
+        if (H.pressure) hl::turbine_pressure(H.g, f, gravity, dry, H.half, H.grad, a, b);
+    }
+    H.F[f] = a[0]; H.F[nf + f] = a[1]; H.F[2 * nf + f] = a[2];
+    H.F[3 * nf + f] = b[1]; H.F[4 * nf + f] = b[2]; H.F[5 * nf + f] = s;
+}
+
+static __global__ void __launch_bounds__(256) kh_budget(HllcView H, MeshView<double> m, StateView<double> work,
+                                                 StateView<double> qh, const double *__restrict__ z,
+                                                 unsigned long long *peak_slots, TurbineView T, double dt, int ncb,
+                                                 int remap)
+{
+    if ((int)blockIdx.x >= ncb) {
+        const int a = ((int)blockIdx.x - ncb) * (blockDim.x >> 6) + (threadIdx.x >> 6);
+        if (a < T.nrot) turbine_sample_wave<double>(T, m, qh, z, 1, a, threadIdx.x & 63);
+        return;
+    }
+    const int lb = remap ? xcd_order(blockIdx.x, ncb) : (int)blockIdx.x;
+    const int i = lb * blockDim.x + threadIdx.x;
+    unsigned long long bits = 0;
+    if (i < H.g.n) {
+        double th, fr;
+        hl::budget(H.g, i, dt, work.h[i], H.F, H.F + 5 * (size_t)H.g.nf, H.pstat[i], th, fr);
+        H.theta[i] = th;
+        bits = (unsigned long long)__double_as_longlong(fr);
+    }
+    block_bits_max_to(bits, peak_slots);
+}
+
+static __global__ void __launch_bounds__(256) kh_update(HllcView H, StateView<double> work, StateView<double> qh,
+                                                 StateView<double> out, TurbineView T, DragView d, Control *ctl,
+                                                 double gravity, double dry, double dt, int remap)
+{
+    const int i = logical_block(remap) * blockDim.x + threadIdx.x;
+    const hl::Geo &g = H.g;
+    const int nf = g.nf;
+    int fail = 0;
+    if (i < g.n) {
+        double v0 = work.h[i], v1 = work.hu[i], v2 = work.hv[i];
+        bool bad = false;
+        const double A = g.area[i];
+        for (int k = g.off[i]; k < g.off[i + 1]; ++k) {
+            const int f = g.faces[k], s = g.signs[k];
+            double mass, d1, d2;
+            bool changed, corr;
+            const int st = hl::limit(g, f, dry, work.h, work.hu, work.hv, H.theta, H.F[f], mass, d1, d2, changed, corr);
+            double F0 = H.F[f];
+            double F1 = s > 0 ? H.F[nf + f] : H.F[3 * nf + f];
+            double F2 = s > 0 ? H.F[2 * nf + f] : H.F[4 * nf + f];
+            if (changed) {
+                F0 = mass;
+                if (corr) { F1 += d1; F2 += d2; }
+            }
+            const double a = dt * s * g.length[f] / A;
+            v0 -= a * F0; v1 -= a * F1; v2 -= a * F2;
+            bad = bad || st;
+        }
+        if (v0 <= dry) { v1 = 0.; v2 = 0.; }
+        bad = bad || !isfinite(v0) || !isfinite(v1) || !isfinite(v2);
+        if (bad || v0 < 0.) {
+            fail = HFAIL_POSITIVE;
+        } else if (H.momentum) {
+            // result += dt * rate, the rate of the rotors sampled on qhalf
+            double r1 = 0., r2 = 0.;
+            if (T.tmask[i]) {
+                double sx, sy;
+                turbine_sink(T, T.tuidx[i], qh.h[i], qh.hu[i], qh.hv[i], sx, sy);
+                r1 = -sx; r2 = -sy;
+            }
+            v0 = v0 + dt * 0.; v1 = v1 + dt * r1; v2 = v2 + dt * r2;
+            if (!isfinite(v0) || !isfinite(v1) || !isfinite(v2)) fail = HFAIL_MOMENTUM;
+        }
+        cell_drag(v0, v1, v2, d.mode > 0 ? d.values[i] : 0., d.mode, .5 * dt, gravity, dry);
+        out.h[i] = v0; out.hu[i] = v1; out.hv[i] = v2;
+    }
+    if (fail) { atomicMax(&ctl->failure[0], fail); atomicMax(&ctl->fail_cell[0], i + 1); }
+}
+
+}  // namespace braids

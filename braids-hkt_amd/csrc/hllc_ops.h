@@ -1,0 +1,286 @@
+// Per-cell and per-face bodies of the HLLC MUSCL-Hancock polygon operators
+// (src/braids_core/hydraulics/hllc.h, hllc_turbines.h), FP64, shared by the
+// reference-named operator entries (engine.hip) and the fused Hancock
+// substep (hllc_kernels.h).  Geometry is the reference owner's operator
+// layout (hllc.py:23-45): component-major SoA, per-cell incident faces in
+// ascending face order with sign +1 for the owner.  Every expression keeps
+// the reference's operation order; the translation units that use these are
+// compiled without FMA contraction, so results are bit-identical to the
+// reference's host build.
+#pragma once
+#include <hip/hip_runtime.h>
+#include "swe_types.h"
+
+namespace braids {
+namespace hl {
+
+using Geo = HllcGeo;
+
+// (eta, u, v, z) of cell i; velocities vanish on dry cells.
+__device__ __forceinline__ void cell_values(int i, double dry, const double *h, const double *hu, const double *hv,
+                                            const double *z, double v[4])
+{
+    const double d = h[i];
+    v[0] = d + z[i];
+    v[1] = d > dry ? hu[i] / d : 0.;
+    v[2] = d > dry ? hv[i] / d : 0.;
+    v[3] = z[i];
+}
+
+// hllc_reconstruct for one cell (hllc.h:18-42): limited LS gradient G of
+// (eta, u, v, z) and the Hancock half-step values (eta, u, v).
+__device__ __forceinline__ void reconstruct(const Geo &g, int i, double gravity, double dry, double dt,
+                                            const double *h, const double *hu, const double *hv, const double *z,
+                                            double G[8], double &eta, double &uh, double &wh)
+{
+    const int n = g.n, nf = g.nf;
+    double c[4], sx[4] = {0., 0., 0., 0.}, sy[4] = {0., 0., 0., 0.}, lo[4], hi[4];
+    cell_values(i, dry, h, hu, hv, z, c);
+#pragma unroll
+    for (int m = 0; m < 4; ++m) { lo[m] = c[m]; hi[m] = c[m]; }
+    const double xi = g.xy[i], yi = g.xy[n + i];
+    const int k0 = g.off[i], k1 = g.off[i + 1];
+    for (int k = k0; k < k1; ++k) {
+        const int f = g.faces[k];
+        const int j = g.owner[f] == i ? g.neighbour[f] : g.owner[f];
+        double e[4], dx, dy;
+        if (j >= 0) {
+            dx = g.xy[j] - xi;
+            dy = g.xy[n + j] - yi;
+            cell_values(j, dry, h, hu, hv, z, e);
+            if (e[0] - e[3] <= dry && e[3] >= c[0]) e[0] = c[0];     // dry bank above the surface
+        } else {
+            if (g.selected[f]) continue;
+            const double nx = g.normal[f], ny = g.normal[nf + f];
+            const double dist = (g.mid[f] - xi) * nx + (g.mid[nf + f] - yi) * ny;
+            dx = 2. * dist * nx;
+            dy = 2. * dist * ny;
+#pragma unroll
+            for (int m = 0; m < 4; ++m) e[m] = c[m];
+            const double un = c[1] * nx + c[2] * ny;
+            e[1] -= 2. * un * nx;
+            e[2] -= 2. * un * ny;
+        }
+        const double d2 = dx * dx + dy * dy;
+        if (d2 <= 0.) continue;
+#pragma unroll
+        for (int m = 0; m < 4; ++m) {
+            const double dv = e[m] - c[m];
+            sx[m] += dv * dx / d2;
+            sy[m] += dv * dy / d2;
+            lo[m] = fmin(lo[m], e[m]);
+            hi[m] = fmax(hi[m], e[m]);
+        }
+    }
+    const double a = g.inverse[i], b = g.inverse[n + i], d = g.inverse[2 * n + i];
+#pragma unroll
+    for (int m = 0; m < 4; ++m) {
+        const double gx = a * sx[m] + b * sy[m];
+        const double gy = b * sx[m] + d * sy[m];
+        double lim = 1.;
+        for (int k = k0; k < k1; ++k) {
+            const int f = g.faces[k];
+            const double rise = gx * (g.mid[f] - xi) + gy * (g.mid[nf + f] - yi);
+            if (rise > 0.) lim = fmin(lim, (hi[m] - c[m]) / rise);
+            else if (rise < 0.) lim = fmin(lim, (lo[m] - c[m]) / rise);
+        }
+        G[2 * m] = lim * gx;
+        G[2 * m + 1] = lim * gy;
+    }
+    const double depth = c[0] - c[3];
+    if (depth <= dry) {
+#pragma unroll
+        for (int k = 0; k < 6; ++k) G[k] = 0.;
+    }
+    const double u = c[1], w = c[2];
+    eta = c[0] - .5 * dt * (u * (G[0] - G[6]) + w * (G[1] - G[7]) + depth * (G[2] + G[5]));
+    uh = u - .5 * dt * (u * G[2] + w * G[3] + gravity * G[0]);
+    wh = w - .5 * dt * (u * G[4] + w * G[5] + gravity * G[1]);
+    if (eta < c[3]) { eta = c[0]; uh = u; wh = w; }
+    if (depth <= dry) { eta = c[0]; uh = 0.; wh = 0.; }
+}
+
+// hllc_flux (hllc.h:48-72): normal-frame HLLC, HLLE at dry faces and at
+// inadmissible contact states.
+__device__ __forceinline__ void flux(double hl, double ul, double vl, double hr, double ur, double vr, double g,
+                                     double dry, double out[3], double &speed)
+{
+    const double cl = sqrt(g * hl), cr = sqrt(g * hr);
+    double sl = fmin(ul - cl, ur - cr), sr = fmax(ul + cl, ur + cr);
+    if (hl <= dry) sl = ur - 2. * cr;
+    if (hr <= dry) sr = ul + 2. * cl;
+    speed = fmax(fabs(sl), fabs(sr));
+    const double fL0 = hl * ul, fL1 = hl * ul * ul + .5 * g * hl * hl, fL2 = hl * ul * vl;
+    const double fR0 = hr * ur, fR1 = hr * ur * ur + .5 * g * hr * hr, fR2 = hr * ur * vr;
+    if (hl <= dry && hr <= dry) { out[0] = out[1] = out[2] = 0.; speed = 0.; return; }
+    if (sl >= 0.) { out[0] = fL0; out[1] = fL1; out[2] = fL2; return; }
+    if (sr <= 0.) { out[0] = fR0; out[1] = fR1; out[2] = fR2; return; }
+    const double ml = hl * (sl - ul), mr = hr * (sr - ur), den = ml - mr;
+    const double sm = fabs(den) > 1e-30 ? (.5 * g * (hr * hr - hl * hl) + ml * ul - mr * ur) / den : 0.;
+    if (hl > dry && hr > dry && sl < sm && sm < sr) {
+        const double hs = sm >= 0. ? ml / (sl - sm) : mr / (sr - sm);
+        if (hs >= 0. && isfinite(hs)) {
+            if (sm >= 0.) {
+                out[0] = fL0 + sl * (hs - hl);
+                out[1] = fL1 + sl * (hs * sm - hl * ul);
+                out[2] = fL2 + sl * (hs * vl - hl * vl);
+            } else {
+                out[0] = fR0 + sr * (hs - hr);
+                out[1] = fR1 + sr * (hs * sm - hr * ur);
+                out[2] = fR2 + sr * (hs * vr - hr * vr);
+            }
+            return;
+        }
+    }
+    const double den2 = sr - sl, ss = sl * sr;
+    out[0] = (sr * fL0 - sl * fR0 + ss * (hr - hl)) / den2;
+    out[1] = (sr * fL1 - sl * fR1 + ss * (hr * ur - hl * ul)) / den2;
+    out[2] = (sr * fL2 - sl * fR2 + ss * (hr * vr - hl * vl)) / den2;
+}
+
+// Extrapolation of the half-step values of cell c to the midpoint of face f.
+__device__ __forceinline__ void extrapolate(const Geo &g, int c, int f, const double *half, const double *grad,
+                                            double v[4])
+{
+    const int n = g.n, nf = g.nf;
+    const double rx = g.mid[f] - g.xy[c], ry = g.mid[nf + f] - g.xy[n + c];
+#pragma unroll
+    for (int m = 0; m < 4; ++m) v[m] = half[m * n + c] + grad[2 * m * n + c] * rx + grad[(2 * m + 1) * n + c] * ry;
+}
+
+// hllc_faces on an interior or wall face (hllc.h:91-112).
+__device__ __forceinline__ void face_interior(const Geo &g, int f, double gravity, double dry, const double *half,
+                                              const double *grad, double fl[3], double fr[3], double &speed)
+{
+    const int n = g.n, nf = g.nf;
+    const int l = g.owner[f], r = g.neighbour[f];
+    const double nx = g.normal[f], ny = g.normal[nf + f];
+    double L[4], R[4];
+    extrapolate(g, l, f, half, grad, L);
+    if (r < 0) {
+#pragma unroll
+        for (int m = 0; m < 4; ++m) R[m] = L[m];
+    } else {
+        extrapolate(g, r, f, half, grad, R);
+    }
+    const bool ldry = half[l] - half[3 * n + l] <= dry;
+    const bool rdry = r >= 0 && half[r] - half[3 * n + r] <= dry;
+    if (ldry || rdry) {
+        L[3] = half[3 * n + l];
+        R[3] = r < 0 ? L[3] : half[3 * n + r];
+    }
+    if (ldry) { L[0] = L[3]; L[1] = 0.; L[2] = 0.; }
+    if (rdry) { R[0] = R[3]; R[1] = 0.; R[2] = 0.; }
+    if (r < 0) {
+        const double un = R[1] * nx + R[2] * ny;
+        R[1] -= 2. * un * nx;
+        R[2] -= 2. * un * ny;
+    }
+    const double zs = fmax(L[3], R[3]);
+    const double hl = fmax(0., L[0] - L[3]), hr = fmax(0., R[0] - R[3]);
+    const double hls = fmax(0., L[0] - zs), hrs = fmax(0., R[0] - zs);
+    double out[3];
+    flux(hls, L[1] * nx + L[2] * ny, -L[1] * ny + L[2] * nx, hrs, R[1] * nx + R[2] * ny, -R[1] * ny + R[2] * nx,
+         gravity, dry, out, speed);
+    const double el = fmax(0., half[l] - L[3]);
+    const double er = r < 0 ? el : fmax(0., half[r] - R[3]);
+    const double pl = .5 * gravity * (hl * hl - hls * hls - el * el);
+    const double pr = .5 * gravity * (hr * hr - hrs * hrs - er * er);
+    fl[0] = fr[0] = r < 0 ? 0. : out[0];
+    fl[1] = (out[1] + pl) * nx - out[2] * ny;
+    fl[2] = (out[1] + pl) * ny + out[2] * nx;
+    fr[1] = (out[1] + pr) * nx - out[2] * ny;
+    fr[2] = (out[1] + pr) * ny + out[2] * nx;
+}
+
+// hllc_faces on a selected (open) face (hllc.h:84-90): the exterior flux net
+// of the owner's cell-centre pressure; ext and tr are the face's exterior
+// flux and trace.
+__device__ __forceinline__ void face_selected(const Geo &g, int f, double gravity, double dry, const double *half,
+                                              const double ext[3], const double tr[3], double fl[3], double &speed)
+{
+    const int n = g.n, nf = g.nf;
+    const int l = g.owner[f];
+    const double nx = g.normal[f], ny = g.normal[nf + f];
+    const double h = fmax(0., half[l] - half[3 * n + l]);
+    fl[0] = ext[0];
+    fl[1] = ext[1] - .5 * gravity * h * h * nx;
+    fl[2] = ext[2] - .5 * gravity * h * h * ny;
+    const double hb = tr[0], ub = hb > dry ? (tr[1] * nx + tr[2] * ny) / hb : 0.;
+    speed = fmax(fabs(half[n + l] * nx + half[2 * n + l] * ny) + sqrt(gravity * h),
+                 fabs(ub) + sqrt(gravity * fmax(hb, 0.)));
+}
+
+// hllc_turbine_pressure on one face (hllc_turbines.h:13-30): the low-Froude
+// acoustic correction added to the momentum fluxes of a fully wet
+// small-jump interior face.
+__device__ __forceinline__ void turbine_pressure(const Geo &g, int f, double gravity, double dry, const double *half,
+                                                 const double *grad, double fl[3], double fr[3])
+{
+    const int n = g.n, nf = g.nf;
+    const int a = g.owner[f], b = g.neighbour[f];
+    if (b < 0) return;
+    if (half[a] - half[3 * n + a] <= dry || half[b] - half[3 * n + b] <= dry) return;
+    double L[4], R[4];
+    extrapolate(g, a, f, half, grad, L);
+    extrapolate(g, b, f, half, grad, R);
+    const double bed = fmax(L[3], R[3]), hl = L[0] - bed, hr = R[0] - bed, h = .5 * (hl + hr);
+    if (fmin(hl, hr) <= dry || fabs(hr - hl) > .005 * h) return;
+    const double c = sqrt(gravity * h);
+    const double froude = fmin(1., fmax(hypot(L[1], L[2]), hypot(R[1], R[2])) / c);
+    const double nx = g.normal[f], ny = g.normal[nf + f];
+    const double du = (R[1] - L[1]) * nx + (R[2] - L[2]) * ny;
+    const double delta = .5 * (1. - froude) * c * (2. * hl * hr / (hl + hr)) * du;
+    fl[1] += delta * nx; fr[1] += delta * nx;
+    fl[2] += delta * ny; fr[2] += delta * ny;
+}
+
+// hllc_budget for one cell (hllc.h:119-131): donor factor theta (-1: an
+// unfunded prescribed withdrawal) and the face-sum frequency.  mass[f] is
+// fl(1,f), speed[f] the face signal speed; depth the cell's stock.
+__device__ __forceinline__ void budget(const Geo &g, int i, double dt, double depth, const double *mass,
+                                       const double *speed, int status, double &theta, double &frequency)
+{
+    double internal = 0., prescribed = 0., sum = 0.;
+    for (int k = g.off[i]; k < g.off[i + 1]; ++k) {
+        const int f = g.faces[k];
+        const double debit = fmax(0., g.signs[k] * mass[f]) * g.length[f];
+        if (g.selected[f]) prescribed += debit;
+        else internal += debit;
+        sum += speed[f] * g.length[f];
+    }
+    frequency = status ? __builtin_inf() : sum / g.area[i];
+    theta = 1.;
+    const double stock = depth * g.area[i] / dt;
+    if (prescribed > stock) theta = -1.;
+    else if (internal > 0. && internal >= stock - prescribed)
+        theta = fmin(1., (stock - prescribed) / internal) * (1. - 32. * 2.2204460492503131e-16);
+}
+
+// hllc_limit for one face (hllc.h:136-141).  water = fl(1,f).  Returns the
+// face status.  When the donor's factor acts (changed), *mass is the limited
+// water of both sides; when moreover the donor is wet (corr), d1 and d2 are
+// the momentum corrections added to both sides.
+__device__ __forceinline__ int limit(const Geo &g, int f, double dry, const double *h, const double *hu,
+                                     const double *hv, const double *theta, double water, double &mass, double &d1,
+                                     double &d2, bool &changed, bool &corr)
+{
+    changed = false; corr = false; mass = water; d1 = 0.; d2 = 0.;
+    const int donor = water < 0. ? g.neighbour[f] : g.owner[f];
+    if (donor < 0 || water == 0.) return 0;
+    if (g.selected[f]) return theta[donor] < 0. ? 1 : 0;
+    if (theta[donor] >= 1.) return 0;
+    const double factor = fmax(0., theta[donor]);
+    const double delta = (factor - 1.) * water;
+    mass = water * factor;
+    changed = true;
+    if (h[donor] > dry) {
+        corr = true;
+        d1 = delta * hu[donor] / h[donor];
+        d2 = delta * hv[donor] / h[donor];
+    }
+    return 0;
+}
+
+}  // namespace hl
+}  // namespace braids

@@ -16,6 +16,10 @@ template void run_exner<double>(const MeshView<double> &, StateView<double>, con
 template void run_river_substep<double>(const MeshView<double> &, StateView<double>, StateView<double>, StateView<double>,
                                         StateView<double>, const double *, Work<double>, Control *, const RiverBuffers &,
                                         const TurbineView *, double, double, const LaunchCfg &, hipStream_t);
+template void run_hllc_substep<double>(const MeshView<double> &, StateView<double>, StateView<double>, StateView<double>,
+                                       StateView<double>, const double *, Work<double>, Control *, const RiverBuffers &,
+                                       const TurbineView *, const HllcView &, double, double, const LaunchCfg &,
+                                       hipStream_t);
 
 void run_footprint(int npairs, const double *tri, const double *D, const double *nodes, const double *weights,
                    double *value, double *clipped, int *order, hipStream_t stream)

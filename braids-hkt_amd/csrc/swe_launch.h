@@ -9,7 +9,8 @@ namespace braids {
 
 // Kernel kinds timed by the optional event profiler.
 enum : int { KK_PREP = 0, KK_TRACES, KK_FACE, KK_DRAIN, KK_GATHER, KK_LEDGER,
-             KK_RDRAG, KK_RTHETA, KK_TURBINE, KK_RUPDATE, KK_COUNT };
+             KK_RDRAG, KK_RTHETA, KK_TURBINE, KK_RUPDATE,
+             KK_HRECON, KK_HFACE, KK_HBUDGET, KK_HUPDATE, KK_COUNT };
 
 struct LaunchCfg {
     int block = 256;    // threads per workgroup (k_prep/k_face/k_drain/k_gather)
@@ -52,6 +53,15 @@ template <class R>
 void run_river_substep(const MeshView<R> &m, StateView<R> base, StateView<R> work, StateView<R> q1,
                        StateView<R> out, const double *z, Work<R> w, Control *ctl, const RiverBuffers &rb,
                        const TurbineView *tv, double dt, double t, const LaunchCfg &cfg, hipStream_t stream);
+
+// One HLLC MUSCL-Hancock substep (HLLCHydraulics._spatial_step): base ->
+// work (drag half-step) -> qh (Hancock predictor) -> out.  FP64 only
+// (instantiated for R = double).  tv may be null.  Control slot 0.
+template <class R>
+void run_hllc_substep(const MeshView<R> &m, StateView<R> base, StateView<R> work, StateView<R> qh,
+                      StateView<R> out, const double *z, Work<R> w, Control *ctl, const RiverBuffers &rb,
+                      const TurbineView *tv, const HllcView &H, double dt, double t, const LaunchCfg &cfg,
+                      hipStream_t stream);
 
 // ---- batched river substeps over many scenarios (slots) ----
 template <class R> struct SlotView {

@@ -6,6 +6,7 @@
 #include "river_kernels.h"
 #include "batch_kernels.h"
 #include "exner_kernels.h"
+#include "hllc_kernels.h"
 #include <hip/hip_ext.h>
 #include <type_traits>
 
@@ -152,6 +153,41 @@ void run_river_substep(const MeshView<R> &m, StateView<R> base, StateView<R> wor
     if (m.nb > 0)
         hipLaunchKernelGGL((k_river_ledger<R>), dim3(m.nb), dim3(TRACE_BLOCK), 0, s, m,
                            (const double *)rb.wlim[0], (const double *)rb.wlim[1], rb.ledger, dt);
+    run_finalize<R>(w, ctl, s);
+}
+
+template <class R>
+void run_hllc_substep(const MeshView<R> &m, StateView<R> base, StateView<R> work, StateView<R> qh,
+                      StateView<R> out, const double *z, Work<R> w, Control *ctl, const RiverBuffers &rb,
+                      const TurbineView *tv, const HllcView &H, double dt, double t, const LaunchCfg &cfg,
+                      hipStream_t s)
+{
+    static_assert(std::is_same<R, double>::value, "the HLLC substep is FP64");
+    const int B = 256;
+    const dim3 blk(B);
+    const int ncb = grid_for(m.nc, B);
+    TurbineView T{};
+    if (tv && tv->nrot > 0) { T = *tv; T.error = &ctl->turbine_error; }
+    const int nrb = (T.nrot + (B / 64) - 1) / (B / 64);
+    const double g = (double)m.grav, dry = (double)m.dry;
+    timed_launch(cfg, 0, KK_RDRAG, k_river_drag<R>, dim3(ncb), blk, s, m.nc, base, work, rb.drag, .5 * dt, g, dry,
+                 cfg.remap);
+    timed_launch(cfg, 0, KK_HRECON, kh_reconstruct, dim3(ncb + nrb), blk, s, H, m, work, z, qh, T, g, dry, dt, ncb,
+                 cfg.remap);
+    if (H.momentum && T.nunion > 0)
+        hipLaunchKernelGGL(kh_predict, dim3(grid_for(T.nunion, B)), blk, 0, s, H, work, qh, T, dry, dt);
+    if (m.nb > 0)
+        timed_launch(cfg, 0, KK_TRACES, k_traces<R, 1>, dim3(m.nb), dim3(TRACE_BLOCK), s, m, qh, z, w, ctl, 0,
+                     t + .5 * dt);
+    timed_launch(cfg, 0, KK_HFACE, kh_faces, dim3(grid_for(m.nf, B)), blk, s, H, qh, (const double *)w.trace, g, dry,
+                 cfg.remap);
+    timed_launch(cfg, 0, KK_HBUDGET, kh_budget, dim3(ncb + nrb), blk, s, H, m, work, qh, z,
+                 w.rpart + rpart_index(RQ_PEAK, 0), T, dt, ncb, cfg.remap);
+    timed_launch(cfg, 0, KK_HUPDATE, kh_update, dim3(ncb), blk, s, H, work, qh, out, T, rb.drag, ctl, g, dry, dt,
+                 cfg.remap);
+    if (m.nb > 0)
+        hipLaunchKernelGGL((k_river_ledger<R>), dim3(m.nb), dim3(TRACE_BLOCK), 0, s, m, (const double *)H.F,
+                           (const double *)H.F, rb.ledger, dt);
     run_finalize<R>(w, ctl, s);
 }
 

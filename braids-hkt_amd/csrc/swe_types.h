@@ -14,6 +14,8 @@ enum : int { ADJ_OWNER = 1, ADJ_OPEN = 2, ADJ_OUTLET2 = 4 };
 enum : int { FAIL_DRAIN = 50, FAIL_TRACE = 100 };
 // River substep failures (Control::failure of stage slots 0 and 1).
 enum : int { RFAIL_NEGATIVE = 3, RFAIL_NONFINITE = 5, RFAIL_BOUNDARY = 60 };
+// HLLC substep failures (Control::failure of slot 0; boundary traces use FAIL_TRACE + code).
+enum : int { HFAIL_MOMENTUM = 1, HFAIL_POSITIVE = 2 };
 
 template <class R> struct MeshView {
     int nc, nf, nb, scheme;
@@ -115,6 +117,26 @@ struct ExnerView {
     int *fail;
     double *result;      // [8]: ok, residual, boundary, change, donor min
     int nblk;
+};
+
+// HLLC path: operator geometry in the reference owner's layout (FP64,
+// component-major SoA; per-cell incident faces in ascending face order, sign
+// +1 for the owner) and the buffers of the fused Hancock substep.
+struct HllcGeo {
+    int n, nf;
+    const double *xy, *mid, *normal, *length, *area, *inverse;   // [2n] [2nf] [2nf] [nf] [n] [3n]
+    const int *owner, *neighbour, *selected, *off, *faces, *signs;
+};
+
+struct HllcView {
+    HllcGeo g;
+    double *grad;       // [8n] limited gradient of (eta, u, v, z)
+    double *half;       // [4n] half-step eta, u, v and the bed
+    uint8_t *pstat;     // [n]  non-finite predictor
+    double *F;          // [6nf] mass, fl momentum (2), fr momentum (2), speed
+    double *theta;      // [n]  donor factor
+    int pressure;       // low-Froude face correction on
+    int momentum;       // rotor array installed
 };
 
 }  // namespace braids

@@ -39,6 +39,42 @@ int river_cell_drag(int n, double dt, double gravity, double dry, int mode, doub
 int river_cell_drag_spatial(int n, double dt, double gravity, double dry, int mode, const double *values,
                             int feedback, const double *components, double *q, double *chezy);
 
+/* HLLC MUSCL-Hancock polygon operators, FP64, bit-identical to the
+ * reference's host build.  SoA a[m*n+i]; per-cell incident faces (off, faces,
+ * signs) in ascending face order, sign +1 for the owner. */
+/* src/braids_core/hydraulics/hllc.h:15-46 */
+int hllc_reconstruct(int n, int nf, double gravity, double dry, double dt, const double *q, const double *z,
+                     const double *xy, const double *mid, const double *normal, const int *owner,
+                     const int *neighbour, const int *selected, const int *off, const int *faces,
+                     const double *inverse, double *grad, double *half, double *qhalf, int *status);
+/* src/braids_core/hydraulics/hllc.h:74-77 */
+int hllc_predict_source(int n, double dry, double dt, const double *rate, double *half, double *qhalf);
+/* src/braids_core/hydraulics/hllc.h:79-114 (with hllc_flux, :48-72) */
+int hllc_faces(int n, int nf, double gravity, double dry, const double *half, const double *grad,
+               const double *xy, const double *mid, const double *normal, const int *owner, const int *neighbour,
+               const int *selected, const double *external, const double *external_trace, double *fl, double *fr,
+               double *speed);
+/* src/braids_core/hydraulics/hllc.h:116-132 */
+int hllc_budget(int n, int nf, double dt, const double *q, const double *area, const double *length,
+                const int *off, const int *faces, const int *signs, const double *fl, const double *speed,
+                const int *selected, const int *status, double *theta, double *frequency);
+/* src/braids_core/hydraulics/hllc.h:134-143 */
+int hllc_limit(int n, int nf, double dry, const double *q, const double *theta, const int *owner,
+               const int *neighbour, const int *selected, double *fl, double *fr, int *status);
+/* src/braids_core/hydraulics/hllc.h:145-155 */
+int hllc_update(int n, int nf, double dt, double dry, const double *q, const double *area, const double *length,
+                const int *off, const int *faces, const int *signs, const double *fl, const double *fr,
+                const int *face_status, double *next, double *status);
+/* src/braids_core/hydraulics/hllc_turbines.h:13-30 (hypot from the device math library: may differ from the
+ * host C library in the last place) */
+int hllc_turbine_pressure(int n, int nf, double gravity, double dry, const double *half, const double *grad,
+                          const double *xy, const double *mid, const double *normal, const int *owner,
+                          const int *nb, double *fl, double *fr);
+/* src/braids_core/hydraulics/hllc_advanced.h:13-22 */
+int hllc_advanced_frequency(int n, int nf, double dry, double nu, const double *q, const double *metric,
+                            const int *owner, const int *nb, const int *off, const int *faces, const double *length,
+                            const double *area, double *out);
+
 #ifdef __cplusplus
 }
 #endif

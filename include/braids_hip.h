@@ -141,6 +141,41 @@ int braids_river_step(braids_solver *s, double dt, double t, braids_river_result
 int braids_river_advance(braids_solver *s, double dt, double time, double *step_hint, int *substeps,
                          int *rejections, double *accepted_cfl_max);
 
+/* ---- HLLC MUSCL-Hancock owner (src/braids_core/hydraulics/hllc.py), FP64 solvers ---- */
+#define BRAIDS_HLLC_EXTERIOR 1    /* the exterior provider failed (boundary trace) */
+#define BRAIDS_HLLC_CFL 2         /* dt * max frequency > cfl (1 + 1e-12), or a non-finite frequency */
+#define BRAIDS_HLLC_POSITIVITY 3  /* endpoint / boundary positivity (hllc_update status) */
+#define BRAIDS_HLLC_MOMENTUM 4    /* non-finite state after the half-time momentum source */
+#define BRAIDS_HLLC_TURBINE 6     /* induction outside its real domain (an error, not a retry) */
+
+/* HLLCHydraulics(mesh, cfl=cfl, integration='hancock') with the river owner's
+ * friction (hllc.py:16-70, river_hllc.py:48-54): face-sum CFL limit cfl in
+ * (0, 1]; drag_mode/drag_values as braids_river_configure; inverse [3*nc] host
+ * (inv00, inv01, inv11) of the pseudo-inverse of each cell's LS metric as
+ * hllc.py:33-43 builds it; pressure 1/0 switches the low-Froude face
+ * correction of hllc_turbines.h on/off, -1 ties it to an installed rotor array
+ * (river_turbines.py:38-42).  The exterior provider of open faces is the
+ * first-order boundary trace (modes 2 and 6) of the predictor state at the
+ * half time with the HLLC flux between the owner's state and the trace. */
+int braids_hllc_configure(braids_solver *s, double cfl, int drag_mode, const double *drag_values,
+                          const double *inverse, int pressure);
+/* One Hancock substep of size dt at time t (HLLCHydraulics._spatial_step,
+ * hllc.py:76-118); commits on success, BRAIDS_REJECTED (reason
+ * BRAIDS_HLLC_*) leaves the state unchanged.  An installed rotor array is
+ * the momentum source (predictor at the start, corrector at the half time). */
+int braids_hllc_step(braids_solver *s, double dt, double t, braids_river_result *res);
+/* HLLCHydraulics.advance (hllc.py:131-160) over an outer interval dt:
+ * truncation at the interval end, growth <= 2x, rejection -> min(dt/2,
+ * .98 cfl / peak); fails after 100 rejections or below dt/2^20.
+ * *step_hint in/out (<= 0: none yet). */
+int braids_hllc_advance(braids_solver *s, double dt, double time, double *step_hint, int *substeps,
+                        int *rejections, double *accepted_cfl_max);
+/* Exactly nsteps accepted Hancock substeps with the same hint rules and no
+ * outer interval (no reference counterpart: the throughput loop of bench);
+ * *time and *step_hint (> 0) in/out. */
+int braids_hllc_run(braids_solver *s, int nsteps, double *time, double *step_hint, int *accepted,
+                    int *rejections);
+
 /* Exactly nsteps accepted substeps with the same hint controller and no
  * outer interval (no reference counterpart: the throughput loop of bench and
  * serving callers); *time and *step_hint (> 0) in/out. */

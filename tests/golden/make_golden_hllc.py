@@ -68,7 +68,7 @@ def chain_inputs(m, seed, dt=None, nan_cell=None):
                 metric=rng.uniform(0.1, 2.0, nf), nu=float(rng.uniform(0.0, 0.5)), grav=9.81, dry=1e-8)
 
 
-def run_chain(ops, g, x):
+def run_chain(ops, g, x, pressure=True):
     n = g['area'].size; nf = g['length'].size
     c = lambda a, t=np.float64: np.ascontiguousarray(a, dtype=t)
     q, z = c(x['q']), c(x['z'])
@@ -84,8 +84,9 @@ def run_chain(ops, g, x):
     ops['hllc_faces'](n, nf, grav, dry, half, grad, g['xy'], g['mid'], g['normal'], g['owner'], g['neighbour'],
                       g['selected'], c(x['external']), c(x['trace']), fl, fr, speed)
     out.update(fl0=fl.copy(), fr0=fr.copy(), speed=speed.copy())
-    ops['hllc_turbine_pressure'](n, nf, grav, dry, half, grad, g['xy'], g['mid'], g['normal'], g['owner'],
-                                 g['neighbour'], fl, fr)
+    if pressure:
+        ops['hllc_turbine_pressure'](n, nf, grav, dry, half, grad, g['xy'], g['mid'], g['normal'], g['owner'],
+                                     g['neighbour'], fl, fr)
     out.update(fl1=fl.copy(), fr1=fr.copy())
     theta = np.zeros(n); freq = np.zeros(n)
     ops['hllc_budget'](n, nf, dt, q, g['area'], g['length'], g['off'], g['faces'], g['signs'], fl, speed,

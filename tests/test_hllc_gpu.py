@@ -1,0 +1,220 @@
+"""GPU parity of the HLLC MUSCL-Hancock path (libbraids_hip against the oracle,
+oracle/hllc_oracle.c + oracle/hllc.py, which test_hllc_oracle.py pins bit for
+bit to the reference's compiled hllc.h and to closed forms).
+
+Tolerances: the Engine operator entries bit exact (hllc_turbine_pressure to
+1e-14 relative: hypot comes from the device math library); one fused
+substep from identical inputs to 1e-14 of each field's magnitude; FP64
+advance over outer intervals to relative L2 <= 1e-10 per field with
+identical substep and rejection counts; the boundary ledger balances the
+mass inventory to 1e-12.
+"""
+import ctypes
+
+import numpy as np
+import pytest
+
+from braids_hip import mesh as M
+
+pytestmark = pytest.mark.gpu
+
+
+def rel_l2(a, b):
+    return np.array([np.linalg.norm(a[j] - b[j]) / max(np.linalg.norm(b[j]), 1e-300) for j in range(3)])
+
+
+def _device_ops():
+    """The engine entries wrapped to take NumPy arrays (copied to the device,
+    results copied back in place), so the golden chain can drive them."""
+    import torch
+    from braids_hip import _lib
+    from oracle.hllc import OPS
+    L = _lib.load()
+    dev = torch.device('cuda:0')
+
+    def wrap(name):
+        fn = getattr(L, name)
+
+        def call(*args):
+            held, conv = [], []
+            for a in args:
+                if isinstance(a, np.ndarray):
+                    t = torch.from_numpy(np.ascontiguousarray(a)).to(dev)
+                    held.append((a, t)); conv.append(ctypes.c_void_p(t.data_ptr()))
+                else:
+                    conv.append(a)
+            assert fn(*conv) == 0, name
+            torch.cuda.synchronize()
+            for a, t in held:
+                a[...] = t.cpu().numpy()
+        return call
+    return {n: wrap(n) for n in OPS}
+
+
+def test_engine_operators_match_oracle():
+    from oracle import hllc as OH
+    from tests.golden.make_golden_hllc import CASES, case_mesh, chain_inputs, run_chain
+    m = case_mesh(nx=14, ny=5)
+    g = OH.geometry(m)
+    dev = _device_ops()
+    for k, (seed, dt, nan_cell) in enumerate(CASES):
+        x = chain_inputs(m, seed + 100, dt, nan_cell)
+        print(f'[hllc] engine ops case {k}', flush=True)
+        want = run_chain(OH.oracle_ops(), g, x, pressure=False)
+        got = run_chain(dev, g, x, pressure=False)
+        for n in want:
+            assert np.array_equal(got[n], want[n], equal_nan=True), (k, n)
+        want = run_chain(OH.oracle_ops(), g, x)
+        got = run_chain(dev, g, x)
+        for n in ('fl1', 'fr1'):
+            a, b = got[n], want[n]
+            fin = np.isfinite(b)
+            assert np.array_equal(np.isfinite(a), fin)
+            assert np.all(np.abs(a[fin] - b[fin]) <= 1e-14 * np.maximum(np.abs(b[fin]), 1.0)), (k, n)
+
+
+# -- the fused Hancock substep ------------------------------------------------
+TABLE = dict(mode='thetis_speed_table', model_name='test rotor', performance=[
+    dict(speed_m_s=s, thrust_coefficient=c, power_coefficient=p) for s, c, p in
+    zip([0.5, 1.0, 1.5, 2.0, 2.5, 3.0, 3.5], [0.8, 0.85, 0.85, 0.8, 0.7, 0.6, 0.5],
+        [0.3, 0.4, 0.42, 0.42, 0.4, 0.35, 0.3])])
+
+
+def rotor(X, Y, hub, D=5.0):
+    return dict(name=f'T{X:g}', X=X, Y=Y, D=D, hub_elevation_m=hub, minimum_bed_clearance_m=0.5,
+                minimum_surface_clearance_m=0.5, control=TABLE)
+
+
+def closed_flume(hr=0.0):
+    m = M.structured_triangles(400, 2, 0.25, 0.5, bed=lambda x, y: np.zeros_like(x), manning=lambda x, y: 0.03,
+                               depth0=lambda x, y, z: np.ones_like(x), inflow_q=0.0, outlet_stage=1.0, name='flume')
+    m.bid[:] = -1; m.nb = 0; m.mode = np.zeros(0, np.int32)
+    m.forcing_q = np.zeros(0); m.forcing_stage = np.zeros(0)
+    m.build_incidence()
+    x = m.xy[0] - 50.0
+    m.q0 = np.zeros((3, m.nc)); m.q0[0] = np.where(x < 0, 2.0, hr)
+    return m, [], 0
+
+
+def channel_flow():
+    m = M.plain_channel(nx=100, ny=20, length=200.0, width=40.0, depth=8.0, q=1.5 * 8.0 * 40.0)
+    m.q0[1] = m.q0[0] * 1.5
+    return m, [], 2
+
+
+def channel_turbines():
+    m, _, mode = channel_flow()
+    return m, [rotor(80.3, 20.1, -0.08 + 3.3), rotor(120.7, 13.3, -0.12 + 3.3)], mode
+
+
+def braided_small():
+    m = M.braided_reach(nx=60, ny=12, length=300.0, width=60.0, amplitude=8.0, wavelength=150.0,
+                        nbars=12, q=30.0, depth=1.0, bar_height=(0.3, 1.0), bar_sigma=(3.0, 10.0), bank=2.0)
+    return m, [], 2
+
+
+def lake():
+    return M.lake_at_rest(), [], 2
+
+
+# (case, outer intervals, interval length)
+CASES = {'dambreak_dry': (closed_flume, 2, 2.0), 'channel_flow': (channel_flow, 3, 1.0),
+         'channel_turbines': (channel_turbines, 3, 1.0), 'braided': (braided_small, 2, 1.0), 'lake': (lake, 2, 2.0)}
+
+
+def make_pair(make, cfl=.9):
+    from oracle import hllc as OH
+    from braids_hip.solver import HydraulicSolver
+    from braids_hip.hllc import HLLCHydraulics
+    from braids_hip.turbines import ThetisArray
+    m, rows, drag = make()
+    o = OH.HLLCOracle(m, cfl=cfl, drag_mode=drag, rows=rows)
+    s = HydraulicSolver(m, precision=64, scheme=5)
+    s.set_state(m.q0, prepare=False)
+    if rows:
+        ThetisArray(m, rows).install(s)
+    h = HLLCHydraulics(s, cfl=cfl, drag_mode=drag, inverse=o.g['inverse'])
+    return m, rows, o, s, h
+
+
+@pytest.mark.parametrize('case', ['channel_turbines', 'braided', 'dambreak_dry'])
+def test_one_substep_matches_oracle(case):
+    """From identical inputs, every Hancock substep of a 60-substep run agrees
+    with the oracle to 1e-14 of each field's magnitude in every cell; rejected
+    trials are rejected by both, for the same reason."""
+    from braids_hip.solver import TrialRejected
+    from oracle.hllc import Rejected
+    m, rows, o, s, h = make_pair(CASES[case][0])
+    q = m.q0.copy(); t = 0.0; dt = 0.02; worst = 0.0; done = 0
+    print(f'[hllc] one-substep {case} nc={m.nc} rotors={len(rows)}', flush=True)
+    for k in range(60):
+        try:
+            qn, _, peak = o.single(q, m.z, dt, t); kind = None
+        except Rejected as e:
+            kind, peak = e.kind, o.peak
+        s.set_state(q, prepare=False)
+        try:
+            info = h.single(dt, t); ok = True
+            assert np.isclose(info['frequency'], peak, rtol=1e-13)
+        except TrialRejected as e:
+            ok = False
+            assert e.info['reason'] == kind, (k, e.info, kind)
+        assert ok == (kind is None), (k, kind)
+        if not ok:
+            dt *= 0.5
+            continue
+        g = s.state()
+        rel = np.max(np.abs(g - qn).max(axis=1) / np.maximum(np.abs(qn).max(axis=1), 1e-300))
+        worst = max(worst, rel); done += 1
+        q = qn; t += dt
+        dt = min(2 * dt, .98 * .9 / peak) if peak > 0 else 2 * dt
+    print(f'[hllc] {case} one-substep max rel err {worst:.3e} over {done} substeps', flush=True)
+    assert done > 30 and worst <= 1e-14
+
+
+@pytest.mark.parametrize('case', list(CASES))
+def test_advance_matches_oracle(case):
+    make, nint, span = CASES[case]
+    m, rows, o, s, h = make_pair(make)
+    print(f'[hllc] advance {case} nc={m.nc} rotors={len(rows)}', flush=True)
+    q = m.q0.copy(); t = 0.0
+    mass0 = s.mass()
+    for k in range(nint):
+        ro = o.advance(q, m.z, span, t)
+        rg = h.advance(span, t)
+        q = ro['q']; t += span
+        print(f'[hllc] {case} interval {k}: substeps gpu={rg["hydraulic_substeps"]} oracle={ro["substeps"]} '
+              f'retries {rg["hydraulic_retries"]}/{ro["retries"]}', flush=True)
+        assert rg['hydraulic_substeps'] == ro['substeps']
+        assert rg['hydraulic_retries'] == ro['retries']
+        assert abs(rg['next_dt_s'] - o.hint) <= 1e-9 * o.hint
+    err = rel_l2(s.state(), q)
+    print(f'[hllc] {case} fp64 rel_l2={err}', flush=True)
+    tol = np.array([1e-10, 1e-10, 1e-10])
+    assert (err <= np.maximum(tol, 1e-10)).all(), err
+    # the boundary ledger balances the inventory
+    drift = abs(s.mass() - mass0 - s.ledger().sum()) / mass0
+    assert drift <= 1e-12, drift
+    if case == 'lake':
+        assert np.abs(s.state() - m.q0).max() < 1e-12
+
+
+def test_rejection_leaves_state_and_reports_the_reason():
+    from braids_hip.solver import TrialRejected
+    m, rows, o, s, h = make_pair(channel_flow)
+    q0 = s.state()
+    with pytest.raises(TrialRejected) as e:
+        h.single(5.0, 0.0)
+    assert e.value.info['reason'] == 'cfl' and e.value.info['frequency'] > 0
+    assert np.array_equal(s.state(), q0)
+    h.single(0.01, 0.0)
+    assert not np.array_equal(s.state(), q0)
+
+
+def test_fp32_solver_refuses_the_hllc_owner():
+    from braids_hip.solver import HydraulicSolver
+    from braids_hip.hllc import HLLCHydraulics
+    m, _, _ = channel_flow()
+    s = HydraulicSolver(m, precision=32, scheme=5)
+    with pytest.raises(ValueError):
+        HLLCHydraulics(s)

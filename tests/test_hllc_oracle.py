@@ -220,3 +220,13 @@ def test_advance_reaches_the_interval_end_within_cfl():
     assert r['cfl_max'] <= .9 * (1 + 1e-12)
     r2 = o.advance(r['q'], m.z, 2.0, 2.0)                          # the hint carries over
     assert r2['retries'] == 0
+
+
+def test_product_metric_inverse_matches_oracle():
+    """The product's vectorised metric (braids_hip.hllc.metric_inverse) against
+    the oracle's per-cell restatement of hllc.py:33-43: equal to the last
+    place (numpy's dot may fuse the squared norm)."""
+    from braids_hip.hllc import metric_inverse
+    for m in (case_mesh(), M.lake_at_rest(), flowing_channel().renumbered()):
+        a, b = metric_inverse(m), OH.metric_inverse(m)
+        assert np.abs(a - b).max() <= 4e-16 * np.abs(b).max()
