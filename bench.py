@@ -7,7 +7,12 @@ A step is one accepted time step of the whole mesh, all Runge-Kutta stages:
           sink and update, driven by the substep hint controller with its
           rejections;
   ssp33   one accepted SSPRK(3,3) step of the cut-cell solver (three Euler
-          stages with draining and Manning), with its retries.
+          stages with draining and Manning), with its retries;
+  hllc    one accepted MUSCL-Hancock substep of the HLLC owner (FP64): Strang
+          drag, LS reconstruction with the half-step predictor, exterior
+          traces, hydrostatic HLLC faces with the rotors' low-Froude
+          correction, budget, limiter and update with the turbine sink at the
+          half time, with its rejections.
 
 Workload (default): the braided synthetic reach, 5 km x 400 m, 498,576
 triangles (1768 x 141 quads of 2.83 m, split), sinusoidal thalweg, 200 seeded
@@ -19,7 +24,7 @@ reconstruction/flux arithmetic with FP64 conserved state, bed, free-surface
 differencing and ledgers.  Inputs are resident in HBM before the timed region.
 
     python bench.py [--gpus 1] [--steps K] [--warmup W] [--precision 32|64]
-                    [--integrator river|ssp33] [--workload NAME]
+                    [--integrator river|ssp33|hllc] [--workload NAME]
 """
 import argparse
 import json
@@ -36,7 +41,8 @@ for p in (ROOT, ROOT / 'braids-hkt_amd'):
 METRIC = 'cell-updates/s; batched scenarios/h + p50/p99 latency vs plain, rel L2≤1e-6'
 HBM_PEAK_GBS = 8000.0
 STAGE_KINDS = {'ssp33': ('prep', 'traces', 'face', 'drain', 'gather', 'ledger'),
-               'river': ('prep', 'traces', 'face', 'rtheta', 'rupdate')}
+               'river': ('prep', 'traces', 'face', 'rtheta', 'rupdate'),
+               'hllc': ('rdrag', 'hrecon', 'traces', 'hface', 'hbudget', 'hupdate')}
 
 
 def algorithmic_bytes(kind, m, precision, scheme, nslot=0):
@@ -66,6 +72,16 @@ def algorithmic_bytes(kind, m, precision, scheme, nslot=0):
         return nc * cell + nadj * 5 + nf * (5 * r + r + 12)           # fluxes, length, owner/neighbour/bid
     if kind == 'rdrag':
         return nc * (state + state + 8)
+    # HLLC (FP64 throughout): reference-layout geometry (xy, mid, normal in FP64)
+    if kind == 'hrecon':
+        cell = 4 + state + 8 + 16 + 24 + 64 + 32 + state + 1    # offset, work, z, xy, inverse; grad, half, qhalf, status
+        return nc * cell + nadj * 4 + nf * (8 + 16 + 16 + 4)     # incidence; owner/neighbour, mid, normal, selected
+    if kind == 'hface':
+        return nc * (32 + 64 + 16) + nf * (8 + 4 + 16 + 16 + 48)  # half, grad, xy; ids, selected, normal, mid, fluxes
+    if kind == 'hbudget':
+        return nc * (4 + 8 + 8 + 1 + 8) + nadj * 8 + nf * (8 + 8 + 8 + 4)
+    if kind == 'hupdate':
+        return nc * (4 + state + 8 + 8 + state + 8 + 1) + nadj * 8 + nf * (40 + 8 + 8 + 4)
     if kind == 'turbine':
         return nslot * (4 + 8 + 8 + 24 + 8 + 8)                       # footprint cell, weights, state, area, k out
     return 0
@@ -82,12 +98,12 @@ def stage_bytes(m, precision, scheme):
     return m.nc * (6 * s_q + 8 + 2 * s_g + 2 * s_g * o2) + m.nf * (8 + 3 * s_g + 2 * s_g * o2)
 
 
-def measured_traffic(workload, precision, kind):
+def measured_traffic(workload, integrator, precision, kind):
     """HBM bytes per launch of `kind` from the committed rocprofv3 summary of
-    this workload (FETCH_SIZE and WRITE_SIZE runs calibrated on a copy kernel),
-    or None when no summary exists."""
+    this workload and integrator (FETCH_SIZE and WRITE_SIZE runs calibrated on
+    a copy kernel), or None when no summary exists."""
     from braids_hip import rocprof
-    path = ROOT / 'profiles' / f'rocprof_{workload}_f{precision}.md'
+    path = ROOT / 'profiles' / f'rocprof_{workload}_{integrator}_f{precision}.md'
     if not path.exists():
         return None
     return rocprof.parse_summary(path).get(kind)
@@ -124,6 +140,15 @@ def cpu_baseline(m, rows, scheme, integrator, seconds=12.0):
                 break
             q = r['q']; t += dt; steps += r['substeps']
         what = f'SSPRK(2,2) substeps with {len(rows)} rotors'
+    elif integrator == 'hllc':
+        from oracle import hllc as OH
+        o = OH.HLLCOracle(m, cfl=.9, rows=rows)
+        q = m.q0.copy(); t = 0.0; o.hint = 0.05
+        t0 = time.perf_counter()
+        while time.perf_counter() - t0 < seconds:
+            r = o.run(q, m.z, 1, t)
+            q, t = r['q'], r['time']; steps += 1
+        what = f'HLLC MUSCL-Hancock substeps with {len(rows)} rotors'
     else:
         from oracle import swe
         o = swe.Oracle(m, scheme=scheme, cfl_target=0.9, dtmax=1.0)
@@ -148,7 +173,7 @@ def main():
     ap.add_argument('--warmup', type=int, default=20)
     ap.add_argument('--precision', type=int, default=32, choices=(32, 64))
     ap.add_argument('--scheme', type=int, default=5, choices=(1, 5))
-    ap.add_argument('--integrator', default='river', choices=('river', 'ssp33'))
+    ap.add_argument('--integrator', default='river', choices=('river', 'ssp33', 'hllc'))
     ap.add_argument('--workload', default=None)
     ap.add_argument('--block', type=int, default=256)
     ap.add_argument('--remap', type=int, default=1)
@@ -158,7 +183,9 @@ def main():
         print('bench.py measures one GPU: this build runs the hydraulic step on a single MI355X '
               '(multi-GPU sharding is a design note only); rerun with --gpus 1', file=sys.stderr)
         sys.exit(2)
-    workload = args.workload or ('braided_reach_500k_12t' if args.integrator == 'river' else 'braided_reach_500k')
+    workload = args.workload or ('braided_reach_500k' if args.integrator == 'ssp33' else 'braided_reach_500k_12t')
+    if args.integrator == 'hllc':
+        args.precision = 64                  # the HLLC owner is FP64
 
     import ctypes
     import torch
@@ -184,6 +211,17 @@ def main():
 
         def run(n):
             return s.river_run(n)
+    elif args.integrator == 'hllc':
+        from braids_hip.hllc import HLLCHydraulics
+        from braids_hip.turbines import ThetisArray
+        s.set_state(m.q0, prepare=False)
+        if rows:
+            arr = ThetisArray(m, rows).install(s)
+            nslot = sum(len(r.cells) for r in arr.rotors)
+        HLLCHydraulics(s, cfl=.9)
+
+        def run(n):
+            return s.hllc_run(n)
     else:
         s.set_state(m.q0)
 
@@ -206,18 +244,24 @@ def main():
     s.profile(False)
     q = s.state()
     finite = bool((q == q).all()) and float(q[0].min()) >= 0.0
-    active = None
+    active = None; inactive = None; simulated = s.time
     if rows:
+        from collections import Counter
         from braids_hip.turbines import diagnostics
-        active = sum(d['active'] for d in diagnostics(s)[0])
+        diag = diagnostics(s)[0]
+        active = sum(d['active'] for d in diag)
+        inactive = dict(Counter(d['inactive_reason'] for d in diag if not d['active']))
 
     dom_ms, dom_n = prof[dom]
     avg_s = dom_ms / dom_n / 1e3
     abytes = algorithmic_bytes(dom, m, args.precision, args.scheme, nslot)
     achieved = abytes / avg_s / 1e9
     stage_ms = sum(v[0] / v[1] for k, v in warm.items() if k in STAGE_KINDS[args.integrator])
-    sbytes = stage_bytes(m, args.precision, args.scheme)
-    traffic = measured_traffic(workload, args.precision, dom)
+    if args.integrator == 'hllc':
+        sbytes = sum(algorithmic_bytes(k, m, 64, args.scheme, nslot) for k in STAGE_KINDS['hllc'] if k != 'traces')
+    else:
+        sbytes = stage_bytes(m, args.precision, args.scheme)
+    traffic = measured_traffic(workload, args.integrator, args.precision, dom)
 
     # STREAM-copy achievable bandwidth on the same device
     nbytes = 1 << 30
@@ -244,13 +288,15 @@ def main():
         'dtype': 'f32' if args.precision == 32 else 'f64',
         'data': 'synthetic',
         'config': {'workload': workload, 'cells': m.nc, 'faces': m.nf, 'scheme': args.scheme,
-                   'integrator': 'SSPRK(2,2) river' if args.integrator == 'river' else 'SSPRK(3,3)',
+                   'integrator': {'river': 'SSPRK(2,2) river', 'ssp33': 'SSPRK(3,3)',
+                                  'hllc': 'HLLC MUSCL-Hancock'}[args.integrator],
                    'cfl': 0.5 if args.integrator == 'river' else 0.9,
                    'precision': ('fp32 reconstruction/flux arithmetic; fp64 conserved state, bed, free-surface '
                                  'differencing and ledgers') if args.precision == 32 else 'fp64',
-                   'turbines': len(rows), 'turbines_active': active, 'renumbering': 'hilbert',
+                   'turbines': len(rows), 'turbines_active': active, 'turbines_inactive': inactive,
+                   'simulated_s': simulated, 'renumbering': 'hilbert',
                    'block': args.block, 'xcd_remap': bool(args.remap),
-                   ('rejections' if args.integrator == 'river' else 'retries'): retries,
+                   ('retries' if args.integrator == 'ssp33' else 'rejections'): retries,
                    'state_finite_nonnegative': finite},
         'roofline': {'bound': 'hbm', 'kernel': dom, 'achieved': achieved, 'peak': HBM_PEAK_GBS, 'unit': 'GB/s',
                      'frac': achieved / HBM_PEAK_GBS, 'traffic': traffic,

@@ -406,9 +406,11 @@ def braided_turbine_reach(nturbines=12, D=5.0, velocity=1.0, spacing=200.0, **kw
     return m, rows
 
 
-def lake_at_rest(nx=20, ny=6, dx=4.0, dy=4.0, stage=1.0, seed=3, bumps=True) -> Mesh:
+def lake_at_rest(nx=20, ny=6, dx=4.0, dy=4.0, stage=1.0, seed=3, bumps=True, min_depth=0.0) -> Mesh:
     """Closed basin with a bumpy bed partly above the still-water level:
-    the well-balanced property keeps it exactly at rest."""
+    the well-balanced property keeps it exactly at rest.  min_depth > 0 turns
+    every wet cell shallower than it into a dry bank 0.1 m above the water
+    (no thin films at the shoreline)."""
     rng = np.random.default_rng(seed)
     noise = rng.normal(0.0, 0.3, 2 * nx * ny)
 
@@ -426,4 +428,8 @@ def lake_at_rest(nx=20, ny=6, dx=4.0, dy=4.0, stage=1.0, seed=3, bumps=True) -> 
     m.forcing_q = np.zeros(0)
     m.forcing_stage = np.zeros(0)
     m.build_incidence()
+    if min_depth > 0:
+        thin = (m.q0[0] > 0) & (m.q0[0] < min_depth)
+        m.z[thin] = stage + 0.1
+        m.q0[:, thin] = 0.0
     return m

@@ -69,6 +69,7 @@ def lib():
             fn.argtypes = args; fn.restype = None
         L.ohl_flux.argtypes = [_D] * 8 + [_P, _P]; L.ohl_flux.restype = None
         L.ohl_exterior.argtypes = [_I, _I, _D, _D] + [_P] * 6; L.ohl_exterior.restype = None
+        L.ohl_metric_inverse.argtypes = [_I, _I] + [_P] * 10; L.ohl_metric_inverse.restype = None
         L.om_boundary_traces.argtypes = [_P, _P, _P, _D, _P, _P]; L.om_boundary_traces.restype = _I
         L._hllc_bound = True
     return L
@@ -109,29 +110,16 @@ def open_faces(mesh):
 
 
 def metric_inverse(mesh):
-    """[3, nc] (inv00, inv01, inv11) of the pseudo-inverse of the LS metric
-    sum d d^T / |d|^2 over neighbours and wall mirrors (open faces skipped)."""
-    nc = mesh.nc
-    xy, mid, nrm = mesh.xy, mesh.mid, mesh.normal
-    sel = open_faces(mesh)
-    inv = np.zeros((3, nc))
-    for i in range(nc):
-        M = np.zeros((2, 2))
-        for k in range(mesh.offset[i], mesh.offset[i + 1]):
-            f = mesh.faces_csr[k]
-            j = mesh.neighbour[f] if mesh.signs[k] > 0 else mesh.owner[f]
-            if j < 0 and sel[f]:
-                continue
-            if j >= 0:
-                d = xy[:, j] - xy[:, i]
-            else:
-                d = 2 * np.dot(mid[:, f] - xy[:, i], nrm[:, f]) * nrm[:, f]
-            nn = np.dot(d, d)
-            if nn > 0:
-                M += np.outer(d, d) / nn
-        P = np.linalg.pinv(M, rcond=1e-12)
-        inv[:, i] = (P[0, 0], P[0, 1], P[1, 1])
-    return inv
+    """[3, nc] (inv00, inv01, inv11) of the inverse LS metric of every cell
+    (hllc.py:33-43), by the closed-form restatement in hllc_oracle.c."""
+    c = np.ascontiguousarray
+    n, nf = mesh.nc, mesh.nf
+    out = np.zeros((3, n))
+    args = [c(mesh.xy, np.float64), c(mesh.mid, np.float64), c(mesh.normal, np.float64), c(mesh.owner, np.int32),
+            c(mesh.neighbour, np.int32), open_faces(mesh), c(mesh.offset, np.int32), c(mesh.faces_csr, np.int32),
+            c(mesh.signs, np.int32), out]
+    lib().ohl_metric_inverse(n, nf, *[a.ctypes.data for a in args])
+    return out
 
 
 def geometry(mesh):

@@ -362,3 +362,48 @@ void ohl_exterior(int n, int nf, double gravity, double dry, const double *qhalf
         external[2 * nf + f] = out[1] * ny + out[2] * nx;
     }
 }
+
+/* Inverse least-squares metric of each cell (hllc.py:33-43): M = sum d d^T /
+   |d|^2 over the neighbours and the wall mirror images (open faces skipped),
+   inverted in closed form; a metric whose smaller eigenvalue is at most
+   1e-12 of the larger gets the rank-one pseudo-inverse v v^T / lambda_max
+   (numpy.linalg.pinv with rcond 1e-12).  out[3n] = (inv00, inv01, inv11). */
+void ohl_metric_inverse(int n, int nf, const double *xy, const double *mid, const double *normal, const int *owner,
+                        const int *neighbour, const int *selected, const int *off, const int *faces,
+                        const int *signs, double *out)
+{
+    for (int i = 0; i < n; ++i) {
+        double a = 0., b = 0., d = 0.;
+        for (int k = off[i]; k < off[i + 1]; ++k) {
+            const int f = faces[k];
+            const int j = signs[k] > 0 ? neighbour[f] : owner[f];
+            double dx, dy;
+            if (j >= 0) {
+                dx = xy[j] - xy[i];
+                dy = xy[n + j] - xy[n + i];
+            } else {
+                if (selected[f]) continue;
+                const double nx = normal[f], ny = normal[nf + f];
+                const double dist = (mid[f] - xy[i]) * nx + (mid[nf + f] - xy[n + i]) * ny;
+                dx = 2. * dist * nx;
+                dy = 2. * dist * ny;
+            }
+            const double nn = dx * dx + dy * dy;
+            if (nn > 0.) { a += dx * dx / nn; b += dx * dy / nn; d += dy * dy / nn; }
+        }
+        const double tr = a + d, disc = sqrt(.25 * (a - d) * (a - d) + b * b);
+        const double l1 = .5 * tr + disc, l2 = .5 * tr - disc;
+        if (l1 <= 0.) { out[i] = out[n + i] = out[2 * n + i] = 0.; continue; }
+        if (l2 > 1e-12 * l1) {
+            const double det = a * d - b * b;
+            out[i] = d / det; out[n + i] = -b / det; out[2 * n + i] = a / det;
+        } else {
+            /* eigenvector of l1 */
+            double vx = b, vy = l1 - a;
+            if (fabs(vx) + fabs(vy) == 0.) { vx = l1 - d; vy = b; }
+            if (fabs(vx) + fabs(vy) == 0.) { vx = 1.; vy = 0.; }
+            const double s = 1. / (vx * vx + vy * vy);
+            out[i] = vx * vx * s / l1; out[n + i] = vx * vy * s / l1; out[2 * n + i] = vy * vy * s / l1;
+        }
+    }
+}

@@ -114,7 +114,7 @@ def braided_small():
 
 
 def lake():
-    return M.lake_at_rest(), [], 2
+    return M.lake_at_rest(min_depth=0.05), [], 2
 
 
 # (case, outer intervals, interval length)
