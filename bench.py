@@ -40,9 +40,10 @@ for p in (ROOT, ROOT / 'braids-hkt_amd'):
 
 METRIC = 'cell-updates/s; batched scenarios/h + p50/p99 latency vs plain, rel L2≤1e-6'
 HBM_PEAK_GBS = 8000.0
-STAGE_KINDS = {'ssp33': ('prep', 'traces', 'face', 'drain', 'gather', 'ledger'),
-               'river': ('prep', 'traces', 'face', 'rtheta', 'rupdate'),
-               'hllc': ('rdrag', 'hrecon', 'traces', 'hface', 'hbudget', 'hupdate')}
+# (the open-boundary traces run inside the face launch)
+STAGE_KINDS = {'ssp33': ('prep', 'face', 'drain', 'gather', 'ledger'),
+               'river': ('prep', 'face', 'rtheta', 'rupdate'),
+               'hllc': ('rdrag', 'hrecon', 'hface', 'hbudget', 'hupdate')}
 
 
 def algorithmic_bytes(kind, m, precision, scheme, nslot=0):
@@ -258,7 +259,7 @@ def main():
     achieved = abytes / avg_s / 1e9
     stage_ms = sum(v[0] / v[1] for k, v in warm.items() if k in STAGE_KINDS[args.integrator])
     if args.integrator == 'hllc':
-        sbytes = sum(algorithmic_bytes(k, m, 64, args.scheme, nslot) for k in STAGE_KINDS['hllc'] if k != 'traces')
+        sbytes = sum(algorithmic_bytes(k, m, 64, args.scheme, nslot) for k in STAGE_KINDS['hllc'])
     else:
         sbytes = stage_bytes(m, args.precision, args.scheme)
     traffic = measured_traffic(workload, args.integrator, args.precision, dom)

@@ -141,10 +141,11 @@ struct SolverBase {
     void collect(int slots, int path = 0)
     {
         if (events.empty()) return;
+        // the open-boundary traces run inside the face launch (k_face_traces, kh_faces)
         static const bool kinds[3][KK_COUNT] = {
-            {true, true, true, true, true, true, false, false, false, false, false, false, false, false},
-            {true, true, true, false, false, false, true, true, false, true, false, false, false, false},
-            {false, true, false, false, false, false, true, false, false, false, true, true, true, true}};
+            {true, false, true, true, true, true, false, false, false, false, false, false, false, false},
+            {true, false, true, false, false, false, true, true, false, true, false, false, false, false},
+            {false, false, false, false, false, false, true, false, false, false, true, true, true, true}};
         const bool river = path == 1;
         for (int slot = 0; slot < slots; ++slot)
             for (int k = 0; k < KK_COUNT; ++k) {
@@ -172,6 +173,8 @@ struct Solver : SolverBase {
     Control *ctl = nullptr;            // device block: Control + 4*nb ledger doubles
     double *ledger_dev = nullptr;
     void *host_block = nullptr;        // pinned readback
+    unsigned int *host_block_dev = nullptr;   // its device-visible address (written by the closing kernels)
+    double *diag_dev_host = nullptr;          // device-visible address of diag_host
     size_t block_bytes = 0, head_bytes = 0;
     LaunchCfg cfg;
     double *qscratch = nullptr;        // [3*nc] double for conversions
@@ -332,11 +335,19 @@ struct Solver : SolverBase {
         ledger_dev = (double *)(blk + sizeof(Control));
         w.rpart = (unsigned long long *)(blk + part_off);
         HIP_TRY(hipHostMalloc(&host_block, head_bytes, hipHostMallocDefault));
+        HIP_TRY(hipHostGetDevicePointer((void **)&host_block_dev, host_block, 0));
         ledger.assign(nb, 0.0);
         hstate.resize(nc);
         return BRAIDS_OK;
     }
 
+    HostMirror mirror() const
+    {
+        HostMirror hm{};
+        hm.head = host_block_dev; hm.head_words = (int)(head_bytes / 4);
+        if (nrot > 0) { hm.diag = tv.diag; hm.diag_host = diag_dev_host; hm.ndiag = 2 * nrot * TURBINE_DIAG; }
+        return hm;
+    }
     void zero_control()
     {
         int words = (int)(block_bytes / 4);
@@ -371,9 +382,8 @@ struct Solver : SolverBase {
         run_stage<R>(m, st[0], st[0], st[1], z, w, ctl, ledger_dev, 0, dt, t, cfg, stream);
         run_stage<R>(m, st[1], st[0], st[2], z, w, ctl, ledger_dev + (nb + 1), 1, dt, t + dt, cfg, stream);
         run_stage<R>(m, st[2], st[0], st[3], z, w, ctl, ledger_dev + 2 * (nb + 1), 2, dt, t + .5 * dt, cfg, stream);
-        run_finalize<R>(w, ctl, stream);
+        run_finalize_mirror<R>(w, ctl, mirror(), stream);
         HIP_TRY(hipGetLastError());
-        HIP_TRY(hipMemcpyAsync(host_block, ctl, head_bytes, hipMemcpyDeviceToHost, stream));
         HIP_TRY(hipStreamSynchronize(stream));
         if (profiling) collect(3);
         const Control *hc = (const Control *)host_block;
@@ -509,6 +519,7 @@ struct Solver : SolverBase {
         if ((rc = alloc(T.diag, 2 * (size_t)n * TURBINE_DIAG))) return rc;
         HIP_TRY(hipMemset(T.diag, 0, 2 * (size_t)n * TURBINE_DIAG * sizeof(double)));
         HIP_TRY(hipHostMalloc(&diag_host, 2 * (size_t)n * TURBINE_DIAG * sizeof(double), hipHostMallocDefault));
+        HIP_TRY(hipHostGetDevicePointer((void **)&diag_dev_host, diag_host, 0));
         tv = T;
         nrot = n;
         energy.assign(n, 0.0); energy_comp.assign(n, 0.0);
@@ -519,13 +530,9 @@ struct Solver : SolverBase {
     {
         int rc = 0;
         if (!rb.theta && (rc = river_buffers())) return rc;
-        zero_control();
-        run_river_substep<R>(m, st[0], st[1], st[2], st[3], z, w, ctl, rb, nrot > 0 ? &tv : nullptr, dt, t, cfg, stream);
+        run_river_substep<R>(m, st[0], st[1], st[2], st[3], z, w, ctl, rb, nrot > 0 ? &tv : nullptr, dt, t, cfg, stream,
+                             (unsigned int *)ctl, (int)(block_bytes / 4), mirror());
         HIP_TRY(hipGetLastError());
-        HIP_TRY(hipMemcpyAsync(host_block, ctl, head_bytes, hipMemcpyDeviceToHost, stream));
-        if (nrot > 0)
-            HIP_TRY(hipMemcpyAsync(diag_host, tv.diag, 2 * (size_t)nrot * TURBINE_DIAG * sizeof(double),
-                                   hipMemcpyDeviceToHost, stream));
         HIP_TRY(hipStreamSynchronize(stream));
         if (profiling) collect(2, 1);
         const Control *hc = (const Control *)host_block;
@@ -623,14 +630,9 @@ struct Solver : SolverBase {
             if (!hv.grad) { set_error("HLLC is not configured (braids_hllc_configure)"); return BRAIDS_EINVAL; }
             hv.momentum = nrot > 0;
             hv.pressure = hllc_pressure < 0 ? nrot > 0 : hllc_pressure != 0;
-            zero_control();
             run_hllc_substep<R>(m, st[0], st[1], st[2], st[3], z, w, ctl, rb, nrot > 0 ? &tv : nullptr, hv, dt, t,
-                                cfg, stream);
+                                cfg, stream, (unsigned int *)ctl, (int)(block_bytes / 4), mirror());
             HIP_TRY(hipGetLastError());
-            HIP_TRY(hipMemcpyAsync(host_block, ctl, head_bytes, hipMemcpyDeviceToHost, stream));
-            if (nrot > 0)
-                HIP_TRY(hipMemcpyAsync(diag_host, tv.diag, 2 * (size_t)nrot * TURBINE_DIAG * sizeof(double),
-                                       hipMemcpyDeviceToHost, stream));
             HIP_TRY(hipStreamSynchronize(stream));
             if (profiling) collect(1, 2);
             const Control *hc = (const Control *)host_block;

@@ -7,16 +7,18 @@
 //                   `work` (the predictor's momentum source)
 //   kh_predict      footprint cells: the rotor sink over dt/2 into qhalf and
 //                   the half-step velocities (hllc_predict_source)
-//   k_traces        open boundaries: first-order traces of qhalf at t + dt/2
-//   kh_faces        faces: exterior flux on open faces (HLLC between the
-//                   owner's predictor state and the trace), hydrostatic HLLC
-//                   elsewhere, plus the low-Froude correction of a rotor array
+//   kh_faces        faces: hydrostatic HLLC plus the low-Froude correction of
+//                   a rotor array; extra workgroups: the first-order traces
+//                   of qhalf at t + dt/2 per open boundary, then the exterior
+//                   flux of its faces (HLLC between the owner's predictor
+//                   state and the trace)
 //   kh_budget       cells: donor factor and face-sum frequency (max-reduced,
 //                   NaN kept); extra workgroups: the rotors sample qhalf
 //   kh_update       cells: hllc_limit applied per incident face on the fly,
 //                   hllc_update, the rotor sink at the half time, the closing
 //                   drag half-step
-//   k_river_ledger  inward water of the open boundaries
+//   k_close         inward water of the open boundaries; the last workgroup
+//                   writes the flags into pinned host memory
 // Rejections are flags in the Control block (slot 0), read once per substep.
 #pragma once
 #include "hllc_ops.h"
@@ -116,23 +118,42 @@ __device__ __forceinline__ void exterior_flux(const hl::Geo &g, int f, double gr
     e[2] = out[1] * ny + out[2] * nx;
 }
 
-static __global__ void __launch_bounds__(256) kh_faces(HllcView H, StateView<double> qh, const double *__restrict__ trace,
-                                                double gravity, double dry, int remap)
+// An open face: the exterior flux from the trace, net of the owner's
+// cell-centre pressure.
+__device__ __forceinline__ void hllc_open_face(const HllcView &H, const StateView<double> &qh,
+                                               const double *__restrict__ trace, int f, double gravity, double dry)
 {
-    const int f = logical_block(remap) * blockDim.x + threadIdx.x;
     const int nf = H.g.nf;
-    if (f >= nf) return;
-    double a[3], b[3], s;
-    if (H.g.selected[f]) {
-        const double tr[3] = {trace[f], trace[nf + f], trace[2 * nf + f]};
-        double e[3];
-        exterior_flux(H.g, f, gravity, dry, qh, tr, e);
-        hl::face_selected(H.g, f, gravity, dry, H.half, e, tr, a, s);
-        b[1] = a[1]; b[2] = a[2];
-    } else {
-        hl::face_interior(H.g, f, gravity, dry, H.half, H.grad, a, b, s);
-        if (H.pressure) hl::turbine_pressure(H.g, f, gravity, dry, H.half, H.grad, a, b);
+    const double tr[3] = {trace[f], trace[nf + f], trace[2 * nf + f]};
+    double e[3], a[3], s;
+    exterior_flux(H.g, f, gravity, dry, qh, tr, e);
+    hl::face_selected(H.g, f, gravity, dry, H.half, e, tr, a, s);
+    H.F[f] = a[0]; H.F[nf + f] = a[1]; H.F[2 * nf + f] = a[2];
+    H.F[3 * nf + f] = a[1]; H.F[4 * nf + f] = a[2]; H.F[5 * nf + f] = s;
+}
+
+// Faces; workgroups past the nfb face blocks each take one open boundary:
+// its first-order traces of qhalf at the half time (k_traces_body), then the
+// exterior fluxes of its faces.  The face blocks skip open faces.
+static __global__ void __launch_bounds__(256) kh_faces(HllcView H, MeshView<double> m, StateView<double> qh,
+                                                       const double *__restrict__ z, Work<double> w, Control *ctl,
+                                                       double t, double gravity, double dry, int nfb, int remap)
+{
+    if ((int)blockIdx.x >= nfb) {
+        const int b = (int)blockIdx.x - nfb;
+        k_traces_body<double, 1>(m, qh, z, w, ctl, 0, t, b);
+        __syncthreads();
+        for (int k = m.bnd_offset[b] + threadIdx.x; k < m.bnd_offset[b + 1]; k += blockDim.x)
+            hllc_open_face(H, qh, w.trace, m.bnd_faces[k], gravity, dry);
+        return;
     }
+    const int lb = remap ? xcd_order(blockIdx.x, nfb) : (int)blockIdx.x;
+    const int f = lb * blockDim.x + threadIdx.x;
+    const int nf = H.g.nf;
+    if (f >= nf || H.g.selected[f]) return;
+    double a[3], b[3], s;
+    hl::face_interior(H.g, f, gravity, dry, H.half, H.grad, a, b, s);
+    if (H.pressure) hl::turbine_pressure(H.g, f, gravity, dry, H.half, H.grad, a, b);
     H.F[f] = a[0]; H.F[nf + f] = a[1]; H.F[2 * nf + f] = a[2];
     H.F[3 * nf + f] = b[1]; H.F[4 * nf + f] = b[2]; H.F[5 * nf + f] = s;
 }

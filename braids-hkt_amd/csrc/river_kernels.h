@@ -60,10 +60,16 @@ __device__ __forceinline__ void k_river_drag_body(int nc, StateView<R> in, State
     out.h[c] = h; out.hu[c] = hu; out.hv[c] = hv;
 }
 
+// The first launch of a substep: the drag half-step, and the clearing of the
+// substep's control words (nzero 32-bit words; the launch has at least that
+// many threads).
 template <class R>
 __global__ void __launch_bounds__(256) k_river_drag(int nc, StateView<R> in, StateView<R> out, DragView d,
-                                                    double dt, double grav, double dry, int remap)
+                                                    double dt, double grav, double dry, int remap,
+                                                    unsigned int *zero, int nzero)
 {
+    const int g = blockIdx.x * blockDim.x + threadIdx.x;
+    if (g < nzero) zero[g] = 0u;
     k_river_drag_body<R>(nc, in, out, d, dt, grav, dry, remap, logical_block(remap));
 }
 
@@ -250,13 +256,6 @@ __device__ __forceinline__ void k_river_theta_body(MeshView<R> m, StateView<R> q
     block_max_to(freq, w.rpart + rpart_index(RQ_PEAK, slot));
 }
 
-// XCD-aware order of block b among n blocks (the logical_block mapping for a
-// sub-range whose first block is a multiple of 8).
-__device__ __forceinline__ int xcd_order(int b, int n)
-{
-    const int qq = n >> 3, r8 = n & 7, x = b & 7, k = b >> 3;
-    return x * qq + (x < r8 ? x : r8) + k;
-}
 
 // Workgroups past the ncb cell blocks sample the rotors, one wavefront each.
 template <class R>
@@ -386,6 +385,56 @@ __global__ void __launch_bounds__(TRACE_BLOCK) k_river_ledger(MeshView<R> m, con
                                                               double *ledger, double dt)
 {
     k_river_ledger_body<R>(m, w1, w2, ledger, dt, blockIdx.x);
+}
+
+// The maxima of the partial slots into the Control block, then its copy
+// (with the stage ledgers) and the rotor diagnostics into pinned host memory.
+__device__ __forceinline__ void finalize_block(const unsigned long long *rpart, Control *ctl, const HostMirror &hm)
+{
+    const int t = threadIdx.x;
+    if (t < 8) {
+        const int q = t >> 2, s = t & 3;
+        const unsigned long long *p = rpart + rpart_index(q, s);
+        unsigned long long m = 0;
+        for (int i = 0; i < RSLOTS; ++i) { const unsigned long long v = p[i * RSTRIDE]; m = v > m ? v : m; }
+        if (q == RQ_HMAX) ctl->hmax_bits[s] = m; else ctl->peak_bits[s] = m;
+    }
+    __threadfence();
+    __syncthreads();
+    if (hm.head) {
+        const volatile unsigned int *src = (const volatile unsigned int *)ctl;
+        for (int i = t; i < hm.head_words; i += blockDim.x) hm.head[i] = src[i];
+        const volatile double *dg = hm.diag;
+        for (int i = t; i < hm.ndiag; i += blockDim.x) hm.diag_host[i] = dg[i];
+        __threadfence_system();
+    }
+}
+
+// Closing launch of a river or HLLC substep: one workgroup per open boundary
+// sums its inward water (k_river_ledger_body); the last workgroup to finish
+// (a counter in the Control block, no waiting) then runs finalize_block.
+template <class R>
+__global__ void __launch_bounds__(TRACE_BLOCK) k_close(MeshView<R> m, const double *w1, const double *w2,
+                                                       double *ledger, double dt, const unsigned long long *rpart,
+                                                       Control *ctl, HostMirror hm)
+{
+    if (m.nb > 0) {
+        __shared__ int last;
+        k_river_ledger_body<R>(m, w1, w2, ledger, dt, blockIdx.x);
+        __threadfence();
+        if (threadIdx.x == 0) last = atomicAdd(&ctl->close_count, 1) == (int)gridDim.x - 1;
+        __syncthreads();
+        if (!last) return;
+        __threadfence();
+    }
+    finalize_block(rpart, ctl, hm);
+}
+
+// The SSPRK(3,3) step's finalize with the host mirror.
+static __global__ void __launch_bounds__(64) k_finalize_mirror(const unsigned long long *rpart, Control *ctl,
+                                                               HostMirror hm)
+{
+    finalize_block(rpart, ctl, hm);
 }
 
 // Footprint weights (river_turbine_geometry.py:29-71): one workgroup per

@@ -9,11 +9,13 @@ template void run_stage<float>(const MeshView<float> &, StateView<float>, StateV
 template void run_rates<float>(const MeshView<float> &, StateView<float>, const double *, Work<float>, Control *,
                                double *, double *, double, const LaunchCfg &, hipStream_t);
 template void run_finalize<float>(Work<float>, Control *, hipStream_t);
+template void run_finalize_mirror<float>(Work<float>, Control *, const HostMirror &, hipStream_t);
 template void run_batch_substep<float>(const BatchPlan &, const SlotView<float> *, SlotRun *, const int *,
                                         unsigned int *const *, hipStream_t);
 template void run_exner<float>(const MeshView<float> &, StateView<float>, const double *, ExnerView, double,
                                 const LaunchCfg &, hipStream_t);
 template void run_river_substep<float>(const MeshView<float> &, StateView<float>, StateView<float>, StateView<float>,
                                         StateView<float>, const double *, Work<float>, Control *, const RiverBuffers &,
-                                        const TurbineView *, double, double, const LaunchCfg &, hipStream_t);
+                                        const TurbineView *, double, double, const LaunchCfg &, hipStream_t,
+                                        unsigned int *, int, const HostMirror &);
 }  // namespace braids

@@ -9,17 +9,19 @@ template void run_stage<double>(const MeshView<double> &, StateView<double>, Sta
 template void run_rates<double>(const MeshView<double> &, StateView<double>, const double *, Work<double>, Control *,
                                 double *, double *, double, const LaunchCfg &, hipStream_t);
 template void run_finalize<double>(Work<double>, Control *, hipStream_t);
+template void run_finalize_mirror<double>(Work<double>, Control *, const HostMirror &, hipStream_t);
 template void run_batch_substep<double>(const BatchPlan &, const SlotView<double> *, SlotRun *, const int *,
                                         unsigned int *const *, hipStream_t);
 template void run_exner<double>(const MeshView<double> &, StateView<double>, const double *, ExnerView, double,
                                 const LaunchCfg &, hipStream_t);
 template void run_river_substep<double>(const MeshView<double> &, StateView<double>, StateView<double>, StateView<double>,
                                         StateView<double>, const double *, Work<double>, Control *, const RiverBuffers &,
-                                        const TurbineView *, double, double, const LaunchCfg &, hipStream_t);
+                                        const TurbineView *, double, double, const LaunchCfg &, hipStream_t,
+                                        unsigned int *, int, const HostMirror &);
 template void run_hllc_substep<double>(const MeshView<double> &, StateView<double>, StateView<double>, StateView<double>,
                                        StateView<double>, const double *, Work<double>, Control *, const RiverBuffers &,
                                        const TurbineView *, const HllcView &, double, double, const LaunchCfg &,
-                                       hipStream_t);
+                                       hipStream_t, unsigned int *, int, const HostMirror &);
 
 void run_footprint(int npairs, const double *tri, const double *D, const double *nodes, const double *weights,
                    double *value, double *clipped, int *order, hipStream_t stream)

@@ -20,6 +20,7 @@
 #pragma once
 #include <hip/hip_runtime.h>
 #include <stdint.h>
+#include <type_traits>
 #include "swe_riemann.h"
 #include "swe_types.h"
 
@@ -106,6 +107,14 @@ __device__ __forceinline__ int logical_block(int remap)
     const int n = gridDim.x, q = n >> 3, r = n & 7;
     const int x = b & 7, k = b >> 3;
     return x * q + (x < r ? x : r) + k;
+}
+
+// XCD-aware order of block b among n blocks (the logical_block mapping for a
+// sub-range whose first block is a multiple of 8).
+__device__ __forceinline__ int xcd_order(int b, int n)
+{
+    const int qq = n >> 3, r8 = n & 7, x = b & 7, k = b >> 3;
+    return x * qq + (x < r8 ? x : r8) + k;
 }
 
 // ---------------------------------------------------------------------------
@@ -527,12 +536,10 @@ __global__ void __launch_bounds__(TRACE_BLOCK) k_traces(MeshView<R> m, StateView
 
 // ---------------------------------------------------------------------------
 template <class R, int SCHEME>
-__device__ __forceinline__ void k_face_body(MeshView<R> m, StateView<R> q, const double *__restrict__ z,
-                                              Work<R> w, int remap, int lb)
+__device__ __forceinline__ void face_one(const MeshView<R> &m, const StateView<R> &q, const double *__restrict__ z,
+                                         const Work<R> &w, int f)
 {
     const int nf = m.nf;
-    const int f = lb * blockDim.x + threadIdx.x;
-    if (f >= nf) return;
     const R g = m.grav, dry = m.dry;
     const int c = m.owner[f], n = m.neighbour[f], b = m.bid[f];
     const R nx = m.normal[f], ny = m.normal[nf + f];
@@ -652,6 +659,15 @@ __device__ __forceinline__ void k_face_body(MeshView<R> m, StateView<R> q, const
 }
 
 template <class R, int SCHEME>
+__device__ __forceinline__ void k_face_body(MeshView<R> m, StateView<R> q, const double *__restrict__ z,
+                                              Work<R> w, int remap, int lb)
+{
+    const int f = lb * blockDim.x + threadIdx.x;
+    if (f >= m.nf) return;
+    face_one<R, SCHEME>(m, q, z, w, f);
+}
+
+template <class R, int SCHEME>
 __global__ void __launch_bounds__(256) k_face(MeshView<R> m, StateView<R> q, const double *__restrict__ z,
                                               Work<R> w, int remap)
 {
@@ -665,13 +681,11 @@ __global__ void __launch_bounds__(256) k_face(MeshView<R> m, StateView<R> q, con
 // which for the wet-column correction equals 1/2 g (h_c + h_f)(eta_f - eta_c),
 // a free-surface difference.  A lake at rest therefore stays at rest in FP32.
 template <int SCHEME>
-__device__ __forceinline__ void k_face_f32_body(MeshView<float> m, StateView<float> q, const double *__restrict__ z,
-                                                  Work<float> w, int remap, int lb)
+__device__ __forceinline__ void face_one_f32(const MeshView<float> &m, const StateView<float> &q,
+                                             const double *__restrict__ z, const Work<float> &w, int f)
 {
     typedef float R;
     const int nf = m.nf;
-    const int f = lb * blockDim.x + threadIdx.x;
-    if (f >= nf) return;
     const R g = m.grav, dry = m.dry;
     const double gg = (double)g, dd = (double)dry;
     const int c = m.owner[f], n = m.neighbour[f], b = m.bid[f];
@@ -769,10 +783,44 @@ __device__ __forceinline__ void k_face_f32_body(MeshView<float> m, StateView<flo
 }
 
 template <int SCHEME>
+__device__ __forceinline__ void k_face_f32_body(MeshView<float> m, StateView<float> q, const double *__restrict__ z,
+                                                  Work<float> w, int remap, int lb)
+{
+    const int f = lb * blockDim.x + threadIdx.x;
+    if (f >= m.nf) return;
+    face_one_f32<SCHEME>(m, q, z, w, f);
+}
+
+template <int SCHEME>
 __global__ void __launch_bounds__(256) k_face_f32(MeshView<float> m, StateView<float> q, const double *__restrict__ z,
                                                   Work<float> w, int remap)
 {
     k_face_f32_body<SCHEME>(m, q, z, w, remap, logical_block(remap));
+}
+
+// The faces of one stage together with the open-boundary traces.  Workgroups
+// past the nfb face blocks each take one open boundary: its traces
+// (k_traces_body), then the fluxes of its faces; the face blocks skip open
+// faces.  One launch instead of the traces launch followed by the faces.
+template <class R, int SCHEME>
+__global__ void __launch_bounds__(256) k_face_traces(MeshView<R> m, StateView<R> q, const double *__restrict__ z,
+                                                     Work<R> w, Control *ctl, int stage, double t, int nfb, int remap)
+{
+    if ((int)blockIdx.x >= nfb) {
+        const int b = (int)blockIdx.x - nfb;
+        k_traces_body<R, SCHEME>(m, q, z, w, ctl, stage, t, b);
+        __syncthreads();
+        for (int k = m.bnd_offset[b] + threadIdx.x; k < m.bnd_offset[b + 1]; k += blockDim.x) {
+            if constexpr (std::is_same<R, float>::value) face_one_f32<SCHEME>(m, q, z, w, m.bnd_faces[k]);
+            else face_one<R, SCHEME>(m, q, z, w, m.bnd_faces[k]);
+        }
+        return;
+    }
+    const int lb = remap ? xcd_order(blockIdx.x, nfb) : (int)blockIdx.x;
+    const int f = lb * blockDim.x + threadIdx.x;
+    if (f >= m.nf || m.bid[f] >= 0) return;
+    if constexpr (std::is_same<R, float>::value) face_one_f32<SCHEME>(m, q, z, w, f);
+    else face_one<R, SCHEME>(m, q, z, w, f);
 }
 
 // ---------------------------------------------------------------------------

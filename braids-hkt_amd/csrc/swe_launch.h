@@ -32,6 +32,9 @@ void run_stage(const MeshView<R> &m, StateView<R> in, StateView<R> base, StateVi
 // Reduce the partial maxima of the stages into the Control block.
 template <class R>
 void run_finalize(Work<R> w, Control *ctl, hipStream_t stream);
+// The same, then the Control block into the host mirror.
+template <class R>
+void run_finalize_mirror(Work<R> w, Control *ctl, const HostMirror &hm, hipStream_t stream);
 
 // Spatial rate without draining or CFL rejection (bend_native_rates):
 // rate = -sum (L/A)(F - 1/2 g h^2 n); water = fl(1,:).
@@ -49,10 +52,13 @@ struct RiverBuffers {
 
 // One river substep (RiverExplicit._single): base -> work (drag half-step)
 // -> q1 -> out.  tv may be null (no rotors).  Stage slots 0 and 1 of ctl.
+// The first launch clears the nzero control words at `zero`; the closing
+// launch writes the Control block and rotor diagnostics into hm.
 template <class R>
 void run_river_substep(const MeshView<R> &m, StateView<R> base, StateView<R> work, StateView<R> q1,
                        StateView<R> out, const double *z, Work<R> w, Control *ctl, const RiverBuffers &rb,
-                       const TurbineView *tv, double dt, double t, const LaunchCfg &cfg, hipStream_t stream);
+                       const TurbineView *tv, double dt, double t, const LaunchCfg &cfg, hipStream_t stream,
+                       unsigned int *zero, int nzero, const HostMirror &hm);
 
 // One HLLC MUSCL-Hancock substep (HLLCHydraulics._spatial_step): base ->
 // work (drag half-step) -> qh (Hancock predictor) -> out.  FP64 only
@@ -61,7 +67,7 @@ template <class R>
 void run_hllc_substep(const MeshView<R> &m, StateView<R> base, StateView<R> work, StateView<R> qh,
                       StateView<R> out, const double *z, Work<R> w, Control *ctl, const RiverBuffers &rb,
                       const TurbineView *tv, const HllcView &H, double dt, double t, const LaunchCfg &cfg,
-                      hipStream_t stream);
+                      hipStream_t stream, unsigned int *zero, int nzero, const HostMirror &hm);
 
 // ---- batched river substeps over many scenarios (slots) ----
 template <class R> struct SlotView {

@@ -53,6 +53,16 @@ static void launch_face(const MeshView<R> &m, StateView<R> in, const double *z, 
         timed_launch(cfg, slot, KK_FACE, k_face<R, SCHEME>, dim3(grid_for(m.nf, B)), dim3(B), s, m, in, z, w, cfg.remap);
 }
 
+// Traces and faces of one stage in one launch (k_face_traces).
+template <class R, int SCHEME>
+static void launch_face_traces(const MeshView<R> &m, StateView<R> in, const double *z, Work<R> w, Control *ctl,
+                               int slot, double t, const LaunchCfg &cfg, hipStream_t s)
+{
+    const int B = 256, nfb = grid_for(m.nf, B);
+    timed_launch(cfg, slot, KK_FACE, k_face_traces<R, SCHEME>, dim3(nfb + m.nb), dim3(B), s, m, in, z, w, ctl, slot, t,
+                 nfb, cfg.remap);
+}
+
 template <class R, int SCHEME>
 static void stage_kernels(const MeshView<R> &m, StateView<R> in, StateView<R> base, StateView<R> out,
                           const double *z, Work<R> w, Control *ctl, double *ledger_stage, int mode,
@@ -61,8 +71,7 @@ static void stage_kernels(const MeshView<R> &m, StateView<R> in, StateView<R> ba
     const int B = cfg.block;
     const dim3 gc(grid_for(m.nc, B)), blk(B), gb(m.nb), tb(TRACE_BLOCK);
     launch_prep<R, SCHEME>(m, in, z, w, ctl, mode, cfg, s);
-    if (m.nb > 0) timed_launch(cfg, mode, KK_TRACES, k_traces<R, SCHEME>, gb, tb, s, m, in, z, w, ctl, mode, t);
-    launch_face<R, SCHEME>(m, in, z, w, mode, cfg, s);
+    launch_face_traces<R, SCHEME>(m, in, z, w, ctl, mode, t, cfg, s);
     timed_launch(cfg, mode, KK_DRAIN, k_drain<R>, gc, blk, s, m, in, w, ctl, mode, dt, cfg.remap);
     switch (mode) {
     case 0: timed_launch(cfg, mode, KK_GATHER, k_gather<R, 0>, gc, blk, s, m, in, base, out, w, ctl, dt, cfg.remap); break;
@@ -79,6 +88,12 @@ template <class R>
 void run_finalize(Work<R> w, Control *ctl, hipStream_t s)
 {
     hipLaunchKernelGGL(k_finalize, dim3(1), dim3(64), 0, s, (const unsigned long long *)w.rpart, ctl);
+}
+
+template <class R>
+void run_finalize_mirror(Work<R> w, Control *ctl, const HostMirror &hm, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_finalize_mirror, dim3(1), dim3(64), 0, s, (const unsigned long long *)w.rpart, ctl, hm);
 }
 
 template <class R>
@@ -119,8 +134,7 @@ static void river_stage(const MeshView<R> &m, StateView<R> in, StateView<R> in0,
     const int B = cfg.block;
     const dim3 gc(grid_for(m.nc, B)), blk(B), gb(m.nb), tb(TRACE_BLOCK);
     launch_prep<R, SCHEME>(m, in, z, w, ctl, slot, cfg, s, 0);     // the river path uses no max depth
-    if (m.nb > 0) timed_launch(cfg, slot, KK_TRACES, k_traces<R, SCHEME>, gb, tb, s, m, in, z, w, ctl, slot, t);
-    launch_face<R, SCHEME>(m, in, z, w, slot, cfg, s);
+    launch_face_traces<R, SCHEME>(m, in, z, w, ctl, slot, t, cfg, s);
     TurbineView T{};
     if (tv && tv->nrot > 0) { T = *tv; T.error = &ctl->turbine_error; }
     const int ncb = grid_for(m.nc, B), nrb = (T.nrot + (B / 64) - 1) / (B / 64);
@@ -137,12 +151,14 @@ static void river_stage(const MeshView<R> &m, StateView<R> in, StateView<R> in0,
 template <class R>
 void run_river_substep(const MeshView<R> &m, StateView<R> base, StateView<R> work, StateView<R> q1,
                        StateView<R> out, const double *z, Work<R> w, Control *ctl, const RiverBuffers &rb,
-                       const TurbineView *tv, double dt, double t, const LaunchCfg &cfg, hipStream_t s)
+                       const TurbineView *tv, double dt, double t, const LaunchCfg &cfg, hipStream_t s,
+                       unsigned int *zero, int nzero, const HostMirror &hm)
 {
     const int B = cfg.block;
-    const dim3 gc(grid_for(m.nc, B)), blk(B);
+    const int gz = grid_for(nzero, B), gcell = grid_for(m.nc, B);
+    const dim3 gc(gcell > gz ? gcell : gz), blk(B);
     timed_launch(cfg, 0, KK_RDRAG, k_river_drag<R>, gc, blk, s, m.nc, base, work, rb.drag, .5 * dt,
-                 (double)m.grav, (double)m.dry, cfg.remap);
+                 (double)m.grav, (double)m.dry, cfg.remap, zero, nzero);
     if (m.scheme == 5) {
         river_stage<R, 5>(m, work, work, q1, z, w, ctl, rb, tv, 0, dt, t, cfg, s);
         river_stage<R, 5>(m, q1, work, out, z, w, ctl, rb, tv, 1, dt, t + dt, cfg, s);
@@ -150,17 +166,16 @@ void run_river_substep(const MeshView<R> &m, StateView<R> base, StateView<R> wor
         river_stage<R, 1>(m, work, work, q1, z, w, ctl, rb, tv, 0, dt, t, cfg, s);
         river_stage<R, 1>(m, q1, work, out, z, w, ctl, rb, tv, 1, dt, t + dt, cfg, s);
     }
-    if (m.nb > 0)
-        hipLaunchKernelGGL((k_river_ledger<R>), dim3(m.nb), dim3(TRACE_BLOCK), 0, s, m,
-                           (const double *)rb.wlim[0], (const double *)rb.wlim[1], rb.ledger, dt);
-    run_finalize<R>(w, ctl, s);
+    hipLaunchKernelGGL((k_close<R>), dim3(m.nb > 0 ? m.nb : 1), dim3(TRACE_BLOCK), 0, s, m,
+                       (const double *)rb.wlim[0], (const double *)rb.wlim[1], rb.ledger, dt,
+                       (const unsigned long long *)w.rpart, ctl, hm);
 }
 
 template <class R>
 void run_hllc_substep(const MeshView<R> &m, StateView<R> base, StateView<R> work, StateView<R> qh,
                       StateView<R> out, const double *z, Work<R> w, Control *ctl, const RiverBuffers &rb,
                       const TurbineView *tv, const HllcView &H, double dt, double t, const LaunchCfg &cfg,
-                      hipStream_t s)
+                      hipStream_t s, unsigned int *zero, int nzero, const HostMirror &hm)
 {
     static_assert(std::is_same<R, double>::value, "the HLLC substep is FP64");
     const int B = 256;
@@ -170,25 +185,22 @@ void run_hllc_substep(const MeshView<R> &m, StateView<R> base, StateView<R> work
     if (tv && tv->nrot > 0) { T = *tv; T.error = &ctl->turbine_error; }
     const int nrb = (T.nrot + (B / 64) - 1) / (B / 64);
     const double g = (double)m.grav, dry = (double)m.dry;
-    timed_launch(cfg, 0, KK_RDRAG, k_river_drag<R>, dim3(ncb), blk, s, m.nc, base, work, rb.drag, .5 * dt, g, dry,
-                 cfg.remap);
+    const int gz = grid_for(nzero, B);
+    timed_launch(cfg, 0, KK_RDRAG, k_river_drag<R>, dim3(ncb > gz ? ncb : gz), blk, s, m.nc, base, work, rb.drag,
+                 .5 * dt, g, dry, cfg.remap, zero, nzero);
     timed_launch(cfg, 0, KK_HRECON, kh_reconstruct, dim3(ncb + nrb), blk, s, H, m, work, z, qh, T, g, dry, dt, ncb,
                  cfg.remap);
     if (H.momentum && T.nunion > 0)
         hipLaunchKernelGGL(kh_predict, dim3(grid_for(T.nunion, B)), blk, 0, s, H, work, qh, T, dry, dt);
-    if (m.nb > 0)
-        timed_launch(cfg, 0, KK_TRACES, k_traces<R, 1>, dim3(m.nb), dim3(TRACE_BLOCK), s, m, qh, z, w, ctl, 0,
-                     t + .5 * dt);
-    timed_launch(cfg, 0, KK_HFACE, kh_faces, dim3(grid_for(m.nf, B)), blk, s, H, qh, (const double *)w.trace, g, dry,
+    const int nfb = grid_for(m.nf, B);
+    timed_launch(cfg, 0, KK_HFACE, kh_faces, dim3(nfb + m.nb), blk, s, H, m, qh, z, w, ctl, t + .5 * dt, g, dry, nfb,
                  cfg.remap);
     timed_launch(cfg, 0, KK_HBUDGET, kh_budget, dim3(ncb + nrb), blk, s, H, m, work, qh, z,
                  w.rpart + rpart_index(RQ_PEAK, 0), T, dt, ncb, cfg.remap);
     timed_launch(cfg, 0, KK_HUPDATE, kh_update, dim3(ncb), blk, s, H, work, qh, out, T, rb.drag, ctl, g, dry, dt,
                  cfg.remap);
-    if (m.nb > 0)
-        hipLaunchKernelGGL((k_river_ledger<R>), dim3(m.nb), dim3(TRACE_BLOCK), 0, s, m, (const double *)H.F,
-                           (const double *)H.F, rb.ledger, dt);
-    run_finalize<R>(w, ctl, s);
+    hipLaunchKernelGGL((k_close<R>), dim3(m.nb > 0 ? m.nb : 1), dim3(TRACE_BLOCK), 0, s, m, (const double *)H.F,
+                       (const double *)H.F, rb.ledger, dt, (const unsigned long long *)w.rpart, ctl, hm);
 }
 
 template <class R, int SCHEME>

@@ -73,6 +73,18 @@ struct Control {
     int drain_any[4];                  // any draining cell
     int fail_cell[4];                  // 1 + largest cell index that failed in the gather
     int turbine_error;                 // induction outside its real domain (river path)
+    int close_count;                   // workgroups of the closing kernel that have finished
+};
+
+// Pinned host words that the closing kernel of a substep writes directly
+// (the Control block with the stage ledgers, and the rotor diagnostics), so
+// that a substep needs no copy command on the stream.
+struct HostMirror {
+    unsigned int *head;        // device-visible pointer of the pinned head mirror (NULL: none)
+    int head_words;
+    const double *diag;        // rotor diagnostics on the device
+    double *diag_host;         // device-visible pointer of their pinned mirror
+    int ndiag;
 };
 
 // River path: roughness of the Strang drag half-steps and the rotor array.
