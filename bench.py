@@ -42,8 +42,8 @@ METRIC = 'cell-updates/s; batched scenarios/h + p50/p99 latency vs plain, rel L2
 HBM_PEAK_GBS = 8000.0
 # (the open-boundary traces run inside the face launch)
 STAGE_KINDS = {'ssp33': ('prep', 'face', 'drain', 'gather', 'ledger'),
-               'river': ('prep', 'face', 'rtheta', 'rupdate'),
-               'hllc': ('rdrag', 'hrecon', 'hface', 'hbudget', 'hupdate')}
+               'river': ('prep', 'face', 'rupdate'),
+               'hllc': ('rdrag', 'hrecon', 'hface', 'hupdate')}
 
 
 def algorithmic_bytes(kind, m, precision, scheme, nslot=0):
@@ -69,8 +69,9 @@ def algorithmic_bytes(kind, m, precision, scheme, nslot=0):
     if kind == 'rtheta':
         return nc * (4 + 8 + r + 8) + nadj * 5 + nf * 3 * r          # offset, h, area, theta out; incidence; F1, speed, length
     if kind == 'rupdate':
-        cell = 4 + state + state / 2 + state + r + 8 / 2 + 1 + 8      # offset, in, base (stage 2), out, area, drag (stage 2), mask, theta
-        return nc * cell + nadj * 5 + nf * (5 * r + r + 12)           # fluxes, length, owner/neighbour/bid
+        # theta and the frequency are computed here from the face fluxes and speeds
+        cell = 4 + state + state / 2 + state + r + 8 / 2 + 1          # offset, in, base (stage 2), out, area, drag (stage 2), mask
+        return nc * cell + nadj * 8 + nf * (6 * r + r + 2 * r + 12)   # fluxes + speed, length, normal, owner/neighbour/bid
     if kind == 'rdrag':
         return nc * (state + state + 8)
     # HLLC (FP64 throughout): reference-layout geometry (xy, mid, normal in FP64)
@@ -82,7 +83,8 @@ def algorithmic_bytes(kind, m, precision, scheme, nslot=0):
     if kind == 'hbudget':
         return nc * (4 + 8 + 8 + 1 + 8) + nadj * 8 + nf * (8 + 8 + 8 + 4)
     if kind == 'hupdate':
-        return nc * (4 + state + 8 + 8 + state + 8 + 1) + nadj * 8 + nf * (40 + 8 + 8 + 4)
+        # with hllc_budget: the face speeds and the predictor status
+        return nc * (4 + state + 8 + 1 + state + 8 + 1) + nadj * 8 + nf * (48 + 8 + 8 + 4)
     if kind == 'turbine':
         return nslot * (4 + 8 + 8 + 24 + 8 + 8)                       # footprint cell, weights, state, area, k out
     return 0

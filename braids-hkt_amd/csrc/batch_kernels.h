@@ -58,18 +58,8 @@ __global__ void __launch_bounds__(256) kb_prep(BatchPlan P, const SlotView<R> *V
         k_prep_body<R, SCHEME>(v.m, in, v.z, v.w, v.ctl, stage, 1, 0, xcd_order(lb, v.ncb));
 }
 
-template <class R, int SCHEME>
-__global__ void __launch_bounds__(TRACE_BLOCK) kb_traces(BatchPlan P, const SlotView<R> *V, const SlotRun *run,
-                                                         int stage)
-{
-    const int s = find_slot(P.bnd, P.nslot, blockIdx.x);
-    if (!run[s].active) return;
-    const SlotView<R> &v = V[s];
-    const StateView<R> in = stage == 0 ? v.work : v.q1;
-    const double t = run[s].time + (stage ? run[s].hint : 0.0);
-    k_traces_body<R, SCHEME>(v.m, in, v.z, v.w, v.ctl, stage, t, blockIdx.x - P.bnd[s]);
-}
-
+// Per scenario: its face blocks and one block per open boundary (traces and
+// their faces) (face_launch_block).
 template <class R, int SCHEME>
 __global__ void __launch_bounds__(256) kb_face(BatchPlan P, const SlotView<R> *V, const SlotRun *run, int stage)
 {
@@ -77,28 +67,22 @@ __global__ void __launch_bounds__(256) kb_face(BatchPlan P, const SlotView<R> *V
     if (!run[s].active) return;
     const SlotView<R> &v = V[s];
     const int nfb = (v.m.nf + 255) >> 8, lb = blockIdx.x - P.face[s];
-    if (lb >= nfb) return;
+    if (lb >= nfb + v.m.nb) return;
     const StateView<R> in = stage == 0 ? v.work : v.q1;
-    if constexpr (std::is_same<R, float>::value)
-        k_face_f32_body<SCHEME>(v.m, in, v.z, v.w, 1, xcd_order(lb, nfb));
-    else
-        k_face_body<R, SCHEME>(v.m, in, v.z, v.w, 1, xcd_order(lb, nfb));
+    const double t = run[s].time + (stage ? run[s].hint : 0.0);
+    face_launch_block<R, SCHEME, false>(v.m, in, v.z, v.w, v.ctl, v.T, stage, t, nfb, lb, 1);
 }
 
+// The rotors of every scenario sampling its stage input (one wavefront each;
+// a launch of its own, one per stage for the whole batch).
 template <class R>
-__global__ void __launch_bounds__(256) kb_theta(BatchPlan P, const SlotView<R> *V, const SlotRun *run, int stage)
+__global__ void __launch_bounds__(256) kb_rotors(BatchPlan P, const SlotView<R> *V, const SlotRun *run, int stage)
 {
-    const int s = find_slot(P.theta, P.nslot, blockIdx.x);
+    const int s = find_slot(P.rot, P.nslot, blockIdx.x);
     if (!run[s].active) return;
     const SlotView<R> &v = V[s];
-    const int lb = blockIdx.x - P.theta[s];
-    const StateView<R> in = stage == 0 ? v.work : v.q1;
-    if (lb >= v.ncb) {
-        const int a = (lb - v.ncb) * (blockDim.x >> 6) + (threadIdx.x >> 6);
-        if (a < v.T.nrot) turbine_sample_wave<R>(v.T, v.m, in, v.z, stage, a, threadIdx.x & 63);
-        return;
-    }
-    k_river_theta_body<R>(v.m, in, v.w, v.rb.theta, stage, run[s].hint, xcd_order(lb, v.ncb));
+    const int a = (blockIdx.x - P.rot[s]) * (blockDim.x >> 6) + (threadIdx.x >> 6);
+    if (a < v.T.nrot) turbine_sample_wave<R>(v.T, v.m, stage == 0 ? v.work : v.q1, v.z, stage, a, threadIdx.x & 63);
 }
 
 template <class R, int STAGE>
@@ -111,11 +95,11 @@ __global__ void __launch_bounds__(256) kb_update(BatchPlan P, const SlotView<R> 
     if (lb >= v.ncb) return;
     const double dt = run[s].hint;
     if (STAGE == 0)
-        k_river_update_body<R, 0>(v.m, v.work, v.work, v.q1, v.w, v.rb.theta, v.T, v.rb.wlim[0], v.rb.drag, v.ctl,
-                                  0, dt, 0.0, 1, xcd_order(lb, v.ncb));
+        k_river_update_body<R, 0>(v.m, v.work, v.work, v.q1, v.w, v.T, v.rb.wlim[0], v.rb.drag, v.ctl, 0, dt, 0.0,
+                                  xcd_order(lb, v.ncb));
     else
-        k_river_update_body<R, 1>(v.m, v.q1, v.work, v.q[1 - run[s].parity], v.w, v.rb.theta, v.T, v.rb.wlim[1],
-                                  v.rb.drag, v.ctl, 1, dt, .5 * dt, 1, xcd_order(lb, v.ncb));
+        k_river_update_body<R, 1>(v.m, v.q1, v.work, v.q[1 - run[s].parity], v.w, v.T, v.rb.wlim[1], v.rb.drag,
+                                  v.ctl, 1, dt, .5 * dt, xcd_order(lb, v.ncb));
 }
 
 template <class R>

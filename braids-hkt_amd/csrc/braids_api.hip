@@ -15,16 +15,26 @@
 #include "swe_launch.h"
 
 using namespace braids;
+using braids::LaunchFault;
 
 static thread_local std::string g_error;
 static void set_error(const std::string &e) { g_error = e; }
 extern "C" const char *braids_last_error(void) { return g_error.c_str(); }
 
+static std::string launch_fault_note()
+{
+    if (!braids::launch_fault.kind) return "";
+    const LaunchFault f = braids::launch_fault;
+    braids::launch_fault = LaunchFault{};
+    return " (first failing timed launch: kind " + std::to_string(f.kind - 1) + ", slot " + std::to_string(f.slot) +
+           ", grid " + std::to_string(f.grid) + " x " + std::to_string(f.block) + ")";
+}
+
 #define HIP_TRY(expr)                                                                       \
     do {                                                                                    \
         hipError_t e_ = (expr);                                                             \
         if (e_ != hipSuccess) {                                                             \
-            set_error(std::string(#expr) + ": " + hipGetErrorString(e_));                   \
+            set_error(std::string(#expr) + ": " + hipGetErrorString(e_) + launch_fault_note()); \
             return BRAIDS_EHIP;                                                             \
         }                                                                                   \
     } while (0)
@@ -142,10 +152,11 @@ struct SolverBase {
     {
         if (events.empty()) return;
         // the open-boundary traces run inside the face launch (k_face_traces, kh_faces)
+        // (the river path's theta and the HLLC budget run inside their update launches)
         static const bool kinds[3][KK_COUNT] = {
             {true, false, true, true, true, true, false, false, false, false, false, false, false, false},
-            {true, false, true, false, false, false, true, true, false, true, false, false, false, false},
-            {false, false, false, false, false, false, true, false, false, false, true, true, true, true}};
+            {true, false, true, false, false, false, true, false, false, true, false, false, false, false},
+            {false, false, false, false, false, false, true, false, false, false, true, true, false, true}};
         const bool river = path == 1;
         for (int slot = 0; slot < slots; ++slot)
             for (int k = 0; k < KK_COUNT; ++k) {
@@ -155,11 +166,14 @@ struct SolverBase {
                 if ((k == KK_TRACES || k == KK_LEDGER) && nb == 0) continue;
                 if (!((prof_mask >> k) & 1)) continue;
                 float ms = 0;
-                if (hipEventElapsedTime(&ms, events[(slot * KK_COUNT + k) * 2], events[(slot * KK_COUNT + k) * 2 + 1]) == hipSuccess) {
-                    prof_ms[k] += ms; prof_n[k] += 1;
-                }
+                const hipError_t e = hipEventElapsedTime(&ms, events[(slot * KK_COUNT + k) * 2],
+                                                         events[(slot * KK_COUNT + k) * 2 + 1]);
+                if (e == hipSuccess) { prof_ms[k] += ms; prof_n[k] += 1; }
+                else prof_miss[k] += 1;
             }
+        (void)hipGetLastError();     // a pair that was not stamped must not fail the next launch check
     }
+    long long prof_miss[KK_COUNT] = {0};
     bool profiling = false;
 };
 
@@ -874,10 +888,10 @@ struct Batch : BatchBase {
             bl[i] = (unsigned int *)x->ctl;
             const int nrb = (v.T.nrot + 3) / 4;
             pre[0 * (n + 1) + i + 1] = pre[0 * (n + 1) + i] + pad8(v.ncb);
-            pre[1 * (n + 1) + i + 1] = pre[1 * (n + 1) + i] + pad8(v.ncb + nrb);
-            pre[2 * (n + 1) + i + 1] = pre[2 * (n + 1) + i] + pad8((x->nf + 255) / 256);
-            pre[3 * (n + 1) + i + 1] = pre[3 * (n + 1) + i] + x->nb;
-            pre[4 * (n + 1) + i + 1] = pre[4 * (n + 1) + i] + (v.nzero + 255) / 256;
+            pre[1 * (n + 1) + i + 1] = pre[1 * (n + 1) + i] + pad8((x->nf + 255) / 256 + x->nb);
+            pre[2 * (n + 1) + i + 1] = pre[2 * (n + 1) + i] + x->nb;
+            pre[3 * (n + 1) + i + 1] = pre[3 * (n + 1) + i] + (v.nzero + 255) / 256;
+            pre[4 * (n + 1) + i + 1] = pre[4 * (n + 1) + i] + nrb;
         }
         if (n) {
             HIP_TRY(hipMemcpy(views, hv.data(), n * sizeof(SlotView<R>), hipMemcpyHostToDevice));
@@ -888,10 +902,10 @@ struct Batch : BatchBase {
         }
         plan = BatchPlan{};
         plan.nslot = (int)n; plan.scheme = scheme;
-        plan.cell = prefix; plan.theta = prefix + (n + 1); plan.face = prefix + 2 * (n + 1);
-        plan.bnd = prefix + 3 * (n + 1); plan.zero = prefix + 4 * (n + 1);
-        plan.ncell = pre[n]; plan.ntheta = pre[(n + 1) + n]; plan.nface = pre[2 * (n + 1) + n];
-        plan.nbnd = pre[3 * (n + 1) + n]; plan.nzero = pre[4 * (n + 1) + n];
+        plan.cell = prefix; plan.face = prefix + (n + 1);
+        plan.bnd = prefix + 2 * (n + 1); plan.zero = prefix + 3 * (n + 1); plan.rot = prefix + 4 * (n + 1);
+        plan.ncell = pre[n]; plan.nface = pre[(n + 1) + n];
+        plan.nbnd = pre[2 * (n + 1) + n]; plan.nzero = pre[3 * (n + 1) + n]; plan.nrot = pre[4 * (n + 1) + n];
         dirty = false; dirty_layout = false;
         if (n == 0) return BRAIDS_OK;
         HIP_TRY(hipStreamBeginCapture(stream, hipStreamCaptureModeThreadLocal));

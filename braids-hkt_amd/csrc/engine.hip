@@ -206,7 +206,8 @@ __global__ void k_hllc_limit(Geo g, double dry, const double *q, const double *t
     if (f >= nf) return;
     double mass, d1, d2;
     bool changed, corr;
-    status[f] = braids::hl::limit(g, f, dry, q, q + n, q + 2 * n, theta, fl[f], mass, d1, d2, changed, corr);
+    const auto theta_of = [theta](int c) { return theta[c]; };
+    status[f] = braids::hl::limit(g, f, dry, q, q + n, q + 2 * n, theta_of, fl[f], mass, d1, d2, changed, corr);
     if (!changed) return;
     fl[f] = fr[f] = mass;
     if (corr) {

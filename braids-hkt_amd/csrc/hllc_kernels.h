@@ -11,10 +11,11 @@
 //                   a rotor array; extra workgroups: the first-order traces
 //                   of qhalf at t + dt/2 per open boundary, then the exterior
 //                   flux of its faces (HLLC between the owner's predictor
-//                   state and the trace)
-//   kh_budget       cells: donor factor and face-sum frequency (max-reduced,
-//                   NaN kept); extra workgroups: the rotors sample qhalf
-//   kh_update       cells: hllc_limit applied per incident face on the fly,
+//                   state and the trace), and one wavefront per rotor
+//                   sampling qhalf
+//   kh_update       cells: donor factor and face-sum frequency (max-reduced,
+//                   NaN kept; factors of the workgroup's cells in LDS), then
+//                   hllc_limit applied per incident face on the fly,
 //                   hllc_update, the rotor sink at the half time, the closing
 //                   drag half-step
 //   k_close         inward water of the open boundaries; the last workgroup
@@ -48,14 +49,17 @@ __device__ __forceinline__ void block_bits_max_to(unsigned long long v, unsigned
     }
 }
 
-static __global__ void __launch_bounds__(256) kh_reconstruct(HllcView H, MeshView<double> m, StateView<double> work,
+template <bool ROTORS>
+__global__ void __launch_bounds__(256) kh_reconstruct(HllcView H, MeshView<double> m, StateView<double> work,
                                                       const double *__restrict__ z, StateView<double> qh,
                                                       TurbineView T, double gravity, double dry, double dt, int ncb,
                                                       int remap)
 {
     if ((int)blockIdx.x >= ncb) {
-        const int a = ((int)blockIdx.x - ncb) * (blockDim.x >> 6) + (threadIdx.x >> 6);
-        if (a < T.nrot) turbine_sample_wave<double>(T, m, work, z, 0, a, threadIdx.x & 63);
+        if constexpr (ROTORS) {
+            const int a = ((int)blockIdx.x - ncb) * (blockDim.x >> 6) + (threadIdx.x >> 6);
+            if (a < T.nrot) turbine_sample_wave<double>(T, m, work, z, 0, a, threadIdx.x & 63);
+        }
         return;
     }
     const int lb = remap ? xcd_order(blockIdx.x, ncb) : (int)blockIdx.x;
@@ -134,11 +138,22 @@ __device__ __forceinline__ void hllc_open_face(const HllcView &H, const StateVie
 
 // Faces; workgroups past the nfb face blocks each take one open boundary:
 // its first-order traces of qhalf at the half time (k_traces_body), then the
-// exterior fluxes of its faces.  The face blocks skip open faces.
-static __global__ void __launch_bounds__(256) kh_faces(HllcView H, MeshView<double> m, StateView<double> qh,
-                                                       const double *__restrict__ z, Work<double> w, Control *ctl,
-                                                       double t, double gravity, double dry, int nfb, int remap)
+// exterior fluxes of its faces; with ROTORS, one wavefront per rotor samples
+// qhalf past those.  The face blocks skip open faces.
+template <bool ROTORS>
+__global__ void __launch_bounds__(256) kh_faces(HllcView H, MeshView<double> m, StateView<double> qh,
+                                                const double *__restrict__ z, Work<double> w, Control *ctl,
+                                                TurbineView T, double t, double gravity, double dry, int nfb,
+                                                int remap)
 {
+    if ((int)blockIdx.x >= nfb + m.nb) {
+        if constexpr (ROTORS) {
+            // the rotors sample qhalf: the momentum source of the corrector
+            const int a = ((int)blockIdx.x - nfb - m.nb) * (blockDim.x >> 6) + (threadIdx.x >> 6);
+            if (a < T.nrot) turbine_sample_wave<double>(T, m, qh, z, 1, a, threadIdx.x & 63);
+        }
+        return;
+    }
     if ((int)blockIdx.x >= nfb) {
         const int b = (int)blockIdx.x - nfb;
         k_traces_body<double, 1>(m, qh, z, w, ctl, 0, t, b);
@@ -158,35 +173,38 @@ static __global__ void __launch_bounds__(256) kh_faces(HllcView H, MeshView<doub
     H.F[3 * nf + f] = b[1]; H.F[4 * nf + f] = b[2]; H.F[5 * nf + f] = s;
 }
 
-static __global__ void __launch_bounds__(256) kh_budget(HllcView H, MeshView<double> m, StateView<double> work,
-                                                 StateView<double> qh, const double *__restrict__ z,
-                                                 unsigned long long *peak_slots, TurbineView T, double dt, int ncb,
-                                                 int remap)
-{
-    if ((int)blockIdx.x >= ncb) {
-        const int a = ((int)blockIdx.x - ncb) * (blockDim.x >> 6) + (threadIdx.x >> 6);
-        if (a < T.nrot) turbine_sample_wave<double>(T, m, qh, z, 1, a, threadIdx.x & 63);
-        return;
-    }
-    const int lb = remap ? xcd_order(blockIdx.x, ncb) : (int)blockIdx.x;
-    const int i = lb * blockDim.x + threadIdx.x;
-    unsigned long long bits = 0;
-    if (i < H.g.n) {
-        double th, fr;
-        hl::budget(H.g, i, dt, work.h[i], H.F, H.F + 5 * (size_t)H.g.nf, H.pstat[i], th, fr);
-        H.theta[i] = th;
-        bits = (unsigned long long)__double_as_longlong(fr);
-    }
-    block_bits_max_to(bits, peak_slots);
-}
-
+// hllc_budget of the workgroup's 256 consecutive cells (factor kept in LDS,
+// frequency max-reduced, NaN kept), then hllc_limit applied per incident face
+// on the fly (a donor outside the workgroup has its factor recomputed from its
+// faces), hllc_update, the rotor sink at the half time and the closing drag.
 static __global__ void __launch_bounds__(256) kh_update(HllcView H, StateView<double> work, StateView<double> qh,
                                                  StateView<double> out, TurbineView T, DragView d, Control *ctl,
-                                                 double gravity, double dry, double dt, int remap)
+                                                 unsigned long long *peak_slots, double gravity, double dry,
+                                                 double dt, int remap)
 {
-    const int i = logical_block(remap) * blockDim.x + threadIdx.x;
+    __shared__ double th_s[256];
+    const int base = logical_block(remap) * blockDim.x;
+    const int i = base + threadIdx.x;
     const hl::Geo &g = H.g;
     const int nf = g.nf;
+    const double *mass_f = H.F, *speed_f = H.F + 5 * (size_t)nf;
+    double thi = 1.;
+    unsigned long long bits = 0;
+    if (i < g.n) {
+        double fr;
+        hl::budget(g, i, dt, work.h[i], mass_f, speed_f, H.pstat[i], thi, fr);
+        bits = (unsigned long long)__double_as_longlong(fr);
+    }
+    th_s[threadIdx.x] = thi;
+    block_bits_max_to(bits, peak_slots);
+    __syncthreads();
+    const auto theta_of = [&](int c) {
+        const int r = c - base;
+        if (r >= 0 && r < (int)blockDim.x) return th_s[r];
+        double th, fr;
+        hl::budget(g, c, dt, work.h[c], mass_f, speed_f, H.pstat[c], th, fr);
+        return th;
+    };
     int fail = 0;
     if (i < g.n) {
         double v0 = work.h[i], v1 = work.hu[i], v2 = work.hv[i];
@@ -196,7 +214,8 @@ static __global__ void __launch_bounds__(256) kh_update(HllcView H, StateView<do
             const int f = g.faces[k], s = g.signs[k];
             double mass, d1, d2;
             bool changed, corr;
-            const int st = hl::limit(g, f, dry, work.h, work.hu, work.hv, H.theta, H.F[f], mass, d1, d2, changed, corr);
+            const int st = hl::limit(g, f, dry, work.h, work.hu, work.hv, theta_of, H.F[f], mass, d1, d2, changed,
+                                     corr);
             double F0 = H.F[f];
             double F1 = s > 0 ? H.F[nf + f] : H.F[3 * nf + f];
             double F2 = s > 0 ? H.F[2 * nf + f] : H.F[4 * nf + f];

@@ -257,20 +257,23 @@ __device__ __forceinline__ void budget(const Geo &g, int i, double dt, double de
         theta = fmin(1., (stock - prescribed) / internal) * (1. - 32. * 2.2204460492503131e-16);
 }
 
-// hllc_limit for one face (hllc.h:136-141).  water = fl(1,f).  Returns the
-// face status.  When the donor's factor acts (changed), *mass is the limited
-// water of both sides; when moreover the donor is wet (corr), d1 and d2 are
-// the momentum corrections added to both sides.
+// hllc_limit for one face (hllc.h:136-141).  water = fl(1,f); theta_of(c)
+// gives the donor factor of cell c.  Returns the face status.  When the
+// donor's factor acts (changed), *mass is the limited water of both sides;
+// when moreover the donor is wet (corr), d1 and d2 are the momentum
+// corrections added to both sides.
+template <class TH>
 __device__ __forceinline__ int limit(const Geo &g, int f, double dry, const double *h, const double *hu,
-                                     const double *hv, const double *theta, double water, double &mass, double &d1,
+                                     const double *hv, const TH &theta_of, double water, double &mass, double &d1,
                                      double &d2, bool &changed, bool &corr)
 {
     changed = false; corr = false; mass = water; d1 = 0.; d2 = 0.;
     const int donor = water < 0. ? g.neighbour[f] : g.owner[f];
     if (donor < 0 || water == 0.) return 0;
-    if (g.selected[f]) return theta[donor] < 0. ? 1 : 0;
-    if (theta[donor] >= 1.) return 0;
-    const double factor = fmax(0., theta[donor]);
+    const double th = theta_of(donor);
+    if (g.selected[f]) return th < 0. ? 1 : 0;
+    if (th >= 1.) return 0;
+    const double factor = fmax(0., th);
     const double delta = (factor - 1.) * water;
     mass = water * factor;
     changed = true;

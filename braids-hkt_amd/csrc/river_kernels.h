@@ -11,15 +11,18 @@
 // limiter of the stage input (river_midpoint.h:13-41) and the turbine sink
 // (river_turbine_thetis.py:263-362).
 //
-// On the device a stage is: k_prep, k_traces, k_face (shared with the
-// SSPRK(3,3) path), then
-//   k_river_theta   cells: inventory factor theta and the CFL frequency;
-//                   extra workgroups: one wavefront per rotor (gating
-//                   statistics, Ct/Cp lookup, induction, K per footprint cell)
-//   k_river_update  cells: the native rate, the limiter applied to each
-//                   incident face on the fly, the turbine sink, the SSP
-//                   combination, positivity, dry momentum and (last stage)
-//                   the drag.
+// On the device a stage is k_prep (shared with the SSPRK(3,3) path), then
+//   k_face_traces   faces (shared with SSPRK(3,3)); extra workgroups: the
+//                   open-boundary traces and their faces, and one wavefront
+//                   per rotor (gating statistics, Ct/Cp lookup, induction, K
+//                   per footprint cell)
+//   k_river_update  cells: the inventory factor theta of the cell (kept in
+//                   LDS for the workgroup) and the CFL frequency, then the
+//                   native rate with the limiter applied to each incident
+//                   face on the fly (a donor outside the workgroup has its
+//                   theta recomputed from its faces), the turbine sink, the
+//                   SSP combination, positivity, dry momentum and (last
+//                   stage) the drag.
 // Nothing is synchronised with the host inside a substep: rejections are
 // flags in the Control block, read once per substep.
 #pragma once
@@ -227,61 +230,101 @@ __device__ __forceinline__ void turbine_sink(const TurbineView &T, int u, double
     }
 }
 
-// midpoint_inventory (river_midpoint.h:13-21) with the stage input as the
-// available inventory, and explicit_frequency (river_explicit.h:13-25, no
-// exterior owner) max-reduced into the stage's CFL slot.
+// midpoint_inventory (river_midpoint.h:13-21) for one cell with the stage
+// input as the available inventory.
 template <class R>
-__device__ __forceinline__ void k_river_theta_body(MeshView<R> m, StateView<R> q, Work<R> w, double *theta, int slot,
-                                                   double dt, int lb)
+__device__ __forceinline__ double river_theta(const MeshView<R> &m, const Work<R> &w, const StateView<R> &q, int c,
+                                              double dt)
 {
-    const int nf = m.nf;
-    const int c = lb * blockDim.x + threadIdx.x;
-    double freq = 0.;
-    if (c < m.nc) {
-        double out = 0., sum = 0.;
-        for (int k = m.offset[c]; k < m.offset[c + 1]; ++k) {
-            const int f = m.adj_face[k];
-            const double sg = (m.adj_flags[k] & ADJ_OWNER) ? 1. : -1.;
-            const double L = (double)m.length[f];
-            out += fmax(sg * (double)w.F[f], 0.) * L;
-            sum += L * (double)w.F[5 * nf + f];
-        }
-        const double A = (double)m.area[c], hA = q.h[c] * A;
-        double th = 1.;
-        if (out > 0. && dt * out >= hA) th = fmin(1., hA / (dt * out)) * (1. - 32. * 2.2204460492503131e-16);
-        theta[c] = th;
-        freq = sum / A;
-        if (!(freq == freq)) freq = 1e300;     // a NaN frequency rejects the trial
+    double out = 0.;
+    for (int k = m.offset[c]; k < m.offset[c + 1]; ++k) {
+        const int f = m.adj_face[k];
+        const double sg = (m.adj_flags[k] & ADJ_OWNER) ? 1. : -1.;
+        out += fmax(sg * (double)w.F[f], 0.) * (double)m.length[f];
     }
-    block_max_to(freq, w.rpart + rpart_index(RQ_PEAK, slot));
+    const double hA = q.h[c] * (double)m.area[c];
+    double th = 1.;
+    if (out > 0. && dt * out >= hA) th = fmin(1., hA / (dt * out)) * (1. - 32. * 2.2204460492503131e-16);
+    return th;
 }
 
-
-// Workgroups past the ncb cell blocks sample the rotors, one wavefront each.
-template <class R>
-__global__ void __launch_bounds__(256) k_river_theta(MeshView<R> m, StateView<R> q, Work<R> w, double *theta,
-                                                     Control *ctl, int slot, double dt, int remap,
-                                                     TurbineView T, const double *__restrict__ z, int ncb)
+// One workgroup of a stage's face launch, lb its index within the launch
+// (or the scenario's range of it): [0, nfb) faces (open faces skipped),
+// [nfb, nfb + nb) one open boundary each - its traces (k_traces_body), then
+// the fluxes of its faces - and, with ROTORS, one wavefront per rotor
+// sampling the stage input past those.  (The rotor path raises the launch's
+// register demand, so launches without rotors use ROTORS = false.)
+template <class R, int SCHEME, bool ROTORS>
+__device__ __forceinline__ void face_launch_block(const MeshView<R> &m, const StateView<R> &q,
+                                                  const double *__restrict__ z, const Work<R> &w, Control *ctl,
+                                                  const TurbineView &T, int stage, double t, int nfb, int lb,
+                                                  int remap)
 {
-    (void)ctl;
-    if ((int)blockIdx.x >= ncb) {
-        const int a = ((int)blockIdx.x - ncb) * (blockDim.x >> 6) + (threadIdx.x >> 6);
-        if (a < T.nrot) turbine_sample_wave<R>(T, m, q, z, slot, a, threadIdx.x & 63);
+    if (lb >= nfb + m.nb) {
+        if constexpr (ROTORS) {
+            const int a = (lb - nfb - m.nb) * (blockDim.x >> 6) + (threadIdx.x >> 6);
+            if (a < T.nrot) turbine_sample_wave<R>(T, m, q, z, stage, a, threadIdx.x & 63);
+        }
         return;
     }
-    k_river_theta_body<R>(m, q, w, theta, slot, dt, remap ? xcd_order(blockIdx.x, ncb) : (int)blockIdx.x);
+    if (lb >= nfb) {
+        const int b = lb - nfb;
+        k_traces_body<R, SCHEME>(m, q, z, w, ctl, stage, t, b);
+        __syncthreads();
+        for (int k = m.bnd_offset[b] + threadIdx.x; k < m.bnd_offset[b + 1]; k += blockDim.x) {
+            if constexpr (std::is_same<R, float>::value) face_one_f32<SCHEME>(m, q, z, w, m.bnd_faces[k]);
+            else face_one<R, SCHEME>(m, q, z, w, m.bnd_faces[k]);
+        }
+        return;
+    }
+    const int f = (remap ? xcd_order(lb, nfb) : lb) * blockDim.x + threadIdx.x;
+    if (f >= m.nf || m.bid[f] >= 0) return;
+    if constexpr (std::is_same<R, float>::value) face_one_f32<SCHEME>(m, q, z, w, f);
+    else face_one<R, SCHEME>(m, q, z, w, f);
+}
+
+template <class R, int SCHEME, bool ROTORS>
+__global__ void __launch_bounds__(256) k_face_traces(MeshView<R> m, StateView<R> q, const double *__restrict__ z,
+                                                     Work<R> w, Control *ctl, TurbineView T, int stage, double t,
+                                                     int nfb, int remap)
+{
+    face_launch_block<R, SCHEME, ROTORS>(m, q, z, w, ctl, T, stage, t, nfb, blockIdx.x, remap);
 }
 
 // STAGE 0: out = in + dt r (in is the dragged base);  STAGE 1: out = in0/2 +
 // (in + dt r)/2 with in0 the dragged base, then the closing drag half-step.
+// First the cell's own inventory factor theta (kept in LDS for the
+// workgroup's 256 consecutive cells) and its face-sum frequency
+// (explicit_frequency, river_explicit.h:13-25, no exterior owner),
+// max-reduced into the stage's CFL slot; a NaN frequency rejects the trial.
 template <class R, int STAGE>
 __device__ __forceinline__ void k_river_update_body(MeshView<R> m, StateView<R> in, StateView<R> in0,
-                                                      StateView<R> out, Work<R> w, const double *__restrict__ theta,
-                                                      TurbineView T, double *wlim, DragView d, Control *ctl,
-                                                      int slot, double dt, double drag_dt, int remap, int lb)
+                                                      StateView<R> out, Work<R> w, TurbineView T, double *wlim,
+                                                      DragView d, Control *ctl, int slot, double dt, double drag_dt,
+                                                      int lb)
 {
+    __shared__ double th_s[256];
     const int nc = m.nc, nf = m.nf;
-    const int c = lb * blockDim.x + threadIdx.x;
+    const int base = lb * blockDim.x;
+    const int c = base + threadIdx.x;
+    double thc = 1., freq = 0.;
+    if (c < nc) {
+        double outv = 0., sum = 0.;
+        for (int k = m.offset[c]; k < m.offset[c + 1]; ++k) {
+            const int f = m.adj_face[k];
+            const double sg = (m.adj_flags[k] & ADJ_OWNER) ? 1. : -1.;
+            const double L = (double)m.length[f];
+            outv += fmax(sg * (double)w.F[f], 0.) * L;
+            sum += L * (double)w.F[5 * nf + f];
+        }
+        const double A = (double)m.area[c], hA = in.h[c] * A;
+        if (outv > 0. && dt * outv >= hA) thc = fmin(1., hA / (dt * outv)) * (1. - 32. * 2.2204460492503131e-16);
+        freq = sum / A;
+        if (!(freq == freq)) freq = 1e300;
+    }
+    th_s[threadIdx.x] = thc;
+    block_max_to(freq, w.rpart + rpart_index(RQ_PEAK, slot));
+    __syncthreads();
     int fail = 0;
     if (c < nc) {
         const double h = in.h[c], dry = (double)m.dry;
@@ -313,10 +356,12 @@ __device__ __forceinline__ void k_river_update_body(MeshView<R> m, StateView<R> 
             const int o = m.owner[f], j = m.neighbour[f];
             if (wl != 0.) {
                 const int donor = wl < 0. ? j : o;
-                if (donor >= 0 && theta[donor] != 1.) {
+                const int r = donor - base;
+                const double th = donor < 0 ? 1. : (r >= 0 && r < (int)blockDim.x ? th_s[r]
+                                                                               : river_theta<R>(m, w, in, donor, dt));
+                if (donor >= 0 && th != 1.) {
                     if (j < 0 && m.bid[f] >= 0) fail = RFAIL_BOUNDARY;
                     else {
-                        const double th = theta[donor];
                         const double delta = (th - 1.) * wl;
                         wl = wl * th;
                         const double hd = in.h[donor];
@@ -355,11 +400,11 @@ __device__ __forceinline__ void k_river_update_body(MeshView<R> m, StateView<R> 
 
 template <class R, int STAGE>
 __global__ void __launch_bounds__(256) k_river_update(MeshView<R> m, StateView<R> in, StateView<R> in0,
-                                                      StateView<R> out, Work<R> w, const double *__restrict__ theta,
-                                                      TurbineView T, double *wlim, DragView d, Control *ctl,
-                                                      int slot, double dt, double drag_dt, int remap)
+                                                      StateView<R> out, Work<R> w, TurbineView T, double *wlim,
+                                                      DragView d, Control *ctl, int slot, double dt, double drag_dt,
+                                                      int remap)
 {
-    k_river_update_body<R, STAGE>(m, in, in0, out, w, theta, T, wlim, d, ctl, slot, dt, drag_dt, remap, logical_block(remap));
+    k_river_update_body<R, STAGE>(m, in, in0, out, w, T, wlim, d, ctl, slot, dt, drag_dt, logical_block(remap));
 }
 
 // Boundary ledger of one substep: ledger[b] = -dt sum_f L (w1 + w2)/2 over

@@ -3,6 +3,7 @@
 #include "swe_launch_impl.h"
 
 namespace braids {
+LaunchFault launch_fault{};
 template void run_stage<double>(const MeshView<double> &, StateView<double>, StateView<double>, StateView<double>,
                                 const double *, Work<double>, Control *, double *, int, double, double,
                                 const LaunchCfg &, hipStream_t);

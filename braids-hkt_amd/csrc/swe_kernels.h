@@ -798,30 +798,7 @@ __global__ void __launch_bounds__(256) k_face_f32(MeshView<float> m, StateView<f
     k_face_f32_body<SCHEME>(m, q, z, w, remap, logical_block(remap));
 }
 
-// The faces of one stage together with the open-boundary traces.  Workgroups
-// past the nfb face blocks each take one open boundary: its traces
-// (k_traces_body), then the fluxes of its faces; the face blocks skip open
-// faces.  One launch instead of the traces launch followed by the faces.
-template <class R, int SCHEME>
-__global__ void __launch_bounds__(256) k_face_traces(MeshView<R> m, StateView<R> q, const double *__restrict__ z,
-                                                     Work<R> w, Control *ctl, int stage, double t, int nfb, int remap)
-{
-    if ((int)blockIdx.x >= nfb) {
-        const int b = (int)blockIdx.x - nfb;
-        k_traces_body<R, SCHEME>(m, q, z, w, ctl, stage, t, b);
-        __syncthreads();
-        for (int k = m.bnd_offset[b] + threadIdx.x; k < m.bnd_offset[b + 1]; k += blockDim.x) {
-            if constexpr (std::is_same<R, float>::value) face_one_f32<SCHEME>(m, q, z, w, m.bnd_faces[k]);
-            else face_one<R, SCHEME>(m, q, z, w, m.bnd_faces[k]);
-        }
-        return;
-    }
-    const int lb = remap ? xcd_order(blockIdx.x, nfb) : (int)blockIdx.x;
-    const int f = lb * blockDim.x + threadIdx.x;
-    if (f >= m.nf || m.bid[f] >= 0) return;
-    if constexpr (std::is_same<R, float>::value) face_one_f32<SCHEME>(m, q, z, w, f);
-    else face_one<R, SCHEME>(m, q, z, w, f);
-}
+
 
 // ---------------------------------------------------------------------------
 template <class R>

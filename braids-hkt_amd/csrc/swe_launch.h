@@ -12,6 +12,11 @@ enum : int { KK_PREP = 0, KK_TRACES, KK_FACE, KK_DRAIN, KK_GATHER, KK_LEDGER,
              KK_RDRAG, KK_RTHETA, KK_TURBINE, KK_RUPDATE,
              KK_HRECON, KK_HFACE, KK_HBUDGET, KK_HUPDATE, KK_COUNT };
 
+// The first timed launch that left an error (kind + 1, 0: none), for the
+// error message of the step that reports it.
+struct LaunchFault { int kind, slot, grid, block; };
+extern LaunchFault launch_fault;
+
 struct LaunchCfg {
     int block = 256;    // threads per workgroup (k_prep/k_face/k_drain/k_gather)
     int remap = 1;      // XCD-aware block remap
@@ -95,10 +100,10 @@ struct SlotRun {
 
 struct BatchPlan {
     int nslot, scheme;
-    int ncell, ntheta, nface, nbnd, nzero;     // grid sizes (block totals)
+    int ncell, nface, nbnd, nzero, nrot;       // grid sizes (block totals)
     const int *cell;            // [nslot+1] prefix of cell blocks (each padded to a multiple of 8)
-    const int *theta;           // cell + rotor blocks
-    const int *face;            // face blocks
+    const int *face;            // face blocks and one block per open boundary (padded to 8)
+    const int *rot;             // rotor blocks (four rotors each)
     const int *bnd;             // boundaries (one block each)
     const int *zero;            // control-clearing blocks
 };

@@ -27,6 +27,10 @@ static inline void timed_launch(const LaunchCfg &cfg, int slot, int kind, K kern
         a = cfg.ev[i]; b = cfg.ev[i + 1];
     }
     hipExtLaunchKernelGGL(kernel, grid, block, 0, s, a, b, 0, args...);
+    if (a && hipPeekAtLastError() != hipSuccess && !launch_fault.kind) {
+        launch_fault.kind = kind + 1; launch_fault.slot = slot;
+        launch_fault.grid = (int)grid.x; launch_fault.block = (int)block.x;
+    }
 }
 
 template <class R, int SCHEME>
@@ -53,14 +57,19 @@ static void launch_face(const MeshView<R> &m, StateView<R> in, const double *z, 
         timed_launch(cfg, slot, KK_FACE, k_face<R, SCHEME>, dim3(grid_for(m.nf, B)), dim3(B), s, m, in, z, w, cfg.remap);
 }
 
-// Traces and faces of one stage in one launch (k_face_traces).
+// Faces, open-boundary traces and (T.nrot > 0) the rotor sampling of one
+// stage in one launch (k_face_traces).
 template <class R, int SCHEME>
 static void launch_face_traces(const MeshView<R> &m, StateView<R> in, const double *z, Work<R> w, Control *ctl,
-                               int slot, double t, const LaunchCfg &cfg, hipStream_t s)
+                               int slot, double t, const LaunchCfg &cfg, hipStream_t s, const TurbineView &T = {})
 {
     const int B = 256, nfb = grid_for(m.nf, B);
-    timed_launch(cfg, slot, KK_FACE, k_face_traces<R, SCHEME>, dim3(nfb + m.nb), dim3(B), s, m, in, z, w, ctl, slot, t,
-                 nfb, cfg.remap);
+    if (T.nrot > 0)
+        timed_launch(cfg, slot, KK_FACE, k_face_traces<R, SCHEME, true>, dim3(nfb + m.nb + (T.nrot + 3) / 4), dim3(B),
+                     s, m, in, z, w, ctl, T, slot, t, nfb, cfg.remap);
+    else
+        timed_launch(cfg, slot, KK_FACE, k_face_traces<R, SCHEME, false>, dim3(nfb + m.nb), dim3(B), s, m, in, z, w,
+                     ctl, T, slot, t, nfb, cfg.remap);
 }
 
 template <class R, int SCHEME>
@@ -128,24 +137,18 @@ void run_rates(const MeshView<R> &m, StateView<R> in, const double *z, Work<R> w
 
 template <class R, int SCHEME>
 static void river_stage(const MeshView<R> &m, StateView<R> in, StateView<R> in0, StateView<R> out, const double *z,
-                        Work<R> w, Control *ctl, const RiverBuffers &rb, const TurbineView *tv, int slot,
+                        Work<R> w, Control *ctl, const RiverBuffers &rb, const TurbineView &T, int slot,
                         double dt, double t, const LaunchCfg &cfg, hipStream_t s)
 {
-    const int B = cfg.block;
-    const dim3 gc(grid_for(m.nc, B)), blk(B), gb(m.nb), tb(TRACE_BLOCK);
     launch_prep<R, SCHEME>(m, in, z, w, ctl, slot, cfg, s, 0);     // the river path uses no max depth
-    launch_face_traces<R, SCHEME>(m, in, z, w, ctl, slot, t, cfg, s);
-    TurbineView T{};
-    if (tv && tv->nrot > 0) { T = *tv; T.error = &ctl->turbine_error; }
-    const int ncb = grid_for(m.nc, B), nrb = (T.nrot + (B / 64) - 1) / (B / 64);
-    timed_launch(cfg, slot, KK_RTHETA, k_river_theta<R>, dim3(ncb + nrb), blk, s, m, in, w, rb.theta, ctl, slot, dt,
-                 cfg.remap, T, z, ncb);
+    launch_face_traces<R, SCHEME>(m, in, z, w, ctl, slot, t, cfg, s, T);
+    const dim3 gu(grid_for(m.nc, 256)), bu(256);      // k_river_update keeps 256 thetas in LDS
     if (slot == 0)
-        timed_launch(cfg, slot, KK_RUPDATE, k_river_update<R, 0>, gc, blk, s, m, in, in0, out, w,
-                     (const double *)rb.theta, T, rb.wlim[0], rb.drag, ctl, slot, dt, 0.0, cfg.remap);
+        timed_launch(cfg, slot, KK_RUPDATE, k_river_update<R, 0>, gu, bu, s, m, in, in0, out, w, T, rb.wlim[0],
+                     rb.drag, ctl, slot, dt, 0.0, cfg.remap);
     else
-        timed_launch(cfg, slot, KK_RUPDATE, k_river_update<R, 1>, gc, blk, s, m, in, in0, out, w,
-                     (const double *)rb.theta, T, rb.wlim[1], rb.drag, ctl, slot, dt, .5 * dt, cfg.remap);
+        timed_launch(cfg, slot, KK_RUPDATE, k_river_update<R, 1>, gu, bu, s, m, in, in0, out, w, T, rb.wlim[1],
+                     rb.drag, ctl, slot, dt, .5 * dt, cfg.remap);
 }
 
 template <class R>
@@ -157,14 +160,16 @@ void run_river_substep(const MeshView<R> &m, StateView<R> base, StateView<R> wor
     const int B = cfg.block;
     const int gz = grid_for(nzero, B), gcell = grid_for(m.nc, B);
     const dim3 gc(gcell > gz ? gcell : gz), blk(B);
+    TurbineView T{};
+    if (tv && tv->nrot > 0) { T = *tv; T.error = &ctl->turbine_error; }
     timed_launch(cfg, 0, KK_RDRAG, k_river_drag<R>, gc, blk, s, m.nc, base, work, rb.drag, .5 * dt,
                  (double)m.grav, (double)m.dry, cfg.remap, zero, nzero);
     if (m.scheme == 5) {
-        river_stage<R, 5>(m, work, work, q1, z, w, ctl, rb, tv, 0, dt, t, cfg, s);
-        river_stage<R, 5>(m, q1, work, out, z, w, ctl, rb, tv, 1, dt, t + dt, cfg, s);
+        river_stage<R, 5>(m, work, work, q1, z, w, ctl, rb, T, 0, dt, t, cfg, s);
+        river_stage<R, 5>(m, q1, work, out, z, w, ctl, rb, T, 1, dt, t + dt, cfg, s);
     } else {
-        river_stage<R, 1>(m, work, work, q1, z, w, ctl, rb, tv, 0, dt, t, cfg, s);
-        river_stage<R, 1>(m, q1, work, out, z, w, ctl, rb, tv, 1, dt, t + dt, cfg, s);
+        river_stage<R, 1>(m, work, work, q1, z, w, ctl, rb, T, 0, dt, t, cfg, s);
+        river_stage<R, 1>(m, q1, work, out, z, w, ctl, rb, T, 1, dt, t + dt, cfg, s);
     }
     hipLaunchKernelGGL((k_close<R>), dim3(m.nb > 0 ? m.nb : 1), dim3(TRACE_BLOCK), 0, s, m,
                        (const double *)rb.wlim[0], (const double *)rb.wlim[1], rb.ledger, dt,
@@ -183,22 +188,28 @@ void run_hllc_substep(const MeshView<R> &m, StateView<R> base, StateView<R> work
     const int ncb = grid_for(m.nc, B);
     TurbineView T{};
     if (tv && tv->nrot > 0) { T = *tv; T.error = &ctl->turbine_error; }
-    const int nrb = (T.nrot + (B / 64) - 1) / (B / 64);
     const double g = (double)m.grav, dry = (double)m.dry;
     const int gz = grid_for(nzero, B);
     timed_launch(cfg, 0, KK_RDRAG, k_river_drag<R>, dim3(ncb > gz ? ncb : gz), blk, s, m.nc, base, work, rb.drag,
                  .5 * dt, g, dry, cfg.remap, zero, nzero);
-    timed_launch(cfg, 0, KK_HRECON, kh_reconstruct, dim3(ncb + nrb), blk, s, H, m, work, z, qh, T, g, dry, dt, ncb,
-                 cfg.remap);
-    if (H.momentum && T.nunion > 0)
-        hipLaunchKernelGGL(kh_predict, dim3(grid_for(T.nunion, B)), blk, 0, s, H, work, qh, T, dry, dt);
-    const int nfb = grid_for(m.nf, B);
-    timed_launch(cfg, 0, KK_HFACE, kh_faces, dim3(nfb + m.nb), blk, s, H, m, qh, z, w, ctl, t + .5 * dt, g, dry, nfb,
-                 cfg.remap);
-    timed_launch(cfg, 0, KK_HBUDGET, kh_budget, dim3(ncb + nrb), blk, s, H, m, work, qh, z,
-                 w.rpart + rpart_index(RQ_PEAK, 0), T, dt, ncb, cfg.remap);
-    timed_launch(cfg, 0, KK_HUPDATE, kh_update, dim3(ncb), blk, s, H, work, qh, out, T, rb.drag, ctl, g, dry, dt,
-                 cfg.remap);
+    const int nrb = (T.nrot + 3) / 4, nfb = grid_for(m.nf, B);
+    if (nrb > 0) {
+        // rotors sample `work` (the predictor's momentum source) and then qhalf
+        // (the corrector's) in extra workgroups of the reconstruction and face launches
+        timed_launch(cfg, 0, KK_HRECON, kh_reconstruct<true>, dim3(ncb + nrb), blk, s, H, m, work, z, qh, T, g, dry,
+                     dt, ncb, cfg.remap);
+        if (H.momentum && T.nunion > 0)
+            hipLaunchKernelGGL(kh_predict, dim3(grid_for(T.nunion, B)), blk, 0, s, H, work, qh, T, dry, dt);
+        timed_launch(cfg, 0, KK_HFACE, kh_faces<true>, dim3(nfb + m.nb + nrb), blk, s, H, m, qh, z, w, ctl, T,
+                     t + .5 * dt, g, dry, nfb, cfg.remap);
+    } else {
+        timed_launch(cfg, 0, KK_HRECON, kh_reconstruct<false>, dim3(ncb), blk, s, H, m, work, z, qh, T, g, dry, dt,
+                     ncb, cfg.remap);
+        timed_launch(cfg, 0, KK_HFACE, kh_faces<false>, dim3(nfb + m.nb), blk, s, H, m, qh, z, w, ctl, T, t + .5 * dt,
+                     g, dry, nfb, cfg.remap);
+    }
+    timed_launch(cfg, 0, KK_HUPDATE, kh_update, dim3(ncb), blk, s, H, work, qh, out, T, rb.drag, ctl,
+                 w.rpart + rpart_index(RQ_PEAK, 0), g, dry, dt, cfg.remap);
     hipLaunchKernelGGL((k_close<R>), dim3(m.nb > 0 ? m.nb : 1), dim3(TRACE_BLOCK), 0, s, m, (const double *)H.F,
                        (const double *)H.F, rb.ledger, dt, (const unsigned long long *)w.rpart, ctl, hm);
 }
@@ -207,10 +218,9 @@ template <class R, int SCHEME>
 static void batch_stage(const BatchPlan &P, const SlotView<R> *V, SlotRun *run, int stage, hipStream_t s)
 {
     hipLaunchKernelGGL((kb_prep<R, SCHEME>), dim3(P.ncell), dim3(256), 0, s, P, V, (const SlotRun *)run, stage);
-    if (P.nbnd > 0)
-        hipLaunchKernelGGL((kb_traces<R, SCHEME>), dim3(P.nbnd), dim3(TRACE_BLOCK), 0, s, P, V, (const SlotRun *)run, stage);
+    if (P.nrot > 0)
+        hipLaunchKernelGGL((kb_rotors<R>), dim3(P.nrot), dim3(256), 0, s, P, V, (const SlotRun *)run, stage);
     hipLaunchKernelGGL((kb_face<R, SCHEME>), dim3(P.nface), dim3(256), 0, s, P, V, (const SlotRun *)run, stage);
-    hipLaunchKernelGGL((kb_theta<R>), dim3(P.ntheta), dim3(256), 0, s, P, V, (const SlotRun *)run, stage);
     if (stage == 0) hipLaunchKernelGGL((kb_update<R, 0>), dim3(P.ncell), dim3(256), 0, s, P, V, (const SlotRun *)run);
     else hipLaunchKernelGGL((kb_update<R, 1>), dim3(P.ncell), dim3(256), 0, s, P, V, (const SlotRun *)run);
 }
