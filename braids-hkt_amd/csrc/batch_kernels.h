@@ -58,19 +58,19 @@ __global__ void __launch_bounds__(256) kb_prep(BatchPlan P, const SlotView<R> *V
         k_prep_body<R, SCHEME>(v.m, in, v.z, v.w, v.ctl, stage, 1, 0, xcd_order(lb, v.ncb));
 }
 
-// Per scenario: its face blocks and one block per open boundary (traces and
-// their faces) (face_launch_block).
+// Per scenario: one block per open boundary (traces and their faces), padded
+// to 8, then its face blocks (face_launch_block).
 template <class R, int SCHEME>
 __global__ void __launch_bounds__(256) kb_face(BatchPlan P, const SlotView<R> *V, const SlotRun *run, int stage)
 {
     const int s = find_slot(P.face, P.nslot, blockIdx.x);
     if (!run[s].active) return;
     const SlotView<R> &v = V[s];
-    const int nfb = (v.m.nf + 255) >> 8, lb = blockIdx.x - P.face[s];
-    if (lb >= nfb + v.m.nb) return;
+    const int nfb = (v.m.nf + 255) >> 8, lead = face_lead(v.m.nb, 0), lb = blockIdx.x - P.face[s];
+    if (lb >= lead + nfb) return;
     const StateView<R> in = stage == 0 ? v.work : v.q1;
     const double t = run[s].time + (stage ? run[s].hint : 0.0);
-    face_launch_block<R, SCHEME, false>(v.m, in, v.z, v.w, v.ctl, v.T, stage, t, nfb, lb, 1);
+    face_launch_block<R, SCHEME, false>(v.m, in, v.z, v.w, v.ctl, v.T, stage, t, nfb, lead, lb, 1);
 }
 
 // The rotors of every scenario sampling its stage input (one wavefront each;

@@ -888,7 +888,7 @@ struct Batch : BatchBase {
             bl[i] = (unsigned int *)x->ctl;
             const int nrb = (v.T.nrot + 3) / 4;
             pre[0 * (n + 1) + i + 1] = pre[0 * (n + 1) + i] + pad8(v.ncb);
-            pre[1 * (n + 1) + i + 1] = pre[1 * (n + 1) + i] + pad8((x->nf + 255) / 256 + x->nb);
+            pre[1 * (n + 1) + i + 1] = pre[1 * (n + 1) + i] + pad8(face_lead(x->nb, 0) + (x->nf + 255) / 256);
             pre[2 * (n + 1) + i + 1] = pre[2 * (n + 1) + i] + x->nb;
             pre[3 * (n + 1) + i + 1] = pre[3 * (n + 1) + i] + (v.nzero + 255) / 256;
             pre[4 * (n + 1) + i + 1] = pre[4 * (n + 1) + i] + nrb;

@@ -49,20 +49,23 @@ __device__ __forceinline__ void block_bits_max_to(unsigned long long v, unsigned
     }
 }
 
+// The first `lead` workgroups (a multiple of 8) sample the rotors with ROTORS;
+// the cell workgroups follow.
 template <bool ROTORS>
 __global__ void __launch_bounds__(256) kh_reconstruct(HllcView H, MeshView<double> m, StateView<double> work,
                                                       const double *__restrict__ z, StateView<double> qh,
                                                       TurbineView T, double gravity, double dry, double dt, int ncb,
-                                                      int remap)
+                                                      int lead, int remap)
 {
-    if ((int)blockIdx.x >= ncb) {
+    if ((int)blockIdx.x < lead) {
         if constexpr (ROTORS) {
-            const int a = ((int)blockIdx.x - ncb) * (blockDim.x >> 6) + (threadIdx.x >> 6);
+            const int a = (int)blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
             if (a < T.nrot) turbine_sample_wave<double>(T, m, work, z, 0, a, threadIdx.x & 63);
         }
         return;
     }
-    const int lb = remap ? xcd_order(blockIdx.x, ncb) : (int)blockIdx.x;
+    const int cb = (int)blockIdx.x - lead;
+    const int lb = remap ? xcd_order(cb, ncb) : cb;
     const int i = lb * blockDim.x + threadIdx.x;
     const int n = H.g.n;
     if (i >= n) return;
@@ -136,33 +139,33 @@ __device__ __forceinline__ void hllc_open_face(const HllcView &H, const StateVie
     H.F[3 * nf + f] = a[1]; H.F[4 * nf + f] = a[2]; H.F[5 * nf + f] = s;
 }
 
-// Faces; workgroups past the nfb face blocks each take one open boundary:
-// its first-order traces of qhalf at the half time (k_traces_body), then the
-// exterior fluxes of its faces; with ROTORS, one wavefront per rotor samples
-// qhalf past those.  The face blocks skip open faces.
+// Faces.  The first `lead` workgroups (a multiple of 8) go first: one per open
+// boundary - its first-order traces of qhalf at the half time
+// (k_traces_body), then the exterior fluxes of its faces - and, with ROTORS,
+// one wavefront per rotor sampling qhalf.  The face workgroups follow and
+// skip open faces.
 template <bool ROTORS>
 __global__ void __launch_bounds__(256) kh_faces(HllcView H, MeshView<double> m, StateView<double> qh,
                                                 const double *__restrict__ z, Work<double> w, Control *ctl,
                                                 TurbineView T, double t, double gravity, double dry, int nfb,
-                                                int remap)
+                                                int lead, int remap)
 {
-    if ((int)blockIdx.x >= nfb + m.nb) {
-        if constexpr (ROTORS) {
+    if ((int)blockIdx.x < lead) {
+        const int b = (int)blockIdx.x;
+        if (b < m.nb) {
+            k_traces_body<double, 1>(m, qh, z, w, ctl, 0, t, b);
+            __syncthreads();
+            for (int k = m.bnd_offset[b] + threadIdx.x; k < m.bnd_offset[b + 1]; k += blockDim.x)
+                hllc_open_face(H, qh, w.trace, m.bnd_faces[k], gravity, dry);
+        } else if constexpr (ROTORS) {
             // the rotors sample qhalf: the momentum source of the corrector
-            const int a = ((int)blockIdx.x - nfb - m.nb) * (blockDim.x >> 6) + (threadIdx.x >> 6);
+            const int a = (b - m.nb) * (blockDim.x >> 6) + (threadIdx.x >> 6);
             if (a < T.nrot) turbine_sample_wave<double>(T, m, qh, z, 1, a, threadIdx.x & 63);
         }
         return;
     }
-    if ((int)blockIdx.x >= nfb) {
-        const int b = (int)blockIdx.x - nfb;
-        k_traces_body<double, 1>(m, qh, z, w, ctl, 0, t, b);
-        __syncthreads();
-        for (int k = m.bnd_offset[b] + threadIdx.x; k < m.bnd_offset[b + 1]; k += blockDim.x)
-            hllc_open_face(H, qh, w.trace, m.bnd_faces[k], gravity, dry);
-        return;
-    }
-    const int lb = remap ? xcd_order(blockIdx.x, nfb) : (int)blockIdx.x;
+    const int fb = (int)blockIdx.x - lead;
+    const int lb = remap ? xcd_order(fb, nfb) : fb;
     const int f = lb * blockDim.x + threadIdx.x;
     const int nf = H.g.nf;
     if (f >= nf || H.g.selected[f]) return;

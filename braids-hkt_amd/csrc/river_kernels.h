@@ -249,35 +249,35 @@ __device__ __forceinline__ double river_theta(const MeshView<R> &m, const Work<R
 }
 
 // One workgroup of a stage's face launch, lb its index within the launch
-// (or the scenario's range of it): [0, nfb) faces (open faces skipped),
-// [nfb, nfb + nb) one open boundary each - its traces (k_traces_body), then
-// the fluxes of its faces - and, with ROTORS, one wavefront per rotor
-// sampling the stage input past those.  (The rotor path raises the launch's
-// register demand, so launches without rotors use ROTORS = false.)
+// (or the scenario's range of it).  The first `lead` workgroups (a multiple of
+// 8, so that the face workgroups keep their XCD grouping) are dispatched
+// first and overlap the faces: [0, nb) one open boundary each - its traces
+// (k_traces_body), then the fluxes of its faces - and, with ROTORS, one
+// wavefront per rotor sampling the stage input; the rest of the lead is idle.
+// Then nfb face workgroups (open faces skipped).  The rotor path needs more
+// registers than the faces: launches without rotors use ROTORS = false.
 template <class R, int SCHEME, bool ROTORS>
 __device__ __forceinline__ void face_launch_block(const MeshView<R> &m, const StateView<R> &q,
                                                   const double *__restrict__ z, const Work<R> &w, Control *ctl,
-                                                  const TurbineView &T, int stage, double t, int nfb, int lb,
+                                                  const TurbineView &T, int stage, double t, int nfb, int lead, int lb,
                                                   int remap)
 {
-    if (lb >= nfb + m.nb) {
-        if constexpr (ROTORS) {
-            const int a = (lb - nfb - m.nb) * (blockDim.x >> 6) + (threadIdx.x >> 6);
+    if (lb < lead) {
+        if (lb < m.nb) {
+            k_traces_body<R, SCHEME>(m, q, z, w, ctl, stage, t, lb);
+            __syncthreads();
+            for (int k = m.bnd_offset[lb] + threadIdx.x; k < m.bnd_offset[lb + 1]; k += blockDim.x) {
+                if constexpr (std::is_same<R, float>::value) face_one_f32<SCHEME>(m, q, z, w, m.bnd_faces[k]);
+                else face_one<R, SCHEME>(m, q, z, w, m.bnd_faces[k]);
+            }
+        } else if constexpr (ROTORS) {
+            const int a = (lb - m.nb) * (blockDim.x >> 6) + (threadIdx.x >> 6);
             if (a < T.nrot) turbine_sample_wave<R>(T, m, q, z, stage, a, threadIdx.x & 63);
         }
         return;
     }
-    if (lb >= nfb) {
-        const int b = lb - nfb;
-        k_traces_body<R, SCHEME>(m, q, z, w, ctl, stage, t, b);
-        __syncthreads();
-        for (int k = m.bnd_offset[b] + threadIdx.x; k < m.bnd_offset[b + 1]; k += blockDim.x) {
-            if constexpr (std::is_same<R, float>::value) face_one_f32<SCHEME>(m, q, z, w, m.bnd_faces[k]);
-            else face_one<R, SCHEME>(m, q, z, w, m.bnd_faces[k]);
-        }
-        return;
-    }
-    const int f = (remap ? xcd_order(lb, nfb) : lb) * blockDim.x + threadIdx.x;
+    const int fb = lb - lead;
+    const int f = (remap ? xcd_order(fb, nfb) : fb) * blockDim.x + threadIdx.x;
     if (f >= m.nf || m.bid[f] >= 0) return;
     if constexpr (std::is_same<R, float>::value) face_one_f32<SCHEME>(m, q, z, w, f);
     else face_one<R, SCHEME>(m, q, z, w, f);
@@ -286,9 +286,9 @@ __device__ __forceinline__ void face_launch_block(const MeshView<R> &m, const St
 template <class R, int SCHEME, bool ROTORS>
 __global__ void __launch_bounds__(256) k_face_traces(MeshView<R> m, StateView<R> q, const double *__restrict__ z,
                                                      Work<R> w, Control *ctl, TurbineView T, int stage, double t,
-                                                     int nfb, int remap)
+                                                     int nfb, int lead, int remap)
 {
-    face_launch_block<R, SCHEME, ROTORS>(m, q, z, w, ctl, T, stage, t, nfb, blockIdx.x, remap);
+    face_launch_block<R, SCHEME, ROTORS>(m, q, z, w, ctl, T, stage, t, nfb, lead, blockIdx.x, remap);
 }
 
 // STAGE 0: out = in + dt r (in is the dragged base);  STAGE 1: out = in0/2 +

@@ -63,13 +63,13 @@ template <class R, int SCHEME>
 static void launch_face_traces(const MeshView<R> &m, StateView<R> in, const double *z, Work<R> w, Control *ctl,
                                int slot, double t, const LaunchCfg &cfg, hipStream_t s, const TurbineView &T = {})
 {
-    const int B = 256, nfb = grid_for(m.nf, B);
+    const int B = 256, nfb = grid_for(m.nf, B), lead = face_lead(m.nb, T.nrot);
     if (T.nrot > 0)
-        timed_launch(cfg, slot, KK_FACE, k_face_traces<R, SCHEME, true>, dim3(nfb + m.nb + (T.nrot + 3) / 4), dim3(B),
-                     s, m, in, z, w, ctl, T, slot, t, nfb, cfg.remap);
+        timed_launch(cfg, slot, KK_FACE, k_face_traces<R, SCHEME, true>, dim3(lead + nfb), dim3(B), s, m, in, z, w,
+                     ctl, T, slot, t, nfb, lead, cfg.remap);
     else
-        timed_launch(cfg, slot, KK_FACE, k_face_traces<R, SCHEME, false>, dim3(nfb + m.nb), dim3(B), s, m, in, z, w,
-                     ctl, T, slot, t, nfb, cfg.remap);
+        timed_launch(cfg, slot, KK_FACE, k_face_traces<R, SCHEME, false>, dim3(lead + nfb), dim3(B), s, m, in, z, w,
+                     ctl, T, slot, t, nfb, lead, cfg.remap);
 }
 
 template <class R, int SCHEME>
@@ -192,21 +192,21 @@ void run_hllc_substep(const MeshView<R> &m, StateView<R> base, StateView<R> work
     const int gz = grid_for(nzero, B);
     timed_launch(cfg, 0, KK_RDRAG, k_river_drag<R>, dim3(ncb > gz ? ncb : gz), blk, s, m.nc, base, work, rb.drag,
                  .5 * dt, g, dry, cfg.remap, zero, nzero);
-    const int nrb = (T.nrot + 3) / 4, nfb = grid_for(m.nf, B);
-    if (nrb > 0) {
+    const int nfb = grid_for(m.nf, B), flead = face_lead(m.nb, T.nrot), clead = face_lead(0, T.nrot);
+    if (T.nrot > 0) {
         // rotors sample `work` (the predictor's momentum source) and then qhalf
-        // (the corrector's) in extra workgroups of the reconstruction and face launches
-        timed_launch(cfg, 0, KK_HRECON, kh_reconstruct<true>, dim3(ncb + nrb), blk, s, H, m, work, z, qh, T, g, dry,
-                     dt, ncb, cfg.remap);
+        // (the corrector's) in leading workgroups of the reconstruction and face launches
+        timed_launch(cfg, 0, KK_HRECON, kh_reconstruct<true>, dim3(clead + ncb), blk, s, H, m, work, z, qh, T, g, dry,
+                     dt, ncb, clead, cfg.remap);
         if (H.momentum && T.nunion > 0)
             hipLaunchKernelGGL(kh_predict, dim3(grid_for(T.nunion, B)), blk, 0, s, H, work, qh, T, dry, dt);
-        timed_launch(cfg, 0, KK_HFACE, kh_faces<true>, dim3(nfb + m.nb + nrb), blk, s, H, m, qh, z, w, ctl, T,
-                     t + .5 * dt, g, dry, nfb, cfg.remap);
+        timed_launch(cfg, 0, KK_HFACE, kh_faces<true>, dim3(flead + nfb), blk, s, H, m, qh, z, w, ctl, T,
+                     t + .5 * dt, g, dry, nfb, flead, cfg.remap);
     } else {
         timed_launch(cfg, 0, KK_HRECON, kh_reconstruct<false>, dim3(ncb), blk, s, H, m, work, z, qh, T, g, dry, dt,
-                     ncb, cfg.remap);
-        timed_launch(cfg, 0, KK_HFACE, kh_faces<false>, dim3(nfb + m.nb), blk, s, H, m, qh, z, w, ctl, T, t + .5 * dt,
-                     g, dry, nfb, cfg.remap);
+                     ncb, 0, cfg.remap);
+        timed_launch(cfg, 0, KK_HFACE, kh_faces<false>, dim3(flead + nfb), blk, s, H, m, qh, z, w, ctl, T,
+                     t + .5 * dt, g, dry, nfb, flead, cfg.remap);
     }
     timed_launch(cfg, 0, KK_HUPDATE, kh_update, dim3(ncb), blk, s, H, work, qh, out, T, rb.drag, ctl,
                  w.rpart + rpart_index(RQ_PEAK, 0), g, dry, dt, cfg.remap);

@@ -53,6 +53,10 @@ constexpr int RPART = RSLOTS * RSTRIDE;      // u64 per (quantity, stage)
 enum : int { RQ_HMAX = 0, RQ_PEAK = 1 };
 __host__ __device__ inline int rpart_index(int quantity, int stage) { return (quantity * 4 + stage) * RPART; }
 
+// Workgroups ahead of the faces in a face launch: one per open boundary and
+// one per four rotor wavefronts, padded to a multiple of 8.
+__host__ __device__ inline int face_lead(int nb, int nrot) { return (nb + (nrot + 3) / 4 + 7) & ~7; }
+
 template <class R> struct Work {
     unsigned long long *rpart;   // [2][4][RPART] partial maxima (bits of non-negative doubles)
     R *grad;       // [8*nc]
