@@ -21,7 +21,8 @@ import csv
 import re
 
 # kernel-name prefix -> kind used by the solver's event profiler
-KINDS = {'k_prep': 'prep', 'k_prep_ls64': 'prep', 'k_traces': 'traces', 'k_face': 'face',
+KINDS = {'k_prep': 'prep', 'k_prep_ls64': 'prep', 'k_traces': 'traces', 'k_face': 'face', 'k_face_traces': 'face',
+         'k_close': 'close', 'k_finalize_mirror': 'finalize',
          'k_face_f32': 'face', 'k_drain': 'drain', 'k_gather': 'gather', 'k_ledger': 'ledger',
          'k_river_theta': 'rtheta', 'k_river_update': 'rupdate', 'k_river_drag': 'rdrag',
          'k_river_ledger': 'ledger', 'kh_reconstruct': 'hrecon', 'kh_faces': 'hface', 'kh_budget': 'hbudget',
