@@ -87,7 +87,7 @@ class Batch:
 
 
 # ---------------------------------------------------------------------------
-# Config-4 requests
+# Scenario requests: 256 arrivals, 20k-400k cells, channel or braided reach
 @dataclass
 class Request:
     rid: int

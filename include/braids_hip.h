@@ -282,9 +282,12 @@ int braids_mass(braids_solver *s, double *mass);
  * BRAIDS_PROFILE wall-clock phases, cutcell_solver.f90:16584-16646).
  * braids_profile(s, mask) brackets the launches of later steps whose kind bit
  * is set in mask (bit k of {prep, traces, face, drain, gather, ledger,
- * river drag, river theta, turbine, river update}; -1 =
- * all, 0 = off) with HIP events on the solver's stream; braids_profile_read
- * returns accumulated milliseconds and launch counts per kind. */
+ * river drag, river theta, turbine, river update, HLLC reconstruction, HLLC
+ * faces, HLLC budget, HLLC update}; -1 = all, 0 = off) with HIP events on the
+ * solver's stream; braids_profile_read returns accumulated milliseconds and
+ * launch counts per kind into ms[14] and launches[14].  The traces, river
+ * theta, turbine and HLLC budget work runs inside the face and update
+ * launches, so those kinds count no launches. */
 int braids_profile(braids_solver *s, int enable);
 int braids_profile_read(braids_solver *s, double *ms, long long *launches);
 /* Plain device copy of `bytes` (multiple of 16) repeated `repeats` times on
