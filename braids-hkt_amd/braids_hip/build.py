@@ -25,7 +25,11 @@ COMMON = ['-O3', '-std=c++17', '-fPIC', f'--offload-arch={ARCH}',
           f'-I{INCLUDE}', f'-I{CSRC}']
 UNITS = [
     ('swe_f64.hip', ['-ffp-contract=off']),
-    ('swe_f32.hip', []),
+    # FP32 form: FP32 division and square root by the hardware reciprocal and
+    # square-root instructions (about 1 ulp instead of correctly rounded) and
+    # FP32 denormals flushed to zero; FP64 arithmetic in this unit is unchanged.
+    # This removes about a third of the instructions of the face launch.
+    ('swe_f32.hip', ['-fno-hip-fp32-correctly-rounded-divide-sqrt', '-fgpu-flush-denormals-to-zero']),
     ('engine.hip', ['-ffp-contract=off']),
     ('braids_api.hip', ['-ffp-contract=off']),
 ]

@@ -237,12 +237,22 @@ __device__ __forceinline__ double river_theta(const MeshView<R> &m, const Work<R
                                               double dt)
 {
     double out = 0.;
-    for (int k = m.offset[c]; k < m.offset[c + 1]; ++k) {
-        const int f = m.adj_face[k];
-        const double sg = (m.adj_flags[k] & ADJ_OWNER) ? 1. : -1.;
-        out += fmax(sg * (double)w.F[f], 0.) * (double)m.length[f];
-    }
+    const int k0 = m.offset[c], k1 = m.offset[c + 1];
     const double hA = q.h[c] * (double)m.area[c];
+    for (int kb = k0; kb < k1; kb += ADJ_CHUNK) {
+        int f[ADJ_CHUNK], fl[ADJ_CHUNK];
+        R F1[ADJ_CHUNK], L[ADJ_CHUNK];
+#pragma unroll
+        for (int i = 0; i < ADJ_CHUNK; ++i) adj_entry(m, kb + i < k1 ? kb + i : k1 - 1, f[i], fl[i]);
+#pragma unroll
+        for (int i = 0; i < ADJ_CHUNK; ++i) { F1[i] = w.F[f[i]]; L[i] = m.length[f[i]]; }
+#pragma unroll
+        for (int i = 0; i < ADJ_CHUNK; ++i) {
+            if (kb + i >= k1) break;
+            const double sg = (fl[i] & ADJ_OWNER) ? 1. : -1.;
+            out += fmax(sg * (double)F1[i], 0.) * (double)L[i];
+        }
+    }
     double th = 1.;
     if (out > 0. && dt * out >= hA) th = fmin(1., hA / (dt * out)) * (1. - 32. * 2.2204460492503131e-16);
     return th;
@@ -308,16 +318,32 @@ __device__ __forceinline__ void k_river_update_body(MeshView<R> m, StateView<R> 
     const int base = lb * blockDim.x;
     const int c = base + threadIdx.x;
     double thc = 1., freq = 0.;
+    // The cell's state is read ahead of the barrier (loads do not cross it).
+    double h = 0., hu = 0., hv = 0., h0 = 0., hu0 = 0., hv0 = 0., Ad = 1.;
+    int k0 = 0, k1 = 0;
     if (c < nc) {
+        k0 = m.offset[c]; k1 = m.offset[c + 1];
+        h = in.h[c]; hu = in.hu[c]; hv = in.hv[c];
+        if (STAGE == 1) { h0 = in0.h[c]; hu0 = in0.hu[c]; hv0 = in0.hv[c]; }
+        Ad = (double)m.area[c];
         double outv = 0., sum = 0.;
-        for (int k = m.offset[c]; k < m.offset[c + 1]; ++k) {
-            const int f = m.adj_face[k];
-            const double sg = (m.adj_flags[k] & ADJ_OWNER) ? 1. : -1.;
-            const double L = (double)m.length[f];
-            outv += fmax(sg * (double)w.F[f], 0.) * L;
-            sum += L * (double)w.F[5 * nf + f];
+        for (int kb = k0; kb < k1; kb += ADJ_CHUNK) {
+            int f[ADJ_CHUNK], fl[ADJ_CHUNK];
+            R F1[ADJ_CHUNK], L[ADJ_CHUNK], S[ADJ_CHUNK];
+#pragma unroll
+            for (int i = 0; i < ADJ_CHUNK; ++i) adj_entry(m, kb + i < k1 ? kb + i : k1 - 1, f[i], fl[i]);
+#pragma unroll
+            for (int i = 0; i < ADJ_CHUNK; ++i) { F1[i] = w.F[f[i]]; L[i] = m.length[f[i]]; S[i] = w.F[5 * nf + f[i]]; }
+#pragma unroll
+            for (int i = 0; i < ADJ_CHUNK; ++i) {
+                if (kb + i >= k1) break;
+                const double sg = (fl[i] & ADJ_OWNER) ? 1. : -1.;
+                const double Ld = (double)L[i];
+                outv += fmax(sg * (double)F1[i], 0.) * Ld;
+                sum += Ld * (double)S[i];
+            }
         }
-        const double A = (double)m.area[c], hA = in.h[c] * A;
+        const double A = Ad, hA = h * A;
         if (outv > 0. && dt * outv >= hA) thc = fmin(1., hA / (dt * outv)) * (1. - 32. * 2.2204460492503131e-16);
         freq = sum / A;
         if (!(freq == freq)) freq = 1e300;
@@ -327,18 +353,32 @@ __device__ __forceinline__ void k_river_update_body(MeshView<R> m, StateView<R> 
     __syncthreads();
     int fail = 0;
     if (c < nc) {
-        const double h = in.h[c], dry = (double)m.dry;
+        const double dry = (double)m.dry;
         const R A = m.area[c];
-        const double Ad = (double)A;
         const double p = .5 * (double)m.grav * (h * h);
         double s0 = 0., s1 = 0., s2 = 0., mass = 0., mx = 0., my = 0.;
-        for (int k = m.offset[c]; k < m.offset[c + 1]; ++k) {
-            const int f = m.adj_face[k];
-            const bool own = (m.adj_flags[k] & ADJ_OWNER) != 0;
-            const R L = m.length[f];
-            const R F1 = w.F[f];
-            const R G2 = own ? w.F[nf + f] : w.F[3 * nf + f];
-            const R G3 = own ? w.F[2 * nf + f] : w.F[4 * nf + f];
+        for (int kb = k0; kb < k1; kb += ADJ_CHUNK) {
+            int fa[ADJ_CHUNK], fl[ADJ_CHUNK], oa[ADJ_CHUNK], ja[ADJ_CHUNK], ba[ADJ_CHUNK];
+            R La[ADJ_CHUNK], F1a[ADJ_CHUNK], G2a[ADJ_CHUNK], G3a[ADJ_CHUNK];
+#pragma unroll
+            for (int i = 0; i < ADJ_CHUNK; ++i) adj_entry(m, kb + i < k1 ? kb + i : k1 - 1, fa[i], fl[i]);
+#pragma unroll
+            for (int i = 0; i < ADJ_CHUNK; ++i) {
+                const int f = fa[i];
+                const bool own = (fl[i] & ADJ_OWNER) != 0;
+                La[i] = m.length[f]; F1a[i] = w.F[f];
+                G2a[i] = w.F[(own ? nf : 3 * nf) + f]; G3a[i] = w.F[(own ? 2 * nf : 4 * nf) + f];
+                oa[i] = m.owner[f]; ja[i] = m.neighbour[f]; ba[i] = m.bid[f];
+            }
+#pragma unroll
+            for (int i = 0; i < ADJ_CHUNK; ++i) {
+            if (kb + i >= k1) break;
+            const int f = fa[i];
+            const bool own = (fl[i] & ADJ_OWNER) != 0;
+            const R L = La[i];
+            const R F1 = F1a[i];
+            const R G2 = G2a[i];
+            const R G3 = G3a[i];
             // native rate (bend_native_rates_tail.inc:4-20): momentum only,
             // the mass rate is replaced by the limited water below
             if (sizeof(R) == 8) {
@@ -353,14 +393,14 @@ __device__ __forceinline__ void k_river_update_body(MeshView<R> m, StateView<R> 
             }
             // midpoint_faces (river_midpoint.h:22-33) for this face
             double wl = (double)F1, cx = 0., cy = 0.;
-            const int o = m.owner[f], j = m.neighbour[f];
+            const int o = oa[i], j = ja[i];
             if (wl != 0.) {
                 const int donor = wl < 0. ? j : o;
                 const int r = donor - base;
                 const double th = donor < 0 ? 1. : (r >= 0 && r < (int)blockDim.x ? th_s[r]
                                                                                : river_theta<R>(m, w, in, donor, dt));
                 if (donor >= 0 && th != 1.) {
-                    if (j < 0 && m.bid[f] >= 0) fail = RFAIL_BOUNDARY;
+                    if (j < 0 && ba[i] >= 0) fail = RFAIL_BOUNDARY;
                     else {
                         const double delta = (th - 1.) * wl;
                         wl = wl * th;
@@ -372,22 +412,23 @@ __device__ __forceinline__ void k_river_update_body(MeshView<R> m, StateView<R> 
             // midpoint_gather (river_midpoint.h:34-41)
             const double sw = (own ? 1. : -1.) * (double)L / Ad;
             mass -= sw * wl; mx -= sw * cx; my -= sw * cy;
-            if (j < 0 && m.bid[f] >= 0) wlim[f] = wl;
+            if (j < 0 && ba[i] >= 0) wlim[f] = wl;
+            }
         }
         (void)s0;
         double r0 = mass, r1 = -s1 + mx, r2 = -s2 + my;
         if (T.nrot > 0 && T.tmask[c]) {
             double sx, sy;
-            turbine_sink(T, T.tuidx[c], in.h[c], in.hu[c], in.hv[c], sx, sy);
+            turbine_sink(T, T.tuidx[c], h, hu, hv, sx, sy);
             r1 = r1 - sx; r2 = r2 - sy;
         }
         double nh, nhu, nhv;
         if (STAGE == 0) {
-            nh = in.h[c] + dt * r0; nhu = in.hu[c] + dt * r1; nhv = in.hv[c] + dt * r2;
+            nh = h + dt * r0; nhu = hu + dt * r1; nhv = hv + dt * r2;
         } else {
-            nh = .5 * in0.h[c] + .5 * (in.h[c] + dt * r0);
-            nhu = .5 * in0.hu[c] + .5 * (in.hu[c] + dt * r1);
-            nhv = .5 * in0.hv[c] + .5 * (in.hv[c] + dt * r2);
+            nh = .5 * h0 + .5 * (h + dt * r0);
+            nhu = .5 * hu0 + .5 * (hu + dt * r1);
+            nhv = .5 * hv0 + .5 * (hv + dt * r2);
         }
         if (!isfinite(nh) || !isfinite(nhu) || !isfinite(nhv)) fail = fail > RFAIL_NONFINITE ? fail : RFAIL_NONFINITE;
         if (nh < 0) fail = fail > RFAIL_NEGATIVE ? fail : RFAIL_NEGATIVE;

@@ -97,6 +97,17 @@ static __global__ void __launch_bounds__(64) k_finalize(const unsigned long long
     if (q == RQ_HMAX) ctl->hmax_bits[s] = m; else ctl->peak_bits[s] = m;
 }
 
+// A cell's incidences are gathered ADJ_CHUNK at a time: the index loads of a
+// chunk, then the loads they address, are issued together, so a triangle's
+// three faces cost two memory latencies instead of six dependent ones.
+// Entries past the cell's last are clamped to it (loaded, then ignored).
+constexpr int ADJ_CHUNK = 3;
+template <class R>
+__device__ __forceinline__ void adj_entry(const MeshView<R> &m, int k, int &face, int &flags)
+{
+    face = m.adj_face[k]; flags = m.adj_flags[k];
+}
+
 // XCD-aware block remap: consecutive logical blocks (neighbouring mesh
 // ranges in Hilbert order) are kept on one XCD so they share its L2.
 __device__ __forceinline__ int logical_block(int remap)
@@ -148,14 +159,29 @@ __device__ __forceinline__ void k_prep_body(MeshView<R> m, StateView<R> q, const
                 R b0[4] = {R(0), R(0), R(0), R(0)}, b1[4] = {R(0), R(0), R(0), R(0)};
                 // lo/hi as offsets from the centre value for eta and z (FP64 differences)
                 R lo[4] = {R(0), uc, vc, R(0)}, hi[4] = {R(0), uc, vc, R(0)};
-                for (int k = k0; k < k1; ++k) {
-                    int o = m.adj_other[k];
-                    if (o < 0) continue;
-                    const R wr0 = m.adj_wr[k], wr1 = m.adj_wr[m.nadj + k];
-                    const double ho = q.h[o], zo = z[o];
+                for (int kb = k0; kb < k1; kb += ADJ_CHUNK) {
+                int oa[ADJ_CHUNK];
+                R wr0a[ADJ_CHUNK], wr1a[ADJ_CHUNK];
+                double hoa[ADJ_CHUNK], zoa[ADJ_CHUNK], huoa[ADJ_CHUNK], hvoa[ADJ_CHUNK];
+#pragma unroll
+                for (int i = 0; i < ADJ_CHUNK; ++i) {
+                    const int k = kb + i < k1 ? kb + i : k1 - 1;
+                    oa[i] = m.adj_other[k]; wr0a[i] = m.adj_wr[k]; wr1a[i] = m.adj_wr[m.nadj + k];
+                }
+#pragma unroll
+                for (int i = 0; i < ADJ_CHUNK; ++i) {
+                    const int oi = oa[i] >= 0 ? oa[i] : c;
+                    hoa[i] = q.h[oi]; zoa[i] = z[oi]; huoa[i] = q.hu[oi]; hvoa[i] = q.hv[oi];
+                }
+#pragma unroll
+                for (int i = 0; i < ADJ_CHUNK; ++i) {
+                    if (kb + i >= k1) break;
+                    if (oa[i] < 0) continue;
+                    const R wr0 = wr0a[i], wr1 = wr1a[i];
+                    const double ho = hoa[i], zo = zoa[i];
                     double etao = ho + zo;
                     R uo = R(0), vo = R(0);
-                    if (ho > (double)dry) { uo = (R)(q.hu[o] / ho); vo = (R)(q.hv[o] / ho); }
+                    if (ho > (double)dry) { uo = (R)(huoa[i] / ho); vo = (R)(hvoa[i] / ho); }
                     if (etac - zc > (double)dry && etao - zo <= (double)dry && zo >= etac) etao = etac;
                     R de = (R)(etao - etac), du = uo - uc, dv = vo - vc, dz = (R)(zo - zc);
                     b0[0] = b0[0] + wr0 * de; b1[0] = b1[0] + wr1 * de;
@@ -166,6 +192,7 @@ __device__ __forceinline__ void k_prep_body(MeshView<R> m, StateView<R> q, const
                     lo[1] = vmin(lo[1], uo); hi[1] = vmax(hi[1], uo);
                     lo[2] = vmin(lo[2], vo); hi[2] = vmax(hi[2], vo);
                     lo[3] = vmin(lo[3], dz); hi[3] = vmax(hi[3], dz);
+                }
                 }
                 const R a11 = m.ls[c], a12 = m.ls[nc + c], a22 = m.ls[2 * nc + c], det = m.ls[3 * nc + c];
                 if (!(det <= R(1e-12) * ((a11 + a22) * (a11 + a22)))) {
@@ -183,12 +210,22 @@ __device__ __forceinline__ void k_prep_body(MeshView<R> m, StateView<R> q, const
                         gx[j] = (a22 * b0[j] - a12 * b1[j]) / det;
                         gy[j] = (a11 * b1[j] - a12 * b0[j]) / det;
                     }
-                    for (int k = k0; k < k1; ++k) {
-                        const R r0 = m.adj_d[k], r1 = m.adj_d[m.nadj + k];
+                    for (int kb = k0; kb < k1; kb += ADJ_CHUNK) {
+                    R r0a[ADJ_CHUNK], r1a[ADJ_CHUNK];
+                    int fla[ADJ_CHUNK];
+#pragma unroll
+                    for (int i = 0; i < ADJ_CHUNK; ++i) {
+                        const int k = kb + i < k1 ? kb + i : k1 - 1;
+                        r0a[i] = m.adj_d[k]; r1a[i] = m.adj_d[m.nadj + k]; fla[i] = m.adj_flags[k];
+                    }
+#pragma unroll
+                    for (int i = 0; i < ADJ_CHUNK; ++i) {
+                        if (kb + i >= k1) break;
+                        const R r0 = r0a[i], r1 = r1a[i];
                         R delta[4];
                         for (int j = 0; j < 4; ++j) delta[j] = gx[j] * r0 + gy[j] * r1;
-                        if ((m.adj_flags[k] & ADJ_OUTLET2) && etac - zc > (double)dry) {
-                            const int f = m.adj_face[k];
+                        if ((fla[i] & ADJ_OUTLET2) && etac - zc > (double)dry) {
+                            const int f = m.adj_face[kb + i];
                             if (fabs(uc * m.normal[f] + vc * m.normal[m.nf + f]) < Num<R>::sqrt_(m.grav * (R)(etac - zc))) {
                                 if ((double)fabs(delta[3]) > e64d * vmax(1.0, fabs(zc))) continue;
                             }
@@ -199,6 +236,7 @@ __device__ __forceinline__ void k_prep_body(MeshView<R> m, StateView<R> q, const
                             if (delta[j] < R(0)) theta[j] = vmin(theta[j], (lo[j] - cen[j]) / delta[j]);
                         }
                     }
+                    }
                     for (int j = 0; j < 4; ++j) {
                         R t = vmax(R(0), theta[j]);
                         gx[j] = gx[j] * t; gy[j] = gy[j] * t;
@@ -208,11 +246,22 @@ __device__ __forceinline__ void k_prep_body(MeshView<R> m, StateView<R> q, const
             // constant-surface flag (prepare_wet_dry_traces)
             bool flatten = h <= (double)dry;
             if (!flatten) {
-                for (int k = k0; k < k1; ++k) {
-                    const R r0 = m.adj_d[k], r1 = m.adj_d[m.nadj + k];
-                    double eta = etac + (double)(gx[0] * r0) + (double)(gy[0] * r1);
-                    double zb = zc + (double)(gx[3] * r0) + (double)(gy[3] * r1);
-                    if (eta - zb <= (double)dry || (m.irregular[c] && fabs(zb - zc) >= h)) { flatten = true; break; }
+                const int irr = m.irregular[c];
+                for (int kb = k0; kb < k1 && !flatten; kb += ADJ_CHUNK) {
+                    R r0a[ADJ_CHUNK], r1a[ADJ_CHUNK];
+#pragma unroll
+                    for (int i = 0; i < ADJ_CHUNK; ++i) {
+                        const int k = kb + i < k1 ? kb + i : k1 - 1;
+                        r0a[i] = m.adj_d[k]; r1a[i] = m.adj_d[m.nadj + k];
+                    }
+#pragma unroll
+                    for (int i = 0; i < ADJ_CHUNK; ++i) {
+                        if (kb + i >= k1) break;
+                        const R r0 = r0a[i], r1 = r1a[i];
+                        double eta = etac + (double)(gx[0] * r0) + (double)(gy[0] * r1);
+                        double zb = zc + (double)(gx[3] * r0) + (double)(gy[3] * r1);
+                        if (eta - zb <= (double)dry || (irr && fabs(zb - zc) >= h)) { flatten = true; break; }
+                    }
                 }
             }
             if (flatten) flags |= FLAG_HCONST;
@@ -259,18 +308,33 @@ static __device__ __forceinline__ void k_prep_ls64_body(MeshView<double> m, Stat
         if (act) {
             double b0[4] = {0, 0, 0, 0}, b1[4] = {0, 0, 0, 0}, lo[4], hi[4];
             for (int j = 0; j < 4; ++j) { lo[j] = vc[j]; hi[j] = vc[j]; }
-            for (int k = k0; k < k1; ++k) {
-                int o = m.adj_other[k];
-                if (o < 0) continue;
-                const double wr0 = m.adj_wr[k], wr1 = m.adj_wr[m.nadj + k];
-                const double ho = q.h[o];
-                double adj[4] = {ho + z[o], 0.0, 0.0, z[o]};
-                if (ho > dry) { adj[1] = q.hu[o] / ho; adj[2] = q.hv[o] / ho; }
-                if (vc[0] - vc[3] > dry && adj[0] - adj[3] <= dry && adj[3] >= vc[0]) adj[0] = vc[0];
-                for (int j = 0; j < 4; ++j) {
-                    b0[j] = b0[j] + wr0 * (adj[j] - vc[j]);
-                    b1[j] = b1[j] + wr1 * (adj[j] - vc[j]);
-                    lo[j] = vmin(lo[j], adj[j]); hi[j] = vmax(hi[j], adj[j]);
+            for (int kb = k0; kb < k1; kb += ADJ_CHUNK) {
+                int oa[ADJ_CHUNK];
+                double wr0a[ADJ_CHUNK], wr1a[ADJ_CHUNK], hoa[ADJ_CHUNK], zoa[ADJ_CHUNK], huoa[ADJ_CHUNK], hvoa[ADJ_CHUNK];
+#pragma unroll
+                for (int i = 0; i < ADJ_CHUNK; ++i) {
+                    const int k = kb + i < k1 ? kb + i : k1 - 1;
+                    oa[i] = m.adj_other[k]; wr0a[i] = m.adj_wr[k]; wr1a[i] = m.adj_wr[m.nadj + k];
+                }
+#pragma unroll
+                for (int i = 0; i < ADJ_CHUNK; ++i) {
+                    const int oi = oa[i] >= 0 ? oa[i] : c;
+                    hoa[i] = q.h[oi]; zoa[i] = z[oi]; huoa[i] = q.hu[oi]; hvoa[i] = q.hv[oi];
+                }
+#pragma unroll
+                for (int i = 0; i < ADJ_CHUNK; ++i) {
+                    if (kb + i >= k1) break;
+                    if (oa[i] < 0) continue;
+                    const double wr0 = wr0a[i], wr1 = wr1a[i];
+                    const double ho = hoa[i];
+                    double adj[4] = {ho + zoa[i], 0.0, 0.0, zoa[i]};
+                    if (ho > dry) { adj[1] = huoa[i] / ho; adj[2] = hvoa[i] / ho; }
+                    if (vc[0] - vc[3] > dry && adj[0] - adj[3] <= dry && adj[3] >= vc[0]) adj[0] = vc[0];
+                    for (int j = 0; j < 4; ++j) {
+                        b0[j] = b0[j] + wr0 * (adj[j] - vc[j]);
+                        b1[j] = b1[j] + wr1 * (adj[j] - vc[j]);
+                        lo[j] = vmin(lo[j], adj[j]); hi[j] = vmax(hi[j], adj[j]);
+                    }
                 }
             }
             const double a11 = m.ls[c], a12 = m.ls[nc + c], a22 = m.ls[2 * nc + c], det = m.ls[3 * nc + c];
@@ -282,20 +346,31 @@ static __device__ __forceinline__ void k_prep_ls64_body(MeshView<double> m, Stat
                     gy[j] = (a11 * b1[j] - a12 * b0[j]) / det;
                     guard[j] = e64 * vmax(vmax(1.0, fabs(vc[j])), vmax(fabs(lo[j]), fabs(hi[j])));
                 }
-                for (int k = k0; k < k1; ++k) {
-                    const double r0 = m.adj_d[k], r1 = m.adj_d[m.nadj + k];
-                    double delta[4];
-                    for (int j = 0; j < 4; ++j) delta[j] = gx[j] * r0 + gy[j] * r1;
-                    if ((m.adj_flags[k] & ADJ_OUTLET2) && vc[0] - vc[3] > dry) {
-                        const int f = m.adj_face[k];
-                        if (fabs(vc[1] * m.normal[f] + vc[2] * m.normal[m.nf + f]) < sqrt(m.grav * (vc[0] - vc[3]))) {
-                            if (fabs(delta[3]) > e64 * vmax(1.0, fabs(vc[3]))) continue;
-                        }
+                for (int kb = k0; kb < k1; kb += ADJ_CHUNK) {
+                    double r0a[ADJ_CHUNK], r1a[ADJ_CHUNK];
+                    int fla[ADJ_CHUNK];
+#pragma unroll
+                    for (int i = 0; i < ADJ_CHUNK; ++i) {
+                        const int k = kb + i < k1 ? kb + i : k1 - 1;
+                        r0a[i] = m.adj_d[k]; r1a[i] = m.adj_d[m.nadj + k]; fla[i] = m.adj_flags[k];
                     }
-                    for (int j = 0; j < 4; ++j) {
-                        if (fabs(delta[j]) <= guard[j]) continue;
-                        if (delta[j] > 0) theta[j] = vmin(theta[j], (hi[j] - vc[j]) / delta[j]);
-                        if (delta[j] < 0) theta[j] = vmin(theta[j], (lo[j] - vc[j]) / delta[j]);
+#pragma unroll
+                    for (int i = 0; i < ADJ_CHUNK; ++i) {
+                        if (kb + i >= k1) break;
+                        const double r0 = r0a[i], r1 = r1a[i];
+                        double delta[4];
+                        for (int j = 0; j < 4; ++j) delta[j] = gx[j] * r0 + gy[j] * r1;
+                        if ((fla[i] & ADJ_OUTLET2) && vc[0] - vc[3] > dry) {
+                            const int f = m.adj_face[kb + i];
+                            if (fabs(vc[1] * m.normal[f] + vc[2] * m.normal[m.nf + f]) < sqrt(m.grav * (vc[0] - vc[3]))) {
+                                if (fabs(delta[3]) > e64 * vmax(1.0, fabs(vc[3]))) continue;
+                            }
+                        }
+                        for (int j = 0; j < 4; ++j) {
+                            if (fabs(delta[j]) <= guard[j]) continue;
+                            if (delta[j] > 0) theta[j] = vmin(theta[j], (hi[j] - vc[j]) / delta[j]);
+                            if (delta[j] < 0) theta[j] = vmin(theta[j], (lo[j] - vc[j]) / delta[j]);
+                        }
                     }
                 }
                 for (int j = 0; j < 4; ++j) {
@@ -306,11 +381,22 @@ static __device__ __forceinline__ void k_prep_ls64_body(MeshView<double> m, Stat
         }
         bool flatten = h <= dry;
         if (!flatten) {
-            for (int k = k0; k < k1; ++k) {
-                const double r0 = m.adj_d[k], r1 = m.adj_d[m.nadj + k];
-                double eta = vc[0] + gx[0] * r0 + gy[0] * r1;
-                double zb = vc[3] + gx[3] * r0 + gy[3] * r1;
-                if (eta - zb <= dry || (m.irregular[c] && fabs(zb - vc[3]) >= h)) { flatten = true; break; }
+            const int irr = m.irregular[c];
+            for (int kb = k0; kb < k1 && !flatten; kb += ADJ_CHUNK) {
+                double r0a[ADJ_CHUNK], r1a[ADJ_CHUNK];
+#pragma unroll
+                for (int i = 0; i < ADJ_CHUNK; ++i) {
+                    const int k = kb + i < k1 ? kb + i : k1 - 1;
+                    r0a[i] = m.adj_d[k]; r1a[i] = m.adj_d[m.nadj + k];
+                }
+#pragma unroll
+                for (int i = 0; i < ADJ_CHUNK; ++i) {
+                    if (kb + i >= k1) break;
+                    const double r0 = r0a[i], r1 = r1a[i];
+                    double eta = vc[0] + gx[0] * r0 + gy[0] * r1;
+                    double zb = vc[3] + gx[3] * r0 + gy[3] * r1;
+                    if (eta - zb <= dry || (irr && fabs(zb - vc[3]) >= h)) { flatten = true; break; }
+                }
             }
         }
         if (flatten) flags |= FLAG_HCONST;
@@ -812,10 +898,19 @@ __device__ __forceinline__ void k_drain_body(MeshView<R> m, StateView<R> q, Work
         if (w.flags[c] & FLAG_ACTIVE) {
             const int k0 = m.offset[c], k1 = m.offset[c + 1];
             double outgoing = 0;
-            for (int k = k0; k < k1; ++k) {
-                const int f = m.adj_face[k];
-                const R s = (m.adj_flags[k] & ADJ_OWNER) ? R(1) : R(-1);
-                outgoing = outgoing + (double)vmax(s * w.F[f], R(0)) * (double)m.length[f];
+            for (int kb = k0; kb < k1; kb += ADJ_CHUNK) {
+                int fa[ADJ_CHUNK], fla[ADJ_CHUNK];
+                R F1a[ADJ_CHUNK], La[ADJ_CHUNK];
+#pragma unroll
+                for (int i = 0; i < ADJ_CHUNK; ++i) adj_entry(m, kb + i < k1 ? kb + i : k1 - 1, fa[i], fla[i]);
+#pragma unroll
+                for (int i = 0; i < ADJ_CHUNK; ++i) { F1a[i] = w.F[fa[i]]; La[i] = m.length[fa[i]]; }
+#pragma unroll
+                for (int i = 0; i < ADJ_CHUNK; ++i) {
+                    if (kb + i >= k1) break;
+                    const R s = (fla[i] & ADJ_OWNER) ? R(1) : R(-1);
+                    outgoing = outgoing + (double)vmax(s * F1a[i], R(0)) * (double)La[i];
+                }
             }
             const double hA = q.h[c] * (double)m.area[c];
             if (dt * outgoing >= hA && outgoing > 0) {
@@ -866,13 +961,29 @@ __device__ __forceinline__ void k_gather_body(MeshView<R> m, StateView<R> in, St
             double s0 = 0.0, s1 = 0.0, s2 = 0.0;
             R cfl = R(0);
             const double p = .5 * (double)g * (h * h);
-            for (int k = k0; k < k1; ++k) {
-                const int f = m.adj_face[k];
-                const bool own = (m.adj_flags[k] & ADJ_OWNER) != 0;
-                R F1 = w.F[f];
-                R G2 = own ? w.F[nf + f] : w.F[3 * nf + f];
-                R G3 = own ? w.F[2 * nf + f] : w.F[4 * nf + f];
-                const R S = w.F[5 * nf + f], L = m.length[f];
+            for (int kb = k0; kb < k1; kb += ADJ_CHUNK) {
+            int fa[ADJ_CHUNK], fla[ADJ_CHUNK];
+            R F1a[ADJ_CHUNK], G2a[ADJ_CHUNK], G3a[ADJ_CHUNK], Sa[ADJ_CHUNK], La[ADJ_CHUNK];
+#pragma unroll
+            for (int i = 0; i < ADJ_CHUNK; ++i) adj_entry(m, kb + i < k1 ? kb + i : k1 - 1, fa[i], fla[i]);
+#pragma unroll
+            for (int i = 0; i < ADJ_CHUNK; ++i) {
+                const int f = fa[i];
+                const bool own = (fla[i] & ADJ_OWNER) != 0;
+                F1a[i] = w.F[f];
+                G2a[i] = w.F[(own ? nf : 3 * nf) + f]; G3a[i] = w.F[(own ? 2 * nf : 4 * nf) + f];
+                Sa[i] = w.F[5 * nf + f]; La[i] = m.length[f];
+            }
+#pragma unroll
+            for (int i = 0; i < ADJ_CHUNK; ++i) {
+                if (kb + i >= k1) break;
+                const int k = kb + i;
+                const int f = fa[i];
+                const bool own = (fla[i] & ADJ_OWNER) != 0;
+                R F1 = F1a[i];
+                R G2 = G2a[i];
+                R G3 = G3a[i];
+                const R S = Sa[i], L = La[i];
                 if (drain && F1 != R(0)) {
                     const int o = m.adj_other[k];
                     // donor: owner for positive flux, neighbour for negative
@@ -908,6 +1019,7 @@ __device__ __forceinline__ void k_gather_body(MeshView<R> m, StateView<R> in, St
                     if (own) { s0 = s0 + wd * (double)F1; s1 = s1 + (double)(wgt * G2); s2 = s2 + (double)(wgt * G3); }
                     else { s0 = s0 - wd * (double)F1; s1 = s1 - (double)(wgt * G2); s2 = s2 - (double)(wgt * G3); }
                 }
+            }
             }
             ratio = (double)(R(dt) * cfl / A);
             if (sizeof(R) == 8) ratio = dt * (double)cfl / (double)A;
