@@ -6,9 +6,11 @@ and HBM budget are free, adds it to a running batch, and keeps enqueueing
 batched substeps.  Every kernel of a substep is one launch over all admitted
 scenarios, each scenario's step controller runs on the device, and the
 substep is a replayed graph, so admitting or retiring a scenario never
-stalls the others beyond one table rebuild.  Each scenario runs a fixed
-number of accepted SSPRK(2,2) substeps; its latency is completion minus
-arrival.
+stalls the others beyond one table rebuild.  Scenario set-up (mesh upload,
+least-squares geometry, rotor footprints) runs on builder threads with their
+own streams while the device advances the batch, so admission does not
+leave the device idle.  Each scenario runs a fixed number of accepted
+SSPRK(2,2) substeps; its latency is completion minus arrival.
 
 The baseline ("plain per-solve GPU runs") solves the same requests one at a
 time in arrival order with the single-scenario path (HydraulicSolver +
@@ -195,53 +197,81 @@ def build_solver(req, precision=32, stream=None):
     return s
 
 
-def serve(requests, cache, precision=32, budget_bytes=64 << 30, max_slots=64, chunk=8, log=None):
+def _builder(precision):
+    """A set-up task for a builder thread.  The solver's set-up runs on the
+    default stream, which does not wait for the batch's (non-blocking)
+    stream; every set-up call has completed its device work when it returns,
+    and the library keeps set-up calls and the capture of a batch's substep
+    graph from overlapping."""
+    def build(req):
+        return build_solver(req, precision, None)
+    return build
+
+
+def serve(requests, cache, precision=32, budget_bytes=64 << 30, max_slots=64, chunk=8, log=None, builders=2,
+          lookahead=8):
     """Continuous batching: returns per-request dicts (arrival, admitted,
-    done, latency) and the makespan."""
+    done, latency) and the makespan.  Up to `lookahead` scenarios beyond the
+    free slots are set up ahead on `builders` threads (within the HBM
+    budget), so a slot that frees up is refilled at the next poll."""
+    import collections
+    from concurrent.futures import ThreadPoolExecutor
     import torch
-    stream = torch.cuda.Stream()
-    batch = Batch(precision, stream)
+    batch = Batch(precision)     # on a stream of its own, non-blocking
     pending = sorted(requests, key=lambda r: r.arrival)
     active = {}                 # slot id -> (req, solver, bytes)
+    building = collections.deque()   # (req, bytes, future) in arrival order
     used = 0
     out = {}
+    pool = ThreadPoolExecutor(max_workers=builders)
+    build = _builder(precision)
     t0 = time.perf_counter()
     i = 0
-    while i < len(pending) or active:
-        now = time.perf_counter() - t0
-        admitted = False
-        while i < len(pending) and pending[i].arrival <= now and len(active) < max_slots:
-            req = cache.realise(pending[i])
-            need = scenario_bytes(req.mesh, len(req.rows), precision)
-            if used + need > budget_bytes and active:
-                break
-            s = build_solver(req, precision, stream)
-            sid = batch.add(s, req.steps, 0.0, 0.01)
-            active[sid] = (req, s, need)
-            used += need
-            out[req.rid] = dict(arrival=req.arrival, admitted=time.perf_counter() - t0, cells=req.mesh.nc,
-                                rotors=len(req.rows), geometry=req.geometry)
-            i += 1; admitted = True
-        if not active:
-            # idle until the next arrival
-            time.sleep(max(0.0, pending[i].arrival - (time.perf_counter() - t0)))
-            continue
-        batch.run(chunk)
-        for st in batch.poll():
-            if st.done:
-                req, s, need = active.pop(st.id)
-                fin = batch.remove(st.id)
-                now = time.perf_counter() - t0
-                out[req.rid].update(done=now, latency=now - req.arrival, accepted=fin.accepted,
-                                    rejections=fin.rejections, reason=fin.reason)
-                used -= need
-                s.close()
-                req.mesh = None
-                if log:
-                    log(f'[serve] request {req.rid} done: {out[req.rid]}')
-    torch.cuda.synchronize()
-    makespan = time.perf_counter() - t0
-    batch.close()
+    try:
+        while i < len(pending) or active or building:
+            now = time.perf_counter() - t0
+            # set up arrived requests ahead, within the slot lookahead and the HBM budget
+            while (i < len(pending) and pending[i].arrival <= now
+                   and len(active) + len(building) < max_slots + lookahead):
+                req = cache.realise(pending[i])
+                need = scenario_bytes(req.mesh, len(req.rows), precision)
+                if used + need > budget_bytes and (active or building):
+                    req.mesh = None
+                    break
+                building.append((req, need, pool.submit(build, req)))
+                out[req.rid] = dict(arrival=req.arrival, cells=req.mesh.nc, rotors=len(req.rows),
+                                    geometry=req.geometry)
+                used += need
+                i += 1
+            # admit finished set-ups into free slots (arrival order)
+            while building and len(active) < max_slots and (building[0][2].done() or not active):
+                req, need, fut = building.popleft()
+                s = fut.result()
+                sid = batch.add(s, req.steps, 0.0, 0.01)
+                active[sid] = (req, s, need)
+                out[req.rid]['admitted'] = time.perf_counter() - t0
+            if not active:
+                # idle until the next arrival
+                time.sleep(max(0.0, pending[i].arrival - (time.perf_counter() - t0)))
+                continue
+            batch.run(chunk)
+            for st in batch.poll():
+                if st.done:
+                    req, s, need = active.pop(st.id)
+                    fin = batch.remove(st.id)
+                    now = time.perf_counter() - t0
+                    out[req.rid].update(done=now, latency=now - req.arrival, accepted=fin.accepted,
+                                        rejections=fin.rejections, reason=fin.reason)
+                    used -= need
+                    s.close()
+                    req.mesh = None
+                    if log:
+                        log(f'[serve] request {req.rid} done: {out[req.rid]}')
+        torch.cuda.synchronize()
+        makespan = time.perf_counter() - t0
+    finally:
+        pool.shutdown(wait=True)
+        batch.close()
     return out, makespan
 
 

@@ -40,7 +40,7 @@ __global__ void __launch_bounds__(256) kb_drag(BatchPlan P, const SlotView<R> *V
     const int lb = blockIdx.x - P.cell[s];
     if (lb >= v.ncb) return;
     k_river_drag_body<R>(v.m.nc, v.q[run[s].parity], v.work, v.rb.drag, .5 * run[s].hint, (double)v.m.grav,
-                         (double)v.m.dry, 1, xcd_order(lb, v.ncb));
+                         (double)v.m.dry, 1, xcd_order(lb, v.ncb), v.w.vel);
 }
 
 template <class R, int SCHEME>

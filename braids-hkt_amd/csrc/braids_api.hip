@@ -9,6 +9,7 @@
 #include <algorithm>
 #include <string>
 #include <vector>
+#include <mutex>
 #include <type_traits>
 
 #include "braids_hip.h"
@@ -19,6 +20,12 @@ using braids::LaunchFault;
 
 static thread_local std::string g_error;
 static void set_error(const std::string &e) { g_error = e; }
+
+// Solver set-up (allocations, uploads, set-up kernels) and the capture of a
+// batch's substep graph exclude each other: a set-up call made on another
+// thread while the substep is being captured invalidates the capture.
+static std::mutex g_setup_mu;
+#define SETUP_GUARD std::lock_guard<std::mutex> setup_guard_(g_setup_mu)
 extern "C" const char *braids_last_error(void) { return g_error.c_str(); }
 
 static std::string launch_fault_note()
@@ -193,6 +200,7 @@ struct Solver : SolverBase {
     LaunchCfg cfg;
     double *qscratch = nullptr;        // [3*nc] double for conversions
     RiverBuffers rb;
+    R *vel_buf = nullptr;              // [2*nc] river stage-input velocities (Work::vel)
     TurbineView tv{};
     ExnerView ex{};
     double *zsnap = nullptr;             // bed snapshot (paired with st[4])
@@ -436,6 +444,7 @@ struct Solver : SolverBase {
         if ((rc = alloc(rb.theta, nc))) return rc;
         if ((rc = alloc(rb.wlim[0], nf))) return rc;
         if ((rc = alloc(rb.wlim[1], nf))) return rc;
+        if ((rc = alloc(vel_buf, 2 * (size_t)nc))) return rc;
         rb.ledger = ledger_dev;
         HIP_TRY(hipMemset(rb.wlim[0], 0, nf * sizeof(double)));
         HIP_TRY(hipMemset(rb.wlim[1], 0, nf * sizeof(double)));
@@ -544,7 +553,9 @@ struct Solver : SolverBase {
     {
         int rc = 0;
         if (!rb.theta && (rc = river_buffers())) return rc;
-        run_river_substep<R>(m, st[0], st[1], st[2], st[3], z, w, ctl, rb, nrot > 0 ? &tv : nullptr, dt, t, cfg, stream,
+        Work<R> wr = w;
+        wr.vel = vel_buf;       // the river stages read their input velocities from the producing launch
+        run_river_substep<R>(m, st[0], st[1], st[2], st[3], z, wr, ctl, rb, nrot > 0 ? &tv : nullptr, dt, t, cfg, stream,
                              (unsigned int *)ctl, (int)(block_bytes / 4), mirror());
         HIP_TRY(hipGetLastError());
         HIP_TRY(hipStreamSynchronize(stream));
@@ -873,7 +884,7 @@ struct Batch : BatchBase {
         for (size_t i = 0; i < n; ++i) {
             Solver<R> *x = slots[i].s;
             SlotView<R> v{};
-            v.m = x->m; v.w = x->w;
+            v.m = x->m; v.w = x->w; v.w.vel = x->vel_buf;
             v.q[0] = x->st[0]; v.q[1] = x->st[3]; v.work = x->st[1]; v.q1 = x->st[2];
             v.z = x->z; v.ctl = x->ctl; v.rb = x->rb;
             v.T = x->tv;
@@ -908,7 +919,10 @@ struct Batch : BatchBase {
         plan.nbnd = pre[2 * (n + 1) + n]; plan.nzero = pre[3 * (n + 1) + n]; plan.nrot = pre[4 * (n + 1) + n];
         dirty = false; dirty_layout = false;
         if (n == 0) return BRAIDS_OK;
-        HIP_TRY(hipStreamBeginCapture(stream, hipStreamCaptureModeThreadLocal));
+        // No solver set-up runs on another thread during the capture (the
+        // set-up lock); other threads' steps and copies are not captured.
+        SETUP_GUARD;
+        HIP_TRY(hipStreamBeginCapture(stream, hipStreamCaptureModeRelaxed));
         run_batch_substep<R>(plan, views, runs, nzero, blocks, stream);
         HIP_TRY(hipStreamEndCapture(stream, &graph));
         HIP_TRY(hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0));
@@ -1037,6 +1051,7 @@ static int check_mesh(const braids_mesh *m)
 
 extern "C" int braids_create(const braids_mesh *mesh, const braids_options *opt, const double *z, braids_solver **out)
 {
+    SETUP_GUARD;
     if (!out || !opt || !z) { set_error("null argument"); return BRAIDS_EINVAL; }
     *out = nullptr;
     int rc = check_mesh(mesh);
@@ -1063,6 +1078,7 @@ extern "C" int braids_create(const braids_mesh *mesh, const braids_options *opt,
 
 extern "C" int braids_destroy(braids_solver *s)
 {
+    SETUP_GUARD;
     if (!s) return BRAIDS_OK;
     delete s->impl;
     delete s;
@@ -1071,6 +1087,7 @@ extern "C" int braids_destroy(braids_solver *s)
 
 extern "C" int braids_set_stream(braids_solver *s, void *stream)
 {
+    SETUP_GUARD;
     if (!s) { set_error("null solver"); return BRAIDS_EINVAL; }
     s->impl->stream = (hipStream_t)stream;
     return BRAIDS_OK;
@@ -1078,6 +1095,7 @@ extern "C" int braids_set_stream(braids_solver *s, void *stream)
 
 extern "C" int braids_set_state(braids_solver *s, const double *q, int prepare)
 {
+    SETUP_GUARD;
     if (!s || !q) { set_error("null argument"); return BRAIDS_EINVAL; }
     return s->impl->set_state(q, prepare);
 }
@@ -1088,6 +1106,7 @@ extern "C" int braids_get_state(braids_solver *s, double *q)
 }
 extern "C" int braids_set_bed(braids_solver *s, const double *z)
 {
+    SETUP_GUARD;
     if (!s || !z) { set_error("null argument"); return BRAIDS_EINVAL; }
     return s->impl->set_bed(z, false);
 }
@@ -1178,6 +1197,7 @@ extern "C" int river_get_face_speed(braids_solver *s, double *speed)
 // -- river SSPRK(2,2) and turbines ---------------------------------------------
 extern "C" int braids_river_configure(braids_solver *s, double cfl, int drag_mode, const double *drag_values)
 {
+    SETUP_GUARD;
     if (!s || !(cfl > 0) || cfl > 1 || drag_mode < 0 || drag_mode > 4) { set_error("invalid river options"); return BRAIDS_EINVAL; }
     return s->impl->river_configure(cfl, drag_mode, drag_values);
 }
@@ -1267,6 +1287,7 @@ extern "C" int braids_river_run(braids_solver *s, int nsteps, double *time, doub
 extern "C" int braids_hllc_configure(braids_solver *s, double cfl, int drag_mode, const double *drag_values,
                                      const double *inverse, int pressure)
 {
+    SETUP_GUARD;
     if (!s || !(cfl > 0) || cfl > 1 || drag_mode < 0 || drag_mode > 4 || !inverse || pressure < -1 || pressure > 1) {
         set_error("invalid HLLC options"); return BRAIDS_EINVAL;
     }
@@ -1357,6 +1378,7 @@ extern "C" int braids_hllc_run(braids_solver *s, int nsteps, double *time, doubl
 
 extern "C" int braids_turbines_set(braids_solver *s, const braids_turbines *t)
 {
+    SETUP_GUARD;
     if (!s || !t || t->nrot < 0) { set_error("null argument"); return BRAIDS_EINVAL; }
     if (t->nrot > 0 && (!t->D || !t->top || !t->bottom || !t->tol || !t->min_surface || !t->min_bed || !t->control ||
                         !t->ct_const || !t->cp_const || !t->table_off || !t->foot_off || !t->foot_cell ||
@@ -1379,33 +1401,46 @@ extern "C" int braids_turbines_read(braids_solver *s, double *diag, double *ener
 extern "C" int braids_turbine_footprint(int npairs, const double *tri, const double *D, const double *nodes,
                                         const double *weights, double *value, double *clipped, int *order)
 {
+    SETUP_GUARD;
     if (npairs < 0 || (npairs > 0 && (!tri || !D || !nodes || !weights || !value || !clipped || !order))) {
         set_error("invalid footprint arguments"); return BRAIDS_EINVAL;
     }
     if (npairs == 0) return BRAIDS_OK;
     const size_t nq = 16 + 32 + 64 + 128;
-    double *dt = nullptr, *dD = nullptr, *dn = nullptr, *dw = nullptr, *dv = nullptr, *dc = nullptr;
-    int *dord = nullptr;
-    auto cleanup = [&]() {
-        for (void *p : {(void *)dt, (void *)dD, (void *)dn, (void *)dw, (void *)dv, (void *)dc, (void *)dord})
-            if (p) (void)hipFree(p);
-    };
-    if (hipMalloc(&dt, 6 * (size_t)npairs * 8) != hipSuccess || hipMalloc(&dD, (size_t)npairs * 8) != hipSuccess ||
-        hipMalloc(&dn, nq * 8) != hipSuccess || hipMalloc(&dw, nq * 8) != hipSuccess ||
-        hipMalloc(&dv, (size_t)npairs * 8) != hipSuccess || hipMalloc(&dc, (size_t)npairs * 8) != hipSuccess ||
-        hipMalloc(&dord, (size_t)npairs * 4) != hipSuccess) {
-        cleanup(); set_error("hipMalloc failed for the footprint"); return BRAIDS_ENOMEM;
+    // One scratch allocation and one stream for the process (grown, never
+    // freed): no hipFree and no device-wide synchronisation, so a solver can
+    // be set up while other streams keep the device busy.
+    static char *scratch = nullptr;           // guarded by the set-up lock
+    static size_t scratch_bytes = 0;
+    static hipStream_t fs = nullptr;
+    const size_t b_tri = 6 * (size_t)npairs * 8, b_pair = (size_t)npairs * 8, b_q = nq * 8;
+    auto up = [](size_t b) { return (b + 255) & ~(size_t)255; };
+    const size_t need = up(b_tri) + 3 * up(b_pair) + 2 * up(b_q) + up((size_t)npairs * 4);
+    if (!fs && hipStreamCreateWithFlags(&fs, hipStreamNonBlocking) != hipSuccess) {
+        fs = nullptr; set_error("hipStreamCreate failed for the footprint"); return BRAIDS_EHIP;
     }
-    hipError_t e = hipMemcpy(dt, tri, 6 * (size_t)npairs * 8, hipMemcpyHostToDevice);
-    if (e == hipSuccess) e = hipMemcpy(dD, D, (size_t)npairs * 8, hipMemcpyHostToDevice);
-    if (e == hipSuccess) e = hipMemcpy(dn, nodes, nq * 8, hipMemcpyHostToDevice);
-    if (e == hipSuccess) e = hipMemcpy(dw, weights, nq * 8, hipMemcpyHostToDevice);
-    if (e == hipSuccess) { run_footprint(npairs, dt, dD, dn, dw, dv, dc, dord, nullptr); e = hipGetLastError(); }
-    if (e == hipSuccess) e = hipDeviceSynchronize();
-    if (e == hipSuccess) e = hipMemcpy(value, dv, (size_t)npairs * 8, hipMemcpyDeviceToHost);
-    if (e == hipSuccess) e = hipMemcpy(clipped, dc, (size_t)npairs * 8, hipMemcpyDeviceToHost);
-    if (e == hipSuccess) e = hipMemcpy(order, dord, (size_t)npairs * 4, hipMemcpyDeviceToHost);
-    cleanup();
+    if (need > scratch_bytes) {
+        char *p = nullptr;
+        if (hipMalloc(&p, need) != hipSuccess) { set_error("hipMalloc failed for the footprint"); return BRAIDS_ENOMEM; }
+        scratch = p; scratch_bytes = need;    // the smaller block is not freed: hipFree would wait for the device
+    }
+    char *c = scratch;
+    double *dt = (double *)c; c += up(b_tri);
+    double *dD = (double *)c; c += up(b_pair);
+    double *dv = (double *)c; c += up(b_pair);
+    double *dc = (double *)c; c += up(b_pair);
+    double *dn = (double *)c; c += up(b_q);
+    double *dw = (double *)c; c += up(b_q);
+    int *dord = (int *)c;
+    hipError_t e = hipMemcpyAsync(dt, tri, b_tri, hipMemcpyHostToDevice, fs);
+    if (e == hipSuccess) e = hipMemcpyAsync(dD, D, b_pair, hipMemcpyHostToDevice, fs);
+    if (e == hipSuccess) e = hipMemcpyAsync(dn, nodes, b_q, hipMemcpyHostToDevice, fs);
+    if (e == hipSuccess) e = hipMemcpyAsync(dw, weights, b_q, hipMemcpyHostToDevice, fs);
+    if (e == hipSuccess) { run_footprint(npairs, dt, dD, dn, dw, dv, dc, dord, fs); e = hipGetLastError(); }
+    if (e == hipSuccess) e = hipMemcpyAsync(value, dv, b_pair, hipMemcpyDeviceToHost, fs);
+    if (e == hipSuccess) e = hipMemcpyAsync(clipped, dc, b_pair, hipMemcpyDeviceToHost, fs);
+    if (e == hipSuccess) e = hipMemcpyAsync(order, dord, (size_t)npairs * 4, hipMemcpyDeviceToHost, fs);
+    if (e == hipSuccess) e = hipStreamSynchronize(fs);
     if (e != hipSuccess) { set_error(std::string("footprint: ") + hipGetErrorString(e)); return BRAIDS_EHIP; }
     return BRAIDS_OK;
 }
@@ -1413,6 +1448,7 @@ extern "C" int braids_turbine_footprint(int npairs, const double *tri, const dou
 // -- Exner bed update ---------------------------------------------------------------
 extern "C" int braids_exner_configure(braids_solver *s, const braids_exner_options *o)
 {
+    SETUP_GUARD;
     if (!s || !o || !o->rock || !(o->d50 > 0) || !(o->relative_density > 0) || !(o->porosity >= 0) ||
         !(o->porosity < 1) || !(o->critical_shields >= 0) || !(o->morfac > 0) || !(o->sediment_depth > 0)) {
         set_error("invalid Exner options"); return BRAIDS_EINVAL;
@@ -1447,7 +1483,7 @@ extern "C" int braids_batch_create(int precision, void *stream, braids_batch **o
     BatchBase *b = precision == 64 ? (BatchBase *)new Batch<double>() : (BatchBase *)new Batch<float>();
     if (stream) b->stream = (hipStream_t)stream;
     else {
-        if (hipStreamCreate(&b->stream) != hipSuccess) { delete b; set_error("hipStreamCreate failed"); return BRAIDS_EHIP; }
+        if (hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking) != hipSuccess) { delete b; set_error("hipStreamCreate failed"); return BRAIDS_EHIP; }
         b->own_stream = true;
     }
     *out = new braids_batch{b, precision};

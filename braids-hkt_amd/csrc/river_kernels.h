@@ -51,9 +51,20 @@ __device__ __forceinline__ void cell_drag(double h, double &hu, double &hv, doub
     hu = hu * factor; hv = hv * factor;
 }
 
+// The stage-input velocities of one cell (Work::vel), exactly as a reader
+// would form them from the stored state.
+template <class R>
+__device__ __forceinline__ void store_velocity(R *vel, int nc, int c, double h, double hu, double hv, double dry)
+{
+    R u = R(0), v = R(0);
+    if (h > dry) { u = (R)(hu / h); v = (R)(hv / h); }
+    vel[c] = u; vel[nc + c] = v;
+}
+
 template <class R>
 __device__ __forceinline__ void k_river_drag_body(int nc, StateView<R> in, StateView<R> out, DragView d,
-                                                    double dt, double grav, double dry, int remap, int lb)
+                                                    double dt, double grav, double dry, int remap, int lb,
+                                                    R *vel)
 {
     const int c = lb * blockDim.x + threadIdx.x;
     if (c >= nc) return;
@@ -61,6 +72,7 @@ __device__ __forceinline__ void k_river_drag_body(int nc, StateView<R> in, State
     double hu = in.hu[c], hv = in.hv[c];
     cell_drag(h, hu, hv, d.mode > 0 ? d.values[c] : 0., d.mode, dt, grav, dry);
     out.h[c] = h; out.hu[c] = hu; out.hv[c] = hv;
+    if (vel) store_velocity(vel, nc, c, h, hu, hv, dry);
 }
 
 // The first launch of a substep: the drag half-step, and the clearing of the
@@ -69,11 +81,11 @@ __device__ __forceinline__ void k_river_drag_body(int nc, StateView<R> in, State
 template <class R>
 __global__ void __launch_bounds__(256) k_river_drag(int nc, StateView<R> in, StateView<R> out, DragView d,
                                                     double dt, double grav, double dry, int remap,
-                                                    unsigned int *zero, int nzero)
+                                                    unsigned int *zero, int nzero, R *vel)
 {
     const int g = blockIdx.x * blockDim.x + threadIdx.x;
     if (g < nzero) zero[g] = 0u;
-    k_river_drag_body<R>(nc, in, out, d, dt, grav, dry, remap, logical_block(remap));
+    k_river_drag_body<R>(nc, in, out, d, dt, grav, dry, remap, logical_block(remap), vel);
 }
 
 // ---------------------------------------------------------------------------
@@ -358,17 +370,24 @@ __device__ __forceinline__ void k_river_update_body(MeshView<R> m, StateView<R> 
         const double p = .5 * (double)m.grav * (h * h);
         double s0 = 0., s1 = 0., s2 = 0., mass = 0., mx = 0., my = 0.;
         for (int kb = k0; kb < k1; kb += ADJ_CHUNK) {
-            int fa[ADJ_CHUNK], fl[ADJ_CHUNK], oa[ADJ_CHUNK], ja[ADJ_CHUNK], ba[ADJ_CHUNK];
+            // owner / neighbour of each face from the incidence (the cell and
+            // the cell across); an open face is one the cell owns with no cell across
+            int fa[ADJ_CHUNK], fl[ADJ_CHUNK], oa[ADJ_CHUNK], ja[ADJ_CHUNK];
             R La[ADJ_CHUNK], F1a[ADJ_CHUNK], G2a[ADJ_CHUNK], G3a[ADJ_CHUNK];
 #pragma unroll
-            for (int i = 0; i < ADJ_CHUNK; ++i) adj_entry(m, kb + i < k1 ? kb + i : k1 - 1, fa[i], fl[i]);
+            for (int i = 0; i < ADJ_CHUNK; ++i) {
+                const int k = kb + i < k1 ? kb + i : k1 - 1;
+                adj_entry(m, k, fa[i], fl[i]);
+                const int other = m.adj_other[k];
+                const bool own = (fl[i] & ADJ_OWNER) != 0;
+                oa[i] = own ? c : other; ja[i] = own ? other : c;
+            }
 #pragma unroll
             for (int i = 0; i < ADJ_CHUNK; ++i) {
                 const int f = fa[i];
                 const bool own = (fl[i] & ADJ_OWNER) != 0;
                 La[i] = m.length[f]; F1a[i] = w.F[f];
                 G2a[i] = w.F[(own ? nf : 3 * nf) + f]; G3a[i] = w.F[(own ? 2 * nf : 4 * nf) + f];
-                oa[i] = m.owner[f]; ja[i] = m.neighbour[f]; ba[i] = m.bid[f];
             }
 #pragma unroll
             for (int i = 0; i < ADJ_CHUNK; ++i) {
@@ -400,7 +419,7 @@ __device__ __forceinline__ void k_river_update_body(MeshView<R> m, StateView<R> 
                 const double th = donor < 0 ? 1. : (r >= 0 && r < (int)blockDim.x ? th_s[r]
                                                                                : river_theta<R>(m, w, in, donor, dt));
                 if (donor >= 0 && th != 1.) {
-                    if (j < 0 && ba[i] >= 0) fail = RFAIL_BOUNDARY;
+                    if (fl[i] & ADJ_OPEN) fail = RFAIL_BOUNDARY;
                     else {
                         const double delta = (th - 1.) * wl;
                         wl = wl * th;
@@ -412,7 +431,7 @@ __device__ __forceinline__ void k_river_update_body(MeshView<R> m, StateView<R> 
             // midpoint_gather (river_midpoint.h:34-41)
             const double sw = (own ? 1. : -1.) * (double)L / Ad;
             mass -= sw * wl; mx -= sw * cx; my -= sw * cy;
-            if (j < 0 && ba[i] >= 0) wlim[f] = wl;
+            if (fl[i] & ADJ_OPEN) wlim[f] = wl;
             }
         }
         (void)s0;
@@ -435,6 +454,7 @@ __device__ __forceinline__ void k_river_update_body(MeshView<R> m, StateView<R> 
         if (nh <= dry) { nhu = 0.; nhv = 0.; }
         if (STAGE == 1) cell_drag(nh, nhu, nhv, d.mode > 0 ? d.values[c] : 0., d.mode, drag_dt, (double)m.grav, dry);
         out.h[c] = nh; out.hu[c] = nhu; out.hv[c] = nhv;
+        if (STAGE == 0 && w.vel) store_velocity(w.vel, nc, c, nh, nhu, nhv, dry);
     }
     if (fail) { atomicMax(&ctl->failure[slot], fail); atomicMax(&ctl->fail_cell[slot], c + 1); }
 }

@@ -154,7 +154,8 @@ __device__ __forceinline__ void k_prep_body(MeshView<R> m, StateView<R> q, const
             R gx[4] = {R(0), R(0), R(0), R(0)}, gy[4] = {R(0), R(0), R(0), R(0)};
             const double zc = z[c], etac = h + zc;
             R uc = R(0), vc = R(0);
-            if (h > (double)dry) { uc = (R)(q.hu[c] / h); vc = (R)(q.hv[c] / h); }
+            if (w.vel) { uc = w.vel[c]; vc = w.vel[nc + c]; }
+            else if (h > (double)dry) { uc = (R)(q.hu[c] / h); vc = (R)(q.hv[c] / h); }
             if (act) {
                 R b0[4] = {R(0), R(0), R(0), R(0)}, b1[4] = {R(0), R(0), R(0), R(0)};
                 // lo/hi as offsets from the centre value for eta and z (FP64 differences)
@@ -162,7 +163,8 @@ __device__ __forceinline__ void k_prep_body(MeshView<R> m, StateView<R> q, const
                 for (int kb = k0; kb < k1; kb += ADJ_CHUNK) {
                 int oa[ADJ_CHUNK];
                 R wr0a[ADJ_CHUNK], wr1a[ADJ_CHUNK];
-                double hoa[ADJ_CHUNK], zoa[ADJ_CHUNK], huoa[ADJ_CHUNK], hvoa[ADJ_CHUNK];
+                double hoa[ADJ_CHUNK], zoa[ADJ_CHUNK];
+                R uoa[ADJ_CHUNK], voa[ADJ_CHUNK];
 #pragma unroll
                 for (int i = 0; i < ADJ_CHUNK; ++i) {
                     const int k = kb + i < k1 ? kb + i : k1 - 1;
@@ -171,7 +173,13 @@ __device__ __forceinline__ void k_prep_body(MeshView<R> m, StateView<R> q, const
 #pragma unroll
                 for (int i = 0; i < ADJ_CHUNK; ++i) {
                     const int oi = oa[i] >= 0 ? oa[i] : c;
-                    hoa[i] = q.h[oi]; zoa[i] = z[oi]; huoa[i] = q.hu[oi]; hvoa[i] = q.hv[oi];
+                    hoa[i] = q.h[oi]; zoa[i] = z[oi];
+                    if (w.vel) { uoa[i] = w.vel[oi]; voa[i] = w.vel[nc + oi]; }
+                    else {
+                        const double hu = q.hu[oi], hv = q.hv[oi];
+                        uoa[i] = hoa[i] > (double)dry ? (R)(hu / hoa[i]) : R(0);
+                        voa[i] = hoa[i] > (double)dry ? (R)(hv / hoa[i]) : R(0);
+                    }
                 }
 #pragma unroll
                 for (int i = 0; i < ADJ_CHUNK; ++i) {
@@ -180,8 +188,7 @@ __device__ __forceinline__ void k_prep_body(MeshView<R> m, StateView<R> q, const
                     const R wr0 = wr0a[i], wr1 = wr1a[i];
                     const double ho = hoa[i], zo = zoa[i];
                     double etao = ho + zo;
-                    R uo = R(0), vo = R(0);
-                    if (ho > (double)dry) { uo = (R)(huoa[i] / ho); vo = (R)(hvoa[i] / ho); }
+                    const R uo = uoa[i], vo = voa[i];
                     if (etac - zc > (double)dry && etao - zo <= (double)dry && zo >= etac) etao = etac;
                     R de = (R)(etao - etac), du = uo - uc, dv = vo - vc, dz = (R)(zo - zc);
                     b0[0] = b0[0] + wr0 * de; b1[0] = b1[0] + wr1 * de;
@@ -303,14 +310,15 @@ static __device__ __forceinline__ void k_prep_ls64_body(MeshView<double> m, Stat
         }
         int flags = act ? FLAG_ACTIVE : 0;
         double vc[4] = {h + z[c], 0.0, 0.0, z[c]};
-        if (h > dry) { vc[1] = q.hu[c] / h; vc[2] = q.hv[c] / h; }
+        if (w.vel) { vc[1] = w.vel[c]; vc[2] = w.vel[nc + c]; }
+        else if (h > dry) { vc[1] = q.hu[c] / h; vc[2] = q.hv[c] / h; }
         double gx[4] = {0, 0, 0, 0}, gy[4] = {0, 0, 0, 0};
         if (act) {
             double b0[4] = {0, 0, 0, 0}, b1[4] = {0, 0, 0, 0}, lo[4], hi[4];
             for (int j = 0; j < 4; ++j) { lo[j] = vc[j]; hi[j] = vc[j]; }
             for (int kb = k0; kb < k1; kb += ADJ_CHUNK) {
                 int oa[ADJ_CHUNK];
-                double wr0a[ADJ_CHUNK], wr1a[ADJ_CHUNK], hoa[ADJ_CHUNK], zoa[ADJ_CHUNK], huoa[ADJ_CHUNK], hvoa[ADJ_CHUNK];
+                double wr0a[ADJ_CHUNK], wr1a[ADJ_CHUNK], hoa[ADJ_CHUNK], zoa[ADJ_CHUNK], uoa[ADJ_CHUNK], voa[ADJ_CHUNK];
 #pragma unroll
                 for (int i = 0; i < ADJ_CHUNK; ++i) {
                     const int k = kb + i < k1 ? kb + i : k1 - 1;
@@ -319,7 +327,12 @@ static __device__ __forceinline__ void k_prep_ls64_body(MeshView<double> m, Stat
 #pragma unroll
                 for (int i = 0; i < ADJ_CHUNK; ++i) {
                     const int oi = oa[i] >= 0 ? oa[i] : c;
-                    hoa[i] = q.h[oi]; zoa[i] = z[oi]; huoa[i] = q.hu[oi]; hvoa[i] = q.hv[oi];
+                    hoa[i] = q.h[oi]; zoa[i] = z[oi];
+                    if (w.vel) { uoa[i] = w.vel[oi]; voa[i] = w.vel[nc + oi]; }
+                    else {
+                        uoa[i] = hoa[i] > dry ? q.hu[oi] / hoa[i] : 0.0;
+                        voa[i] = hoa[i] > dry ? q.hv[oi] / hoa[i] : 0.0;
+                    }
                 }
 #pragma unroll
                 for (int i = 0; i < ADJ_CHUNK; ++i) {
@@ -327,8 +340,7 @@ static __device__ __forceinline__ void k_prep_ls64_body(MeshView<double> m, Stat
                     if (oa[i] < 0) continue;
                     const double wr0 = wr0a[i], wr1 = wr1a[i];
                     const double ho = hoa[i];
-                    double adj[4] = {ho + zoa[i], 0.0, 0.0, zoa[i]};
-                    if (ho > dry) { adj[1] = huoa[i] / ho; adj[2] = hvoa[i] / ho; }
+                    double adj[4] = {ho + zoa[i], uoa[i], voa[i], zoa[i]};
                     if (vc[0] - vc[3] > dry && adj[0] - adj[3] <= dry && adj[3] >= vc[0]) adj[0] = vc[0];
                     for (int j = 0; j < 4; ++j) {
                         b0[j] = b0[j] + wr0 * (adj[j] - vc[j]);
@@ -423,7 +435,8 @@ __device__ __forceinline__ void face_value(const MeshView<R> &m, const StateView
     const double h = q.h[c];
     const double zc = z[c], etac = h + zc;
     R uc = R(0), vc = R(0);
-    if (h > (double)m.dry) { uc = (R)(q.hu[c] / h); vc = (R)(q.hv[c] / h); }
+    if (w.vel) { uc = w.vel[c]; vc = w.vel[m.nc + c]; }
+    else if (h > (double)m.dry) { uc = (R)(q.hu[c] / h); vc = (R)(q.hv[c] / h); }
     if (SCHEME == 1) { eta = etac; u = uc; v = vc; zb = zc; return; }
     const int nc = m.nc;
     const R g0x = w.grad[c], g0y = w.grad[nc + c];

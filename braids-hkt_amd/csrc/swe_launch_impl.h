@@ -163,7 +163,7 @@ void run_river_substep(const MeshView<R> &m, StateView<R> base, StateView<R> wor
     TurbineView T{};
     if (tv && tv->nrot > 0) { T = *tv; T.error = &ctl->turbine_error; }
     timed_launch(cfg, 0, KK_RDRAG, k_river_drag<R>, gc, blk, s, m.nc, base, work, rb.drag, .5 * dt,
-                 (double)m.grav, (double)m.dry, cfg.remap, zero, nzero);
+                 (double)m.grav, (double)m.dry, cfg.remap, zero, nzero, w.vel);
     if (m.scheme == 5) {
         river_stage<R, 5>(m, work, work, q1, z, w, ctl, rb, T, 0, dt, t, cfg, s);
         river_stage<R, 5>(m, q1, work, out, z, w, ctl, rb, T, 1, dt, t + dt, cfg, s);
@@ -191,7 +191,7 @@ void run_hllc_substep(const MeshView<R> &m, StateView<R> base, StateView<R> work
     const double g = (double)m.grav, dry = (double)m.dry;
     const int gz = grid_for(nzero, B);
     timed_launch(cfg, 0, KK_RDRAG, k_river_drag<R>, dim3(ncb > gz ? ncb : gz), blk, s, m.nc, base, work, rb.drag,
-                 .5 * dt, g, dry, cfg.remap, zero, nzero);
+                 .5 * dt, g, dry, cfg.remap, zero, nzero, (R *)nullptr);
     const int nfb = grid_for(m.nf, B), flead = face_lead(m.nb, T.nrot), clead = face_lead(0, T.nrot);
     if (T.nrot > 0) {
         // rotors sample `work` (the predictor's momentum source) and then qhalf

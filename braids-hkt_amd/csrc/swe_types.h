@@ -66,6 +66,10 @@ template <class R> struct Work {
     R *theta;      // [nc]
     double *hydraulic; // [2*nb] current stage discharge / stage
     R *target_u;   // [nf] mode-6 weights by boundary position
+    // [2*nc] velocities (u, v) = (hu/h, hv/h) (0 where dry) of the current
+    // stage input, written by the launch that produces that input (river
+    // path); null: the readers divide the state themselves.
+    R *vel;
 };
 
 // Per-slot reductions; slots 0-2 are the SSPRK(3,3) stages, slot 3 a

@@ -261,7 +261,9 @@ typedef struct braids_slot_status {
     int id, accepted, target, rejections, done, reason;
     double time, hint, cfl_max;
 } braids_slot_status;
-/* stream: hipStream_t as void* (NULL: the batch creates its own). */
+/* stream: hipStream_t as void* (NULL: the batch creates its own, non-blocking).
+ * The substep graph is captured in relaxed mode, so other threads may set up
+ * solvers (allocations, copies) while a batch is rebuilt or running. */
 int braids_batch_create(int precision, void *stream, braids_batch **out);
 int braids_batch_destroy(braids_batch *b);
 /* Admit a scenario for target_steps accepted substeps starting at time with
