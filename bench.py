@@ -119,6 +119,8 @@ def build_workload(name):
         m, rows = M.braided_turbine_reach()
     elif name == 'braided_reach_500k':
         m = M.braided_reach()
+    elif name == 'meander_reach_8m_64t':
+        m, rows = M.meander_turbine_reach()
     elif name == 'plain_channel_25k':
         m = M.plain_channel()
     else:

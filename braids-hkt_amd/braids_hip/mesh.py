@@ -326,13 +326,15 @@ def braided_reach(nx=1768, ny=141, length=5000.0, width=400.0, slope=1e-3, ampli
                   wavelength=800.0, nbars=200, q=1500.0, seed=2, depth=3.0,
                   bar_height=(0.5, 2.0), bar_sigma=(15.0, 60.0), bank=4.0,
                   open_min_depth=None, pools=(), pool_depth=6.0, pool_sigma=25.0,
-                  velocity=0.0) -> Mesh:
+                  velocity=0.0, bar_cutoff=None) -> Mesh:
     """Braided synthetic reach 5 km x 400 m: sinusoidal thalweg (amplitude
     60 m, wavelength 800 m), parabolic cross-section, 200 seeded Gaussian bars
     (height 0.5-2 m, sigma 15-60 m, centred away from the inlet and outlet
     sections) that create wet/dry fronts, four Manning
     zones n in {0.025, 0.030, 0.035, 0.040}, inflow Q=1500 m^3/s.  The default
-    2.83 m quads give about 500k triangles."""
+    2.83 m quads give about 500k triangles.  bar_cutoff (in bar sigmas)
+    limits each bar to the cells within that distance of its centre (for
+    meshes of millions of cells; None: every bar over every cell)."""
     dx = length / nx; dy = width / ny
     rng = np.random.default_rng(seed)
     bx = rng.uniform(0.1 * length, 0.9 * length, nbars); by = rng.uniform(0, width, nbars)
@@ -343,8 +345,18 @@ def braided_reach(nx=1768, ny=141, length=5000.0, width=400.0, slope=1e-3, ampli
 
     def bed(x, y):
         z = -slope * x + bank * ((y - thalweg(x)) / (0.5 * width)) ** 2
-        for k in range(nbars):
-            z = z + bh[k] * np.exp(-((x - bx[k]) ** 2 + (y - by[k]) ** 2) / (2 * bs[k] ** 2))
+        if bar_cutoff is None:
+            for k in range(nbars):
+                z = z + bh[k] * np.exp(-((x - bx[k]) ** 2 + (y - by[k]) ** 2) / (2 * bs[k] ** 2))
+        else:
+            order = np.argsort(x, kind='stable')
+            xs = x[order]
+            for k in range(nbars):
+                r = bar_cutoff * bs[k]
+                lo, hi = np.searchsorted(xs, [bx[k] - r, bx[k] + r])
+                i = order[lo:hi]
+                i = i[np.abs(y[i] - by[k]) <= r]
+                z[i] += bh[k] * np.exp(-((x[i] - bx[k]) ** 2 + (y[i] - by[k]) ** 2) / (2 * bs[k] ** 2))
         for (px, py) in pools:          # scour pools that site the turbines
             z = z - pool_depth * np.exp(-((x - px) ** 2 + (y - py) ** 2) / (2 * pool_sigma ** 2))
         return z
@@ -396,6 +408,32 @@ def braided_turbine_reach(nturbines=12, D=5.0, velocity=1.0, spacing=200.0, **kw
     sites = turbine_sites(nturbines, spacing=spacing,
                           **{k: kw[k] for k in ('length', 'width', 'amplitude', 'wavelength') if k in kw})
     m = braided_reach(pools=sites, velocity=velocity, **kw)
+    rows = []
+    for k, (x, y) in enumerate(sites):
+        r = max(D, 1.5 * max(m.dx, m.dy))
+        near = (np.abs(m.xy[0] - x) < r) & (np.abs(m.xy[1] - y) < r)
+        zbed = float(m.z[near].max())
+        rows.append(dict(name=f'T{k + 1:02d}', X=x, Y=y, D=D, hub_elevation_m=zbed + 1.0 + D / 2,
+                         minimum_bed_clearance_m=0.5, minimum_surface_clearance_m=0.5, control=THETIS_TABLE))
+    return m, rows
+
+
+def meander_turbine_reach(nturbines=64, nx=11547, ny=346, D=5.0):
+    """Large single reach for the roofline: 20 km x 600 m meandering river
+    (thalweg amplitude 90 m, wavelength 1200 m), 800 seeded bars, about 8M
+    triangles of 1.73 m quads, inflow Q=4000 m^3/s and nturbines rotors in
+    staggered arrays: pairs 40 m either side of the thalweg every 150 m,
+    alternate pairs shifted 75 m downstream."""
+    length, width, amplitude, wavelength = 20000.0, 600.0, 90.0, 1200.0
+    x0 = 0.3 * length
+    sites = []
+    for k in range(nturbines):
+        row, side = divmod(k, 2)
+        x = x0 + 150.0 * row + (75.0 if row % 2 else 0.0)
+        y = 0.5 * width + amplitude * np.sin(2 * np.pi * x / wavelength) + (40.0 if side else -40.0)
+        sites.append((float(x), float(y)))
+    m = braided_reach(nx=nx, ny=ny, length=length, width=width, amplitude=amplitude, wavelength=wavelength,
+                      nbars=800, q=4000.0, pools=sites, velocity=1.0, bar_cutoff=6.0)
     rows = []
     for k, (x, y) in enumerate(sites):
         r = max(D, 1.5 * max(m.dx, m.dy))
