@@ -44,7 +44,8 @@ __global__ void __launch_bounds__(256) kb_drag(BatchPlan P, const SlotView<R> *V
 }
 
 template <class R, int SCHEME>
-__global__ void __launch_bounds__(256) kb_prep(BatchPlan P, const SlotView<R> *V, const SlotRun *run, int stage)
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(sizeof(R) == 4 ? 6 : 1)))
+kb_prep(BatchPlan P, const SlotView<R> *V, const SlotRun *run, int stage)
 {
     const int s = find_slot(P.cell, P.nslot, blockIdx.x);
     if (!run[s].active) return;
@@ -86,7 +87,8 @@ __global__ void __launch_bounds__(256) kb_rotors(BatchPlan P, const SlotView<R> 
 }
 
 template <class R, int STAGE>
-__global__ void __launch_bounds__(256) kb_update(BatchPlan P, const SlotView<R> *V, const SlotRun *run)
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(sizeof(R) == 4 ? 6 : 1)))
+kb_update(BatchPlan P, const SlotView<R> *V, const SlotRun *run)
 {
     const int s = find_slot(P.cell, P.nslot, blockIdx.x);
     if (!run[s].active) return;

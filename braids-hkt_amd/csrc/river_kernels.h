@@ -306,7 +306,7 @@ __device__ __forceinline__ void face_launch_block(const MeshView<R> &m, const St
 }
 
 template <class R, int SCHEME, bool ROTORS>
-__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(5))) k_face_traces(MeshView<R> m, StateView<R> q, const double *__restrict__ z,
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(sizeof(R) == 4 ? 5 : 1))) k_face_traces(MeshView<R> m, StateView<R> q, const double *__restrict__ z,
                                                      Work<R> w, Control *ctl, TurbineView T, int stage, double t,
                                                      int nfb, int lead, int remap)
 {
@@ -460,7 +460,7 @@ __device__ __forceinline__ void k_river_update_body(MeshView<R> m, StateView<R> 
 }
 
 template <class R, int STAGE>
-__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(6))) k_river_update(MeshView<R> m, StateView<R> in, StateView<R> in0,
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(sizeof(R) == 4 ? 6 : 1))) k_river_update(MeshView<R> m, StateView<R> in, StateView<R> in0,
                                                       StateView<R> out, Work<R> w, TurbineView T, double *wlim,
                                                       DragView d, Control *ctl, int slot, double dt, double drag_dt,
                                                       int remap)
