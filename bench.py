@@ -46,15 +46,18 @@ STAGE_KINDS = {'ssp33': ('prep', 'face', 'drain', 'gather', 'ledger'),
                'hllc': ('rdrag', 'hrecon', 'hface', 'hupdate')}
 
 
-def algorithmic_bytes(kind, m, precision, scheme, nslot=0):
+def algorithmic_bytes(kind, m, precision, scheme, nslot=0, vel=False):
     """Compulsory bytes of one launch: every array element the kernel must
     read or write, counted once (neighbour re-reads, incidence duplication and
-    cache misses are implementation traffic and are excluded)."""
+    cache misses are implementation traffic and are excluded).  vel: the
+    stage input's velocities are read from the array its producer wrote (the
+    river path) instead of being formed from hu, hv."""
     r = 4 if precision == 32 else 8
     nc, nf, nadj = m.nc, m.nf, m.nadj
     state = 24                             # h, hu, hv: FP64 conserved state in both modes
+    inp = 8 + 2 * r if vel else state      # what prep and the faces read of the stage input
     if kind == 'face':
-        cell = state + 8 + 1 + (8 * r if scheme == 5 else 0)        # state, z, flags, gradient
+        cell = inp + 8 + 1 + (8 * r if scheme == 5 else 0)          # stage input, z, flags, gradient
         face = 3 * 4 + 2 * r + (4 * r if scheme == 5 else 2 * r) + 6 * r   # ids, normal, displacements, fluxes out
         return nc * cell + nf * face
     if kind == 'gather':
@@ -62,18 +65,19 @@ def algorithmic_bytes(kind, m, precision, scheme, nslot=0):
         face = 6 * r + r + 2 * r                                      # fluxes + speed, length, normal
         return nc * cell + nadj * 8 + nf * face
     if kind == 'prep':
-        cell = 4 + state + 8 + 4 * r + 4 + (8 * r if scheme == 5 else 0) + 1
+        cell = 4 + inp + 8 + 4 * r + 4 + (8 * r if scheme == 5 else 0) + 1
         return nc * cell + nadj * (8 + 4 * r)
     if kind == 'drain':
         return nc * (4 + 8 + r + 1 + r) + nadj * 8 + nf * (r + r)
     if kind == 'rtheta':
         return nc * (4 + 8 + r + 8) + nadj * 5 + nf * 3 * r          # offset, h, area, theta out; incidence; F1, speed, length
     if kind == 'rupdate':
-        # theta and the frequency are computed here from the face fluxes and speeds
-        cell = 4 + state + state / 2 + state + r + 8 / 2 + 1          # offset, in, base (stage 2), out, area, drag (stage 2), mask
-        return nc * cell + nadj * 8 + nf * (6 * r + r + 2 * r + 12)   # fluxes + speed, length, normal, owner/neighbour/bid
+        # theta and the frequency are computed here from the face fluxes and speeds;
+        # mean of the two stages (stage 1 writes the velocities, stage 2 reads the base and drag)
+        cell = 4 + state + state / 2 + state + r + 8 / 2 + 1 + r     # offset, in, base, out, area, drag, mask, velocities
+        return nc * cell + nadj * 12 + nf * (6 * r + r + (2 * r if r == 8 else 0))   # incidence; fluxes + speed, length, normal
     if kind == 'rdrag':
-        return nc * (state + state + 8)
+        return nc * (state + state + 8 + 2 * r)                       # base, work, drag value, velocities
     # HLLC (FP64 throughout): reference-layout geometry (xy, mid, normal in FP64)
     if kind == 'hrecon':
         cell = 4 + state + 8 + 16 + 24 + 64 + 32 + state + 1    # offset, work, z, xy, inverse; grad, half, qhalf, status
@@ -259,7 +263,7 @@ def main():
 
     dom_ms, dom_n = prof[dom]
     avg_s = dom_ms / dom_n / 1e3
-    abytes = algorithmic_bytes(dom, m, args.precision, args.scheme, nslot)
+    abytes = algorithmic_bytes(dom, m, args.precision, args.scheme, nslot, vel=args.integrator == 'river')
     achieved = abytes / avg_s / 1e9
     stage_ms = sum(v[0] / v[1] for k, v in warm.items() if k in STAGE_KINDS[args.integrator])
     if args.integrator == 'hllc':
