@@ -43,6 +43,7 @@ HBM_PEAK_GBS = 8000.0
 # (the open-boundary traces run inside the face launch)
 STAGE_KINDS = {'ssp33': ('prep', 'face', 'drain', 'gather', 'ledger'),
                'river': ('prep', 'face', 'rupdate'),
+               'morph': ('prep', 'face', 'rupdate'),
                'hllc': ('rdrag', 'hrecon', 'hface', 'hupdate')}
 
 
@@ -125,6 +126,8 @@ def build_workload(name):
         m = M.braided_reach()
     elif name == 'meander_reach_8m_64t':
         m, rows = M.meander_turbine_reach()
+    elif name == 'braided_reach_2m_24t':
+        m, rows = M.morph_turbine_reach()
     elif name == 'plain_channel_25k':
         m = M.plain_channel()
     else:
@@ -137,7 +140,25 @@ def cpu_baseline(m, rows, scheme, integrator, seconds=12.0):
     sample of the same workload: whole-mesh accepted steps from the same
     initial state until about `seconds` of CPU work."""
     steps = 0
-    if integrator == 'river':
+    if integrator == 'morph':
+        from oracle import river as OR
+        from oracle import exner as OX
+        o = OR.RiverOracle(m, scheme=scheme, cfl=.5, rows=rows)
+        ex = OX.ExnerOracle(m, m.z - 2.0, d50=0.02, porosity=0.4)
+        q = m.q0.copy(); z = m.z.copy(); t = 0.0; o.hint = 0.05
+        t0 = time.perf_counter()
+        while time.perf_counter() - t0 < seconds:
+            dt = o.hint
+            r = o.advance(q, z, dt, t)
+            if r['substeps'] <= 0:
+                break
+            b = ex.finish(r['q'], z, dt)
+            if not b['ok']:
+                o.hint = .5 * dt
+                continue
+            q, z = r['q'], b['z']; t += dt; steps += r['substeps']
+        what = f'SSPRK(2,2) substeps each followed by the Exner bed update, {len(rows)} rotors'
+    elif integrator == 'river':
         from oracle import river as OR
         o = OR.RiverOracle(m, scheme=scheme, cfl=.5, rows=rows)
         q = m.q0.copy(); t = 0.0; o.hint = 0.05
@@ -182,7 +203,7 @@ def main():
     ap.add_argument('--warmup', type=int, default=20)
     ap.add_argument('--precision', type=int, default=32, choices=(32, 64))
     ap.add_argument('--scheme', type=int, default=5, choices=(1, 5))
-    ap.add_argument('--integrator', default='river', choices=('river', 'ssp33', 'hllc'))
+    ap.add_argument('--integrator', default='river', choices=('river', 'ssp33', 'hllc', 'morph'))
     ap.add_argument('--workload', default=None)
     ap.add_argument('--block', type=int, default=256)
     ap.add_argument('--remap', type=int, default=1)
@@ -192,7 +213,8 @@ def main():
         print('bench.py measures one GPU: this build runs the hydraulic step on a single MI355X '
               '(multi-GPU sharding is a design note only); rerun with --gpus 1', file=sys.stderr)
         sys.exit(2)
-    workload = args.workload or ('braided_reach_500k' if args.integrator == 'ssp33' else 'braided_reach_500k_12t')
+    workload = args.workload or {'ssp33': 'braided_reach_500k', 'morph': 'braided_reach_2m_24t'}.get(
+        args.integrator, 'braided_reach_500k_12t')
     if args.integrator == 'hllc':
         args.precision = 64                  # the HLLC owner is FP64
 
@@ -220,6 +242,36 @@ def main():
 
         def run(n):
             return s.river_run(n)
+    elif args.integrator == 'morph':
+        # one accepted hydraulic substep, then the Exner bed update over it
+        # (MPM, d50 20 mm, porosity 0.4; the non-erodible layer 2 m below the
+        # initial bed); a rejected bed update rolls the interval back and it
+        # is retried at half length (Morphodynamics)
+        from braids_hip.river import RiverExplicit
+        from braids_hip.turbines import ThetisArray
+        from braids_hip.exner import ExnerUpdate, Morphodynamics
+        from braids_hip.solver import TrialRejected
+        s.set_state(m.q0, prepare=False)
+        owner = RiverExplicit(s, cfl=.5)
+        if rows:
+            arr = ThetisArray(m, rows).install(s)
+            nslot = sum(len(r.cells) for r in arr.rotors)
+        morph = Morphodynamics(owner, ExnerUpdate(s, m.z - 2.0, d50=0.02, porosity=0.4))
+        clock = {'t': 0.0}
+
+        def run(n):
+            acc = rej = 0
+            while acc < n:
+                dt = owner.step_hint if owner.step_hint and owner.step_hint < 1.0 else 0.05
+                try:
+                    rep = morph.step(dt, clock['t'])
+                except TrialRejected:
+                    owner.step_hint = .5 * dt
+                    rej += 1
+                    continue
+                clock['t'] += dt
+                acc += rep['hydraulic_substeps']; rej += rep['hydraulic_rejections']
+            return acc, rej
     elif args.integrator == 'hllc':
         from braids_hip.hllc import HLLCHydraulics
         from braids_hip.turbines import ThetisArray
@@ -263,7 +315,7 @@ def main():
 
     dom_ms, dom_n = prof[dom]
     avg_s = dom_ms / dom_n / 1e3
-    abytes = algorithmic_bytes(dom, m, args.precision, args.scheme, nslot, vel=args.integrator == 'river')
+    abytes = algorithmic_bytes(dom, m, args.precision, args.scheme, nslot, vel=args.integrator in ('river', 'morph'))
     achieved = abytes / avg_s / 1e9
     stage_ms = sum(v[0] / v[1] for k, v in warm.items() if k in STAGE_KINDS[args.integrator])
     if args.integrator == 'hllc':
@@ -298,8 +350,9 @@ def main():
         'data': 'synthetic',
         'config': {'workload': workload, 'cells': m.nc, 'faces': m.nf, 'scheme': args.scheme,
                    'integrator': {'river': 'SSPRK(2,2) river', 'ssp33': 'SSPRK(3,3)',
-                                  'hllc': 'HLLC MUSCL-Hancock'}[args.integrator],
-                   'cfl': 0.5 if args.integrator == 'river' else 0.9,
+                                  'hllc': 'HLLC MUSCL-Hancock',
+                                  'morph': 'SSPRK(2,2) river + Exner bed update (MPM) every substep'}[args.integrator],
+                   'cfl': 0.5 if args.integrator in ('river', 'morph') else 0.9,
                    'precision': ('fp32 reconstruction/flux arithmetic; fp64 conserved state, bed, free-surface '
                                  'differencing and ledgers') if args.precision == 32 else 'fp64',
                    'turbines': len(rows), 'turbines_active': active, 'turbines_inactive': inactive,

@@ -418,6 +418,30 @@ def braided_turbine_reach(nturbines=12, D=5.0, velocity=1.0, spacing=200.0, **kw
     return m, rows
 
 
+def morph_turbine_reach(nturbines=24, nx=3536, ny=282, seed=11, noise=0.05, q0=800.0, q1=2000.0,
+                        ramp_s=1800.0):
+    """Morphodynamic run: the braided reach at about 2M triangles (1.41 m
+    quads) with nturbines rotors 150 m apart on the thalweg, seeded random
+    bed perturbations (normal, sigma `noise` m; the water surface is kept,
+    so the depth absorbs them), the inlet hydrograph ramping linearly from
+    q0 to q1 m^3/s over ramp_s seconds (then held).  Returns (mesh, rows);
+    the run puts the non-erodible layer 2 m below the initial bed."""
+    m, rows = braided_turbine_reach(nturbines=nturbines, nx=nx, ny=ny, spacing=150.0, q=q0)
+    rng = np.random.default_rng(seed)
+    dz = rng.normal(0.0, noise, m.nc)
+    h = m.q0[0]
+    wet = h > 0.0
+    u = np.where(h > 0.5, m.q0[1] / np.maximum(h, 1e-12), 0.0)
+    m.z = m.z + dz
+    h = np.where(wet, np.maximum(h - dz, 0.0), 0.0)
+    m.q0 = np.vstack([h, np.where(h > 0.5, h * u, 0.0), np.zeros(m.nc)])
+    m.series_offset = np.array([0, 2, 2], np.int32)
+    m.series_time = np.array([0.0, float(ramp_s)])
+    m.series_q = np.array([float(q0), float(q1)])
+    m.series_stage = np.array([0.0, 0.0])
+    return m, rows
+
+
 def meander_turbine_reach(nturbines=64, nx=11547, ny=346, D=5.0):
     """Large single reach for the roofline: 20 km x 600 m meandering river
     (thalweg amplitude 90 m, wavelength 1200 m), 800 seeded bars, about 8M
