@@ -302,6 +302,132 @@ def structured_triangles(nx: int, ny: int, dx: float, dy: float, *,
 # Workloads (synthetic inputs; seeded)
 # ---------------------------------------------------------------------------
 
+def refined_triangles(nx: int, ny: int, dx: float, dy: float, refine: np.ndarray, r: int, *,
+                      bed: Callable[[np.ndarray, np.ndarray], np.ndarray],
+                      manning: Callable[[np.ndarray, np.ndarray], np.ndarray],
+                      depth0: Callable[[np.ndarray, np.ndarray, np.ndarray], np.ndarray],
+                      inflow_q: float, outlet_stage: float, series=None, name='refined') -> Mesh:
+    """Locally refined conforming triangulation of the structured_triangles
+    rectangle.  Quads flagged in refine[ny, nx] are split into r x r
+    sub-quads, each cut by a diagonal of alternating orientation.  A quad
+    that is not refined but shares a side with a refined one (a transition
+    quad) is cut into a fan of triangles from its centre to its corners and
+    to the r-1 fine vertices on each side it shares with a refined quad, so
+    every cell is a triangle and every face is shared whole by two cells.
+    The other quads are cut in two as in structured_triangles.  Boundary 0 is
+    the inlet at x=0 (mode 6), boundary 1 the stage outlet at x=nx*dx
+    (mode 2), y=0 and y=ny*dy are walls.  Vertices live on a lattice of half
+    fine steps (dx/2r, dy/2r), so quad centres are lattice points; faces are
+    found by matching the triangles' edges there, and the owner is the cell
+    that runs the edge counter-clockwise, so the normal points out of it."""
+    refine = np.asarray(refine, bool).reshape(ny, nx)
+    s2 = 2 * r                                       # lattice units per quad side
+    W = nx * s2 + 1
+    ux, uy = dx / s2, dy / s2
+
+    def ref_at(i, j):
+        return 0 <= i < nx and 0 <= j < ny and bool(refine[j, i])
+
+    pad = np.zeros((ny + 2, nx + 2), bool)
+    pad[1:-1, 1:-1] = refine
+    touch = (pad[1:-1, :-2] | pad[1:-1, 2:] | pad[:-2, 1:-1] | pad[2:, 1:-1]) & ~refine
+    plain = ~refine & ~touch
+    tris, quad_of = [], []
+    # plain quads: two triangles, diagonal of alternating orientation
+    J, I = np.nonzero(plain)
+    slash = ((I + J) % 2) == 0
+    a0, b0 = I * s2, J * s2
+    a1, b1 = a0 + s2, b0 + s2
+    tris.append(np.stack([a0, b0, a1, b0, np.where(slash, a1, a0), b1]))
+    tris.append(np.stack([np.where(slash, a0, a1), b0, a1, b1, a0, b1]))
+    quad_of += [np.stack([I, J]), np.stack([I, J])]
+    # refined quads: r x r sub-quads of two triangles each
+    JF, IF = np.nonzero(refine)
+    sa, sb = np.meshgrid(np.arange(r), np.arange(r))
+    fa = (IF[:, None] * r + sa.ravel()[None, :]).ravel()
+    fb = (JF[:, None] * r + sb.ravel()[None, :]).ravel()
+    fq = np.repeat(np.arange(IF.size), r * r)
+    fslash = ((fa + fb) % 2) == 0
+    fa, fb = 2 * fa, 2 * fb
+    tris.append(np.stack([fa, fb, fa + 2, fb, np.where(fslash, fa + 2, fa), fb + 2]))
+    tris.append(np.stack([np.where(fslash, fa, fa + 2), fb, fa + 2, fb + 2, fa, fb + 2]))
+    quad_of += [np.stack([IF[fq], JF[fq]]), np.stack([IF[fq], JF[fq]])]
+    # transition quads: a fan from the centre over the counter-clockwise boundary
+    JT, IT = np.nonzero(touch)
+    fan, fanq = [], []
+    for i, j in zip(IT.tolist(), JT.tolist()):
+        x0, y0, x1, y1 = i * s2, j * s2, (i + 1) * s2, (j + 1) * s2
+        pts = []
+        sides = (((x0, y0), (x1, y0), ref_at(i, j - 1)), ((x1, y0), (x1, y1), ref_at(i + 1, j)),
+                 ((x1, y1), (x0, y1), ref_at(i, j + 1)), ((x0, y1), (x0, y0), ref_at(i - 1, j)))
+        for (pa, pb), (qa, qb), split in sides:
+            n = r if split else 1
+            for k in range(n):
+                pts.append((pa + (qa - pa) * k // n, pb + (qb - pb) * k // n))
+        ca, cb = x0 + r, y0 + r
+        for k in range(len(pts)):
+            (pa, pb), (qa, qb) = pts[k], pts[(k + 1) % len(pts)]
+            fan.append((ca, cb, pa, pb, qa, qb))
+            fanq.append((i, j))
+    if fan:
+        tris.append(np.asarray(fan, np.int64).T)
+        quad_of.append(np.asarray(fanq, np.int64).T)
+    tri = np.concatenate(tris, axis=1).astype(np.int64)          # [6, nc] lattice vertices, CCW
+    quad_i, quad_j = np.concatenate(quad_of, axis=1)
+    nc = tri.shape[1]
+
+    # edges: each triangle's three, matched by their end points
+    pa = np.concatenate([tri[0], tri[2], tri[4]]); pb = np.concatenate([tri[1], tri[3], tri[5]])
+    qa = np.concatenate([tri[2], tri[4], tri[0]]); qb = np.concatenate([tri[3], tri[5], tri[1]])
+    ec = np.tile(np.arange(nc), 3)
+    kp, kq = pb * W + pa, qb * W + qa
+    lo, hi = np.minimum(kp, kq), np.maximum(kp, kq)
+    order = np.lexsort((ec, hi, lo))
+    lo, hi, ec, pa, pb, qa, qb = lo[order], hi[order], ec[order], pa[order], pb[order], qa[order], qb[order]
+    first = np.ones(lo.size, bool)
+    first[1:] = (lo[1:] != lo[:-1]) | (hi[1:] != hi[:-1])
+    start = np.nonzero(first)[0]
+    count = np.diff(np.append(start, lo.size))
+    if count.max() > 2:
+        raise ValueError('an edge is shared by more than two cells')
+    pair = count == 2
+    o = ec[start]
+    nbr = np.where(pair, ec[np.minimum(start + 1, lo.size - 1)], -1)
+    pa, pb, qa, qb = pa[start], pb[start], qa[start], qb[start]
+    px, py, qx, qy = pa * ux, pb * uy, qa * ux, qb * uy
+    ddx, ddy = qx - px, qy - py
+    length = np.hypot(ddx, ddy)
+    normal = np.vstack([ddy / length, -ddx / length])
+    mid = np.vstack([0.5 * (px + qx), 0.5 * (py + qy)])
+    bid = np.full(o.size, -1, np.int64)
+    ext = ~pair
+    bid[ext & (pa == 0) & (qa == 0)] = 0
+    bid[ext & (pa == nx * s2) & (qa == nx * s2)] = 1
+
+    cxv = tri[0::2] * ux
+    cyv = tri[1::2] * uy
+    cx = cxv.mean(axis=0); cy = cyv.mean(axis=0)
+    area = 0.5 * np.abs((cxv[1] - cxv[0]) * (cyv[2] - cyv[0]) - (cxv[2] - cxv[0]) * (cyv[1] - cyv[0]))
+    verts = np.empty((6, nc))
+    verts[0::2] = cxv; verts[1::2] = cyv
+    z = np.asarray(bed(cx, cy), np.float64)
+    n = np.asarray(manning(cx, cy), np.float64) * np.ones(nc)
+    h = np.maximum(np.asarray(depth0(cx, cy, z), np.float64), 0.0)
+    so = st = sq = ss = None
+    if series is not None:
+        so, st, sq, ss = series
+    m = Mesh(nc=nc, nf=o.size, nb=2, area=area, xy=np.vstack([cx, cy]), owner=o.astype(np.int32),
+             neighbour=nbr.astype(np.int32), bid=bid.astype(np.int32), normal=normal, length=length, mid=mid,
+             manning=n, z=z, q0=np.vstack([h, np.zeros(nc), np.zeros(nc)]),
+             mode=np.array([INLET_TOTAL_DISCHARGE, OUTLET_STAGE], np.int32),
+             forcing_q=np.array([float(inflow_q), 0.0]), forcing_stage=np.array([0.0, float(outlet_stage)]),
+             irregular=np.ones(nc, np.int32), ii=(quad_i + 1).astype(np.int32), jj=(quad_j + 1).astype(np.int32),
+             nx=nx, ny=ny, dx=dx, dy=dy, series_offset=so, series_time=st, series_q=sq, series_stage=ss,
+             name=name, verts=verts)
+    m.fine = (dx / r, dy / r)
+    return m
+
+
 def plain_channel(nx=500, ny=25, length=2000.0, width=100.0, slope=1e-3, sigma=0.05,
                   n=0.03, depth=2.0, q=400.0, seed=1) -> Mesh:
     """Straight rectangular channel 2000 m x 100 m, bed slope 1e-3 plus seeded
@@ -326,7 +452,7 @@ def braided_reach(nx=1768, ny=141, length=5000.0, width=400.0, slope=1e-3, ampli
                   wavelength=800.0, nbars=200, q=1500.0, seed=2, depth=3.0,
                   bar_height=(0.5, 2.0), bar_sigma=(15.0, 60.0), bank=4.0,
                   open_min_depth=None, pools=(), pool_depth=6.0, pool_sigma=25.0,
-                  velocity=0.0, bar_cutoff=None) -> Mesh:
+                  velocity=0.0, bar_cutoff=None, refine=None) -> Mesh:
     """Braided synthetic reach 5 km x 400 m: sinusoidal thalweg (amplitude
     60 m, wavelength 800 m), parabolic cross-section, 200 seeded Gaussian bars
     (height 0.5-2 m, sigma 15-60 m, centred away from the inlet and outlet
@@ -334,7 +460,9 @@ def braided_reach(nx=1768, ny=141, length=5000.0, width=400.0, slope=1e-3, ampli
     zones n in {0.025, 0.030, 0.035, 0.040}, inflow Q=1500 m^3/s.  The default
     2.83 m quads give about 500k triangles.  bar_cutoff (in bar sigmas)
     limits each bar to the cells within that distance of its centre (for
-    meshes of millions of cells; None: every bar over every cell)."""
+    meshes of millions of cells; None: every bar over every cell).  refine =
+    (sites, radius, r) splits every quad whose centre lies within radius of a
+    site into r x r sub-quads (refined_triangles)."""
     dx = length / nx; dy = width / ny
     rng = np.random.default_rng(seed)
     bx = rng.uniform(0.1 * length, 0.9 * length, nbars); by = rng.uniform(0, width, nbars)
@@ -368,8 +496,18 @@ def braided_reach(nx=1768, ny=141, length=5000.0, width=400.0, slope=1e-3, ampli
     def depth0(x, y, z):
         return (-slope * x + depth) - z
 
-    m = structured_triangles(nx, ny, dx, dy, bed=bed, manning=manning, depth0=depth0,
-                             inflow_q=q, outlet_stage=-slope * length + depth, name='braided_reach')
+    if refine is None:
+        m = structured_triangles(nx, ny, dx, dy, bed=bed, manning=manning, depth0=depth0,
+                                 inflow_q=q, outlet_stage=-slope * length + depth, name='braided_reach')
+    else:
+        sites, radius, r = refine
+        qx = (np.arange(nx) + 0.5) * dx
+        qy = (np.arange(ny) + 0.5) * dy
+        near = np.zeros((ny, nx), bool)
+        for (px, py) in sites:
+            near |= ((qx[None, :] - px) ** 2 + (qy[:, None] - py) ** 2) <= radius ** 2
+        m = refined_triangles(nx, ny, dx, dy, near, r, bed=bed, manning=manning, depth0=depth0,
+                              inflow_q=q, outlet_stage=-slope * length + depth, name='braided_reach_refined')
     # The inlet and outlet sections span the active channel only: end faces
     # whose owner starts shallower than open_min_depth (the bank margins)
     # are walls, so no open face ever feeds a nearly dry cell.
@@ -398,16 +536,17 @@ def turbine_sites(n=12, length=5000.0, width=400.0, amplitude=60.0, wavelength=8
     return [(float(x), float(y)) for x, y in zip(xs, ys)]
 
 
-def braided_turbine_reach(nturbines=12, D=5.0, velocity=1.0, spacing=200.0, **kw):
+def braided_turbine_reach(nturbines=12, D=5.0, velocity=1.0, spacing=200.0, refine=None, **kw):
     """The braided reach with nturbines D-diameter rotors (tabulated Ct/Cp
     vs speed) sited in scour pools on the thalweg, and an initial stream of
     `velocity` m/s (in water deeper than 0.5 m) so the rotors start covered
-    and running.  Returns the mesh
-    and the turbine rows (rotor bottom 1 m above the highest bed cell
-    around the site)."""
+    and running.  refine = (radius, r): the quads within radius of a rotor
+    are split r x r.  Returns the mesh and the turbine rows (rotor bottom 1 m
+    above the highest bed cell around the site)."""
     sites = turbine_sites(nturbines, spacing=spacing,
                           **{k: kw[k] for k in ('length', 'width', 'amplitude', 'wavelength') if k in kw})
-    m = braided_reach(pools=sites, velocity=velocity, **kw)
+    m = braided_reach(pools=sites, velocity=velocity,
+                      refine=None if refine is None else (sites, refine[0], refine[1]), **kw)
     rows = []
     for k, (x, y) in enumerate(sites):
         r = max(D, 1.5 * max(m.dx, m.dy))
@@ -418,7 +557,7 @@ def braided_turbine_reach(nturbines=12, D=5.0, velocity=1.0, spacing=200.0, **kw
     return m, rows
 
 
-def morph_turbine_reach(nturbines=24, nx=3536, ny=282, seed=11, noise=0.05, q0=800.0, q1=2000.0,
+def morph_turbine_reach(nturbines=24, nx=3536, ny=282, seed=11, noise=0.02, q0=800.0, q1=2000.0,
                         ramp_s=1800.0):
     """Morphodynamic run: the braided reach at about 2M triangles (1.41 m
     quads) with nturbines rotors 150 m apart on the thalweg, seeded random

@@ -74,7 +74,7 @@ __global__ void __launch_bounds__(256) kb_face(BatchPlan P, const SlotView<R> *V
     face_launch_block<R, SCHEME, false>(v.m, in, v.z, v.w, v.ctl, v.T, stage, t, nfb, lead, lb, 1);
 }
 
-// The rotors of every scenario sampling its stage input (one wavefront each;
+// The rotors of every scenario sampling its stage input (one workgroup each;
 // a launch of its own, one per stage for the whole batch).
 template <class R>
 __global__ void __launch_bounds__(256) kb_rotors(BatchPlan P, const SlotView<R> *V, const SlotRun *run, int stage)
@@ -82,8 +82,8 @@ __global__ void __launch_bounds__(256) kb_rotors(BatchPlan P, const SlotView<R> 
     const int s = find_slot(P.rot, P.nslot, blockIdx.x);
     if (!run[s].active) return;
     const SlotView<R> &v = V[s];
-    const int a = (blockIdx.x - P.rot[s]) * (blockDim.x >> 6) + (threadIdx.x >> 6);
-    if (a < v.T.nrot) turbine_sample_wave<R>(v.T, v.m, stage == 0 ? v.work : v.q1, v.z, stage, a, threadIdx.x & 63);
+    const int a = blockIdx.x - P.rot[s];      // one workgroup per rotor
+    if (a < v.T.nrot) turbine_sample<R>(v.T, v.m, stage == 0 ? v.work : v.q1, v.z, stage, a);
 }
 
 template <class R, int STAGE>

@@ -897,7 +897,7 @@ struct Batch : BatchBase {
             hr[i] = slots[i].run;
             nz[i] = v.nzero;
             bl[i] = (unsigned int *)x->ctl;
-            const int nrb = (v.T.nrot + 3) / 4;
+            const int nrb = v.T.nrot;         // kb_rotors: one workgroup per rotor
             pre[0 * (n + 1) + i + 1] = pre[0 * (n + 1) + i] + pad8(v.ncb);
             pre[1 * (n + 1) + i + 1] = pre[1 * (n + 1) + i] + pad8(face_lead(x->nb, 0) + (x->nf + 255) / 256);
             pre[2 * (n + 1) + i + 1] = pre[2 * (n + 1) + i] + x->nb;

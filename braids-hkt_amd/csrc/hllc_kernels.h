@@ -3,7 +3,7 @@
 // FP64.  One substep of size dt is, with `work` the Strang-dragged base:
 //   k_river_drag    work = drag(base, dt/2)
 //   kh_reconstruct  cells: limited LS gradient, Hancock predictor -> half,
-//                   qhalf; extra workgroups: one wavefront per rotor samples
+//                   qhalf; extra workgroups: one workgroup per rotor samples
 //                   `work` (the predictor's momentum source)
 //   kh_predict      footprint cells: the rotor sink over dt/2 into qhalf and
 //                   the half-step velocities (hllc_predict_source)
@@ -11,7 +11,7 @@
 //                   a rotor array; extra workgroups: the first-order traces
 //                   of qhalf at t + dt/2 per open boundary, then the exterior
 //                   flux of its faces (HLLC between the owner's predictor
-//                   state and the trace), and one wavefront per rotor
+//                   state and the trace), and one workgroup per rotor
 //                   sampling qhalf
 //   kh_update       cells: donor factor and face-sum frequency (max-reduced,
 //                   NaN kept; factors of the workgroup's cells in LDS), then
@@ -59,8 +59,8 @@ __global__ void __launch_bounds__(256) kh_reconstruct(HllcView H, MeshView<doubl
 {
     if ((int)blockIdx.x < lead) {
         if constexpr (ROTORS) {
-            const int a = (int)blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
-            if (a < T.nrot) turbine_sample_wave<double>(T, m, work, z, 0, a, threadIdx.x & 63);
+            const int a = (int)blockIdx.x;    // one workgroup per rotor
+            if (a < T.nrot) turbine_sample<double>(T, m, work, z, 0, a);
         }
         return;
     }
@@ -142,7 +142,7 @@ __device__ __forceinline__ void hllc_open_face(const HllcView &H, const StateVie
 // Faces.  The first `lead` workgroups (a multiple of 8) go first: one per open
 // boundary - its first-order traces of qhalf at the half time
 // (k_traces_body), then the exterior fluxes of its faces - and, with ROTORS,
-// one wavefront per rotor sampling qhalf.  The face workgroups follow and
+// one workgroup per rotor sampling qhalf.  The face workgroups follow and
 // skip open faces.
 template <bool ROTORS>
 __global__ void __launch_bounds__(256) kh_faces(HllcView H, MeshView<double> m, StateView<double> qh,
@@ -159,8 +159,8 @@ __global__ void __launch_bounds__(256) kh_faces(HllcView H, MeshView<double> m, 
                 hllc_open_face(H, qh, w.trace, m.bnd_faces[k], gravity, dry);
         } else if constexpr (ROTORS) {
             // the rotors sample qhalf: the momentum source of the corrector
-            const int a = (b - m.nb) * (blockDim.x >> 6) + (threadIdx.x >> 6);
-            if (a < T.nrot) turbine_sample_wave<double>(T, m, qh, z, 1, a, threadIdx.x & 63);
+            const int a = b - m.nb;           // one workgroup per rotor
+            if (a < T.nrot) turbine_sample<double>(T, m, qh, z, 1, a);
         }
         return;
     }

@@ -116,24 +116,50 @@ __device__ __forceinline__ double wave_min_d(double v)
     return v;
 }
 
-// One wavefront per rotor: CoveredRotor._operating_state gating
-// (river_turbine_flow.py:136-180) on the rotor-plane surface/bed samples and
-// the footprint means, stop_below_cut_in (river_turbine_thetis.py:219-240),
-// then cell_terms (:182-218) per footprint cell.  All reductions are
-// butterfly shuffles, so every lane holds the rotor's statistics and takes
-// the same gating decision; the statistics the reference moves to the host
-// (5 + 6 scalars per rotor and stage) stay on the device.
-template <class R>
-__device__ void turbine_sample_wave(const TurbineView &T, const MeshView<R> &m, StateView<R> q,
-                                    const double *__restrict__ z, int slot, int a, int lane)
+// Workgroup-wide sum and minimum (every thread gets the result): wave
+// butterflies, then the per-wave partials through LDS in a fixed order.
+__device__ __forceinline__ double group_sum(double v, double *sh)
 {
+    v = wave_sum_d(v);
+    __syncthreads();
+    if ((threadIdx.x & 63) == 0) sh[threadIdx.x >> 6] = v;
+    __syncthreads();
+    double s = 0.;
+    for (int i = 0; i < (int)(blockDim.x >> 6); ++i) s += sh[i];
+    return s;
+}
+__device__ __forceinline__ double group_min(double v, double *sh)
+{
+    v = wave_min_d(v);
+    __syncthreads();
+    if ((threadIdx.x & 63) == 0) sh[threadIdx.x >> 6] = v;
+    __syncthreads();
+    double s = sh[0];
+    for (int i = 1; i < (int)(blockDim.x >> 6); ++i) s = fmin(s, sh[i]);
+    return s;
+}
+
+// One workgroup per rotor (every thread of the workgroup calls this):
+// CoveredRotor._operating_state gating (river_turbine_flow.py:136-180) on the
+// rotor-plane surface/bed samples and the footprint means,
+// stop_below_cut_in (river_turbine_thetis.py:219-240), then cell_terms
+// (:182-218) per footprint cell.  The reductions are workgroup-wide, so every
+// thread holds the rotor's statistics and takes the same gating decision; the
+// statistics the reference moves to the host (5 + 6 scalars per rotor and
+// stage) stay on the device.  A footprint at 0.5 m cells is about 400 cells.
+template <class R>
+__device__ void turbine_sample(const TurbineView &T, const MeshView<R> &m, StateView<R> q,
+                               const double *__restrict__ z, int slot, int a)
+{
+    __shared__ double sh[16];
+    const int lane = threadIdx.x, step = blockDim.x;
     const int nc = m.nc;
     const double dry = 1e-6;                 // CoveredRotor(dry=1e-6)
     const double mdry = (double)m.dry;
     // rotor plane: wet LS surface gradient and full LS bed gradient of the
     // row's cell (hkt_wet_gradients, river_turbine_flow.h:12-33)
     double low = 1e300, high = -1e300;
-    for (int k = T.plane_off[a] + lane; k < T.plane_off[a + 1]; k += 64) {
+    for (int k = T.plane_off[a] + lane; k < T.plane_off[a + 1]; k += step) {
         const int i = T.plane_cell[k];
         double xx = 0., xy = 0., yy = 0., bx = 0., by = 0., bz0 = 0., bz1 = 0.;
         const bool wet = q.h[i] > mdry;
@@ -159,11 +185,11 @@ __device__ void turbine_sample_wave(const TurbineView &T, const MeshView<R> &m, 
         low = fmin(low, fmin(surface, eta));
         high = fmax(high, fmax(bed, z[i]));
     }
-    low = wave_min_d(low);
-    high = -wave_min_d(-high);
+    low = group_min(low, sh);
+    high = -group_min(-high, sh);
     // footprint means
     double u1 = 0., depth = 0., smin = 1e300, anydry = 0.;
-    for (int k = T.foot_off[a] + lane; k < T.foot_off[a + 1]; k += 64) {
+    for (int k = T.foot_off[a] + lane; k < T.foot_off[a + 1]; k += step) {
         const int c = T.foot_cell[k];
         const double h = q.h[c], hs = fmax(h, dry);
         const double u = q.hu[c] / hs, v = q.hv[c] / hs, sp = hypot(u, v);
@@ -172,10 +198,10 @@ __device__ void turbine_sample_wave(const TurbineView &T, const MeshView<R> &m, 
         depth += T.foot_norm[k] * h;
         smin = fmin(smin, sp);
     }
-    u1 = wave_sum_d(u1);
-    depth = wave_sum_d(depth);
-    smin = wave_min_d(smin);
-    anydry = -wave_min_d(-anydry);
+    u1 = group_sum(u1, sh);
+    depth = group_sum(depth, sh);
+    smin = group_min(smin, sh);
+    anydry = -group_min(-anydry, sh);
     const double margin = low - T.top[a], clearance = T.bottom[a] - high;
     const int t0 = T.table_off[a], nt = T.table_off[a + 1] - t0;
     const double tol = T.tol[a];
@@ -190,7 +216,7 @@ __device__ void turbine_sample_wave(const TurbineView &T, const MeshView<R> &m, 
     const double D = T.D[a], Aro = 3.14159265358979323846 * (D * D) / 4;
     double F = 0., work = 0., P = 0., inc = 0., ctm = 0., cpm = 0.;
     int err = 0;
-    for (int k = T.foot_off[a] + lane; k < T.foot_off[a + 1]; k += 64) {
+    for (int k = T.foot_off[a] + lane; k < T.foot_off[a + 1]; k += step) {
         double kk = 0.;
         if (reason == 0) {
             const int c = T.foot_cell[k];
@@ -217,8 +243,8 @@ __device__ void turbine_sample_wave(const TurbineView &T, const MeshView<R> &m, 
         T.kslot[k] = kk;
     }
     if (err) atomicOr(T.error, 1 << slot);
-    F = wave_sum_d(F); work = wave_sum_d(work); P = wave_sum_d(P);
-    inc = wave_sum_d(inc); ctm = wave_sum_d(ctm); cpm = wave_sum_d(cpm);
+    F = group_sum(F, sh); work = group_sum(work, sh); P = group_sum(P, sh);
+    inc = group_sum(inc, sh); ctm = group_sum(ctm, sh); cpm = group_sum(cpm, sh);
     if (lane == 0) {
         double *d = T.diag + ((size_t)slot * T.nrot + a) * TURBINE_DIAG;
         d[0] = reason == 0; d[1] = reason; d[2] = low; d[3] = high; d[4] = u1; d[5] = depth;
@@ -275,7 +301,7 @@ __device__ __forceinline__ double river_theta(const MeshView<R> &m, const Work<R
 // 8, so that the face workgroups keep their XCD grouping) are dispatched
 // first and overlap the faces: [0, nb) one open boundary each - its traces
 // (k_traces_body), then the fluxes of its faces - and, with ROTORS, one
-// wavefront per rotor sampling the stage input; the rest of the lead is idle.
+// workgroup per rotor sampling the stage input; the rest of the lead is idle.
 // Then nfb face workgroups (open faces skipped).  The rotor path needs more
 // registers than the faces: launches without rotors use ROTORS = false.
 template <class R, int SCHEME, bool ROTORS>
@@ -293,8 +319,8 @@ __device__ __forceinline__ void face_launch_block(const MeshView<R> &m, const St
                 else face_one<R, SCHEME>(m, q, z, w, m.bnd_faces[k]);
             }
         } else if constexpr (ROTORS) {
-            const int a = (lb - m.nb) * (blockDim.x >> 6) + (threadIdx.x >> 6);
-            if (a < T.nrot) turbine_sample_wave<R>(T, m, q, z, stage, a, threadIdx.x & 63);
+            const int a = lb - m.nb;          // one workgroup per rotor
+            if (a < T.nrot) turbine_sample<R>(T, m, q, z, stage, a);
         }
         return;
     }

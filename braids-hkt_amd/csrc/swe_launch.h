@@ -103,7 +103,7 @@ struct BatchPlan {
     int ncell, nface, nbnd, nzero, nrot;       // grid sizes (block totals)
     const int *cell;            // [nslot+1] prefix of cell blocks (each padded to a multiple of 8)
     const int *face;            // face blocks and one block per open boundary (padded to 8)
-    const int *rot;             // rotor blocks (four rotors each)
+    const int *rot;             // rotor blocks (one rotor each)
     const int *bnd;             // boundaries (one block each)
     const int *zero;            // control-clearing blocks
 };

@@ -54,8 +54,8 @@ enum : int { RQ_HMAX = 0, RQ_PEAK = 1 };
 __host__ __device__ inline int rpart_index(int quantity, int stage) { return (quantity * 4 + stage) * RPART; }
 
 // Workgroups ahead of the faces in a face launch: one per open boundary and
-// one per four rotor wavefronts, padded to a multiple of 8.
-__host__ __device__ inline int face_lead(int nb, int nrot) { return (nb + (nrot + 3) / 4 + 7) & ~7; }
+// one per rotor, padded to a multiple of 8.
+__host__ __device__ inline int face_lead(int nb, int nrot) { return (nb + nrot + 7) & ~7; }
 
 template <class R> struct Work {
     unsigned long long *rpart;   // [2][4][RPART] partial maxima (bits of non-negative doubles)
@@ -101,7 +101,7 @@ struct DragView {
     const double *values;   // [nc] roughness per cell
 };
 
-constexpr int TURBINE_DIAG = 16;   // per rotor and stage, see turbine_sample_wave
+constexpr int TURBINE_DIAG = 16;   // per rotor and stage, see turbine_sample
 
 struct TurbineView {
     int nrot, nslot, nunion;

@@ -72,3 +72,34 @@ def test_renumbered_oracle_run_maps_back():
     qb = r.to_original_cells(qb)
     for j in range(3):
         assert np.linalg.norm(qa[j] - qb[j]) <= 1e-10 * np.linalg.norm(qa[j])
+
+
+def test_refined_mesh_contract():
+    """A locally refined conforming triangulation: every cell is a triangle
+    and closes (sum of L n over its faces is zero), normals point out of the
+    owner, the areas tile the rectangle, each transition quad is a fan of
+    4 + (r-1) x (refined sides) triangles."""
+    nx, ny, r = 12, 6, 3
+    ref = np.zeros((ny, nx), bool)
+    ref[2:4, 4:7] = True
+    ref[0, 0] = True
+    m = M.refined_triangles(nx, ny, 2.0, 2.0, ref, r, bed=lambda x, y: 0 * x, manning=lambda x, y: 0.03 + 0 * x,
+                            depth0=lambda x, y, z: 1.0 - z, inflow_q=1.0, outlet_stage=1.0)
+    pad = np.pad(ref, 1)
+    sides = (pad[1:-1, :-2].astype(int) + pad[1:-1, 2:] + pad[:-2, 1:-1] + pad[2:, 1:-1]) * ~ref
+    fan = (sides > 0)
+    assert m.nc == 2 * (~ref & ~fan).sum() + 2 * r * r * ref.sum() + (4 * fan + (r - 1) * sides).sum()
+    assert np.isclose(m.area.sum(), nx * ny * 4.0, rtol=1e-14)
+    vec = m.length[None, :] * m.normal
+    closure = np.zeros((2, m.nc))
+    inner = m.neighbour >= 0
+    for a in range(2):
+        closure[a] += np.bincount(m.owner, weights=vec[a], minlength=m.nc)
+        closure[a] -= np.bincount(m.neighbour[inner], weights=vec[a][inner], minlength=m.nc)
+    assert np.abs(closure).max() < 1e-12
+    assert (((m.mid - m.xy[:, m.owner]) * m.normal).sum(0) > 0).all()
+    assert (np.diff(m.offset) == 3).all()
+    assert m.area.min() > 0
+    assert np.isclose(m.length[m.bid == 0].sum(), ny * 2.0) and np.isclose(m.length[m.bid == 1].sum(), ny * 2.0)
+    rm = m.renumbered('hilbert')
+    assert np.array_equal(np.sort(rm.cell_perm), np.arange(m.nc))

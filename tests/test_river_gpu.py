@@ -42,6 +42,24 @@ def channel_turbines():
     return m, [rotor(80.3, 20.1, -0.08 + 3.3), rotor(120.7, 13.3, -0.12 + 3.3)]
 
 
+def refined_turbines():
+    """Channel of 4 m quads refined 4x4 (1 m) within 12 m of two rotors, with
+    fans of triangles in the transition quads (a conforming graded mesh)."""
+    nx, ny, dx = 50, 10, 4.0
+    sites = [(80.3, 20.1), (120.7, 13.3)]
+    qx = (np.arange(nx) + 0.5) * dx
+    qy = (np.arange(ny) + 0.5) * dx
+    near = np.zeros((ny, nx), bool)
+    for (px, py) in sites:
+        near |= (qx[None, :] - px) ** 2 + (qy[:, None] - py) ** 2 <= 12.0 ** 2
+    m = M.refined_triangles(nx, ny, dx, dx, near, 4,
+                            bed=lambda x, y: -1e-3 * x + 0.05 * np.sin(x / 7.0) * np.cos(y / 5.0),
+                            manning=lambda x, y: np.full_like(x, 0.03), depth0=lambda x, y, z: np.full_like(x, 8.0),
+                            inflow_q=1.5 * 8.0 * 40.0, outlet_stage=-0.2 + 8.0, name='refined_channel')
+    m.q0[1] = m.q0[0] * 1.5
+    return m, [rotor(80.3, 20.1, -0.08 + 3.3), rotor(120.7, 13.3, -0.12 + 3.3)]
+
+
 def braided_small():
     m = M.braided_reach(nx=60, ny=12, length=300.0, width=60.0, amplitude=8.0, wavelength=150.0,
                         nbars=12, q=30.0, depth=1.0, bar_height=(0.3, 1.0), bar_sigma=(3.0, 10.0), bank=2.0)
@@ -53,6 +71,7 @@ def braided_small():
 # agrees to 7e-15, test_river_fp64_one_substep_matches_oracle; the transverse
 # momentum hv reaches 2e-10 relative L2 at 3 s, 7e-12 at 2 s).
 CASES = {'channel_flow': (channel_flow, 5, 3), 'channel_turbines': (channel_turbines, 5, 3),
+         'refined_turbines': (refined_turbines, 5, 3),
          'braided_s5': (braided_small, 5, 2), 'braided_s1': (braided_small, 1, 3)}
 
 
