@@ -42,8 +42,8 @@ METRIC = 'cell-updates/s; batched scenarios/h + p50/p99 latency vs plain, rel L2
 HBM_PEAK_GBS = 8000.0
 # (the open-boundary traces run inside the face launch)
 STAGE_KINDS = {'ssp33': ('prep', 'face', 'drain', 'gather', 'ledger'),
-               'river': ('prep', 'face', 'rupdate'),
-               'morph': ('prep', 'face', 'rupdate'),
+               'river': ('prep', 'face', 'rupdate', 'traces', 'rfused'),
+               'morph': ('prep', 'face', 'rupdate', 'traces', 'rfused'),
                'hllc': ('rdrag', 'hrecon', 'hface', 'hupdate')}
 
 
@@ -79,6 +79,16 @@ def algorithmic_bytes(kind, m, precision, scheme, nslot=0, vel=False):
         return nc * cell + nadj * 12 + nf * (6 * r + r + (2 * r if r == 8 else 0))   # incidence; fluxes + speed, length, normal
     if kind == 'rdrag':
         return nc * (state + state + 8 + 2 * r)                       # base, work, drag value, velocities
+    if kind == 'rfused':
+        # faces: stage input (state and velocities), bed, flags, gradient of both sides; face geometry;
+        # update: base (stage 2), out, area, drag (stage 2), mask, velocities out (stage 1); incidence
+        # with its table slots; no face record leaves the workgroup
+        cell = state + 2 * r + 8 + 1 + 8 * r + 4 + state / 2 + state + r + 8 / 2 + 1 + r
+        return nc * cell + nadj * 16 + nf * (12 + 2 * r + 4 * r + r + 4)
+    if kind == 'traces':
+        bf = getattr(m, 'boundary_faces', None)
+        nbf = 0 if bf is None else int(bf.size)
+        return nbf * (state + 8 + 8 * r + 12 + 2 * r + 4 * r + 6 * r + 3 * r)
     # HLLC (FP64 throughout): reference-layout geometry (xy, mid, normal in FP64)
     if kind == 'hrecon':
         cell = 4 + state + 8 + 16 + 24 + 64 + 32 + state + 1    # offset, work, z, xy, inverse; grad, half, qhalf, status

@@ -13,7 +13,7 @@ _P = ctypes.c_void_p
 BRAIDS_OK = 0
 BRAIDS_REJECTED = 1
 KERNEL_KINDS = ('prep', 'traces', 'face', 'drain', 'gather', 'ledger',
-                'rdrag', 'rtheta', 'turbine', 'rupdate',
+                'rdrag', 'rfused', 'turbine', 'rupdate',
                 'hrecon', 'hface', 'hbudget', 'hupdate')
 TURBINE_DIAG = 16
 RIVER_REASONS = {0: None, 1: 'spatial', 2: 'cfl', 3: 'boundary', 4: 'nonfinite', 5: 'negative', 6: 'turbine'}
@@ -70,6 +70,7 @@ SIGNATURES = {
     'braids_create': (_I, [ctypes.POINTER(BraidsMesh), ctypes.POINTER(BraidsOptions), _P, ctypes.POINTER(_P)]),
     'braids_destroy': (_I, [_P]),
     'braids_set_stream': (_I, [_P, _P]),
+    'braids_river_fused': (_I, [_P, _I]),
     'braids_set_state': (_I, [_P, _P, _I]),
     'braids_get_state': (_I, [_P, _P]),
     'braids_set_bed': (_I, [_P, _P]),

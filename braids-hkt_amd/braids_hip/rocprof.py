@@ -24,7 +24,7 @@ import re
 KINDS = {'k_prep': 'prep', 'k_prep_ls64': 'prep', 'k_traces': 'traces', 'k_face': 'face', 'k_face_traces': 'face',
          'k_close': 'close', 'k_finalize_mirror': 'finalize',
          'k_face_f32': 'face', 'k_drain': 'drain', 'k_gather': 'gather', 'k_ledger': 'ledger',
-         'k_river_theta': 'rtheta', 'k_river_update': 'rupdate', 'k_river_drag': 'rdrag',
+         'k_river_fused': 'rfused', 'k_river_update': 'rupdate', 'k_river_drag': 'rdrag',
          'k_river_ledger': 'ledger', 'kh_reconstruct': 'hrecon', 'kh_faces': 'hface', 'kh_budget': 'hbudget',
          'kh_update': 'hupdate', 'kh_predict': 'hpredict', 'k_stream_copy': 'copy'}
 

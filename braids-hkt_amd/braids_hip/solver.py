@@ -96,6 +96,12 @@ class HydraulicSolver:
         raw = getattr(stream, 'cuda_stream', stream)
         _lib.check(self.L.braids_set_stream(self.h, ctypes.c_void_p(int(raw) if raw else 0)), 'braids_set_stream')
 
+    def river_fused(self, enable=True):
+        """FP32 form: run each river stage as a fused face-and-update launch
+        (face records in LDS) or as separate face and update launches (the
+        default, faster on MI355X); the results are identical."""
+        _lib.check(self.L.braids_river_fused(self.h, int(bool(enable))), 'braids_river_fused')
+
     # -- state -------------------------------------------------------------
     def set_state(self, q, prepare=True):
         q = np.ascontiguousarray(q, dtype=np.float64).reshape(3, self.nc)

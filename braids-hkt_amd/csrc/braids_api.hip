@@ -112,6 +112,7 @@ struct SolverBase {
     virtual int native_rates(const double *q, const double *z, double dt, double t, double *rate, double *water, int *code) = 0;
     virtual int face_speed(double *speed) = 0;
     virtual int river_configure(double cfl, int drag_mode, const double *drag_values) = 0;
+    virtual int river_fused(int enable) = 0;
     virtual int river_step(double dt, double t, braids_river_result *res) = 0;
     virtual int turbines_set(const braids_turbines *t) = 0;
     virtual int hllc_configure(double cfl, int drag_mode, const double *drag_values, const double *inverse,
@@ -154,23 +155,26 @@ struct SolverBase {
         set_events(on ? events.data() : nullptr, prof_mask);
         return BRAIDS_OK;
     }
-    // path: 0 SSPRK(3,3), 1 river SSPRK(2,2), 2 HLLC Hancock
+    // path: 0 SSPRK(3,3), 1 river SSPRK(2,2), 2 HLLC Hancock, 3 river SSPRK(2,2)
+    // with the fused face-and-update launch
     void collect(int slots, int path = 0)
     {
         if (events.empty()) return;
         // the open-boundary traces run inside the face launch (k_face_traces, kh_faces)
-        // (the river path's theta and the HLLC budget run inside their update launches)
-        static const bool kinds[3][KK_COUNT] = {
+        // (the river path's theta and the HLLC budget run inside their update launches);
+        // on the fused river path the traces and rotors have a launch of their own
+        static const bool kinds[4][KK_COUNT] = {
             {true, false, true, true, true, true, false, false, false, false, false, false, false, false},
             {true, false, true, false, false, false, true, false, false, true, false, false, false, false},
-            {false, false, false, false, false, false, true, false, false, false, true, true, false, true}};
-        const bool river = path == 1;
+            {false, false, false, false, false, false, true, false, false, false, true, true, false, true},
+            {true, true, false, false, false, false, true, true, false, false, false, false, false, false}};
+        const bool river = path == 1 || path == 3;
         for (int slot = 0; slot < slots; ++slot)
             for (int k = 0; k < KK_COUNT; ++k) {
                 if (!kinds[path][k]) continue;
                 if (river && k == KK_RDRAG && slot > 0) continue;
                 if (river && k == KK_TURBINE && nrot == 0) continue;
-                if ((k == KK_TRACES || k == KK_LEDGER) && nb == 0) continue;
+                if ((k == KK_TRACES || k == KK_LEDGER) && nb == 0 && !(path == 3 && nrot > 0)) continue;
                 if (!((prof_mask >> k) & 1)) continue;
                 float ms = 0;
                 const hipError_t e = hipEventElapsedTime(&ms, events[(slot * KK_COUNT + k) * 2],
@@ -209,7 +213,9 @@ struct Solver : SolverBase {
     HllcView hv{};                     // HLLC Hancock substep (FP64 solvers)
     int hllc_pressure = -1;            // -1: the low-Froude correction follows the rotor array
     std::vector<double> mid_host;
-    std::vector<int> signs_host, bid_host, neighbour_host;
+    std::vector<int> signs_host, bid_host, neighbour_host, owner_host, offset_host, faces_host;
+    FusedPlan fplan{};
+    bool fused_built = false, fused_on = false;     // braids_river_fused (off by default)
     void set_events(hipEvent_t *ev, int mask) override { cfg.ev = ev; cfg.ev_mask = mask; profiling = ev != nullptr; }
 
     ~Solver() override
@@ -329,6 +335,9 @@ struct Solver : SolverBase {
         signs_host.assign(mesh->signs, mesh->signs + nadj);
         bid_host.assign(mesh->bid, mesh->bid + nf);
         neighbour_host.assign(mesh->neighbour, mesh->neighbour + nf);
+        owner_host.assign(mesh->owner, mesh->owner + nf);
+        offset_host.assign(mesh->offset, mesh->offset + nc + 1);
+        faces_host.assign(mesh->faces, mesh->faces + nadj);
         for (auto &s : st) {
             if ((rc = alloc(s.h, nc))) return rc;
             if ((rc = alloc(s.hu, nc))) return rc;
@@ -456,6 +465,88 @@ struct Solver : SolverBase {
         river_cfl = cfl;
         return drag_configure(drag_mode, drag_values);
     }
+    // The tables of the fused face-and-update launch (FP32 form): per workgroup
+    // of 256 cells, its ring-1 cells and the faces of both, within FUSED_MAXF
+    // faces and FUSED_MAXH ring-1 cells.  Returns false (tables not built) when
+    // a workgroup exceeds them.
+    bool fused_build()
+    {
+        const int B = 256, nblk = (nc + B - 1) / B;
+        std::vector<int> face_off(1, 0), faces, halo_off(1, 0), halo_cell, hinc_off(1, 0);
+        const size_t nadj = faces_host.size();
+        std::vector<unsigned short> adj_slot(nadj, 0), hinc_slot;
+        std::vector<short> adj_halo(nadj, -1);
+        std::vector<int> slot_of(nf, -1), halo_of(nc, -1), stamp(nf, -1), hstamp(nc, -1);
+        std::vector<int> fl, hl;
+        auto other = [&](int c, int k) {
+            const int f = faces_host[k];
+            return signs_host[k] > 0 ? neighbour_host[f] : owner_host[f];
+        };
+        for (int b = 0; b < nblk; ++b) {
+            const int c0 = b * B, c1 = std::min(nc, c0 + B);
+            fl.clear(); hl.clear();
+            for (int c = c0; c < c1; ++c)
+                for (int k = offset_host[c]; k < offset_host[c + 1]; ++k) {
+                    const int o = other(c, k);
+                    if (o >= 0 && (o < c0 || o >= c1) && hstamp[o] != b) { hstamp[o] = b; hl.push_back(o); }
+                }
+            if ((int)hl.size() > FUSED_MAXH) return false;
+            auto add_faces = [&](int c) {
+                for (int k = offset_host[c]; k < offset_host[c + 1]; ++k) {
+                    const int f = faces_host[k];
+                    if (stamp[f] != b) { stamp[f] = b; fl.push_back(f); }
+                }
+            };
+            for (int c = c0; c < c1; ++c) add_faces(c);
+            for (int h : hl) add_faces(h);
+            if ((int)fl.size() > FUSED_MAXF) return false;
+            std::sort(fl.begin(), fl.end());
+            for (int s = 0; s < (int)fl.size(); ++s) slot_of[fl[s]] = s;
+            for (int e = 0; e < (int)hl.size(); ++e) halo_of[hl[e]] = e;
+            for (int c = c0; c < c1; ++c)
+                for (int k = offset_host[c]; k < offset_host[c + 1]; ++k) {
+                    adj_slot[k] = (unsigned short)slot_of[faces_host[k]];
+                    const int o = other(c, k);
+                    adj_halo[k] = (short)((o >= 0 && (o < c0 || o >= c1)) ? halo_of[o] : -1);
+                }
+            for (int h : hl) {
+                for (int k = offset_host[h]; k < offset_host[h + 1]; ++k)
+                    hinc_slot.push_back((unsigned short)slot_of[faces_host[k]]);
+                hinc_off.push_back((int)hinc_slot.size());
+                halo_cell.push_back(h);
+            }
+            faces.insert(faces.end(), fl.begin(), fl.end());
+            face_off.push_back((int)faces.size());
+            halo_off.push_back((int)halo_cell.size());
+        }
+        FusedPlan P{};
+        P.nblk = nblk;
+        const int *p_face_off = nullptr, *p_faces = nullptr, *p_halo_off = nullptr, *p_halo_cell = nullptr,
+                  *p_hinc_off = nullptr;
+        const unsigned short *p_adj_slot = nullptr, *p_hinc_slot = nullptr;
+        const short *p_adj_halo = nullptr;
+        if (upload(p_face_off, face_off) || upload(p_faces, faces) || upload(p_halo_off, halo_off) ||
+            upload(p_halo_cell, halo_cell.empty() ? std::vector<int>(1, 0) : halo_cell) ||
+            upload(p_hinc_off, hinc_off) || upload(p_adj_slot, adj_slot) ||
+            upload(p_hinc_slot, hinc_slot.empty() ? std::vector<unsigned short>(1, 0) : hinc_slot) ||
+            upload(p_adj_halo, adj_halo))
+            return false;
+        P.face_off = p_face_off; P.faces = p_faces; P.halo_off = p_halo_off; P.halo_cell = p_halo_cell;
+        P.hinc_off = p_hinc_off; P.adj_slot = p_adj_slot; P.hinc_slot = p_hinc_slot; P.adj_halo = p_adj_halo;
+        fplan = P;
+        return true;
+    }
+    int river_fused(int enable) override
+    {
+        if (!enable) { fused_on = false; return BRAIDS_OK; }
+        if (!std::is_same<R, float>::value) { set_error("the fused river launch is the FP32 form's"); return BRAIDS_EINVAL; }
+        if (!fused_built) {
+            if (!fused_build()) { set_error("a workgroup of the mesh exceeds the fused launch's face tables"); return BRAIDS_EINVAL; }
+            fused_built = true;
+        }
+        fused_on = true;
+        return BRAIDS_OK;
+    }
     int drag_configure(int drag_mode, const double *drag_values)
     {
         int rc = 0;
@@ -555,11 +646,13 @@ struct Solver : SolverBase {
         if (!rb.theta && (rc = river_buffers())) return rc;
         Work<R> wr = w;
         wr.vel = vel_buf;       // the river stages read their input velocities from the producing launch
-        run_river_substep<R>(m, st[0], st[1], st[2], st[3], z, wr, ctl, rb, nrot > 0 ? &tv : nullptr, dt, t, cfg, stream,
+        LaunchCfg rc_cfg = cfg;
+        rc_cfg.fused = fused_on ? &fplan : nullptr;
+        run_river_substep<R>(m, st[0], st[1], st[2], st[3], z, wr, ctl, rb, nrot > 0 ? &tv : nullptr, dt, t, rc_cfg, stream,
                              (unsigned int *)ctl, (int)(block_bytes / 4), mirror());
         HIP_TRY(hipGetLastError());
         HIP_TRY(hipStreamSynchronize(stream));
-        if (profiling) collect(2, 1);
+        if (profiling) collect(2, fused_on ? 3 : 1);
         const Control *hc = (const Control *)host_block;
         const double *hl = (const double *)((const char *)host_block + sizeof(Control));
         res->accepted = 0; res->reason = 0; res->failed_stage = -1; res->fail_cell = -1; res->frequency_peak = 0;
@@ -1195,6 +1288,13 @@ extern "C" int river_get_face_speed(braids_solver *s, double *speed)
 }
 
 // -- river SSPRK(2,2) and turbines ---------------------------------------------
+extern "C" int braids_river_fused(braids_solver *s, int enable)
+{
+    SETUP_GUARD;
+    if (!s) { set_error("null solver"); return BRAIDS_EINVAL; }
+    return s->impl->river_fused(enable);
+}
+
 extern "C" int braids_river_configure(braids_solver *s, double cfl, int drag_mode, const double *drag_values)
 {
     SETUP_GUARD;

@@ -339,21 +339,30 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(sizeof
     face_launch_block<R, SCHEME, ROTORS>(m, q, z, w, ctl, T, stage, t, nfb, lead, blockIdx.x, remap);
 }
 
+// A face's record as one cell sees it: mass flux, this side's momentum flux,
+// speed.
+template <class R> struct SideRecord { R F1, G2, G3, S; };
+
 // STAGE 0: out = in + dt r (in is the dragged base);  STAGE 1: out = in0/2 +
 // (in + dt r)/2 with in0 the dragged base, then the closing drag half-step.
 // First the cell's own inventory factor theta (kept in LDS for the
 // workgroup's 256 consecutive cells) and its face-sum frequency
 // (explicit_frequency, river_explicit.h:13-25, no exterior owner),
 // max-reduced into the stage's CFL slot; a NaN frequency rejects the trial.
-template <class R, int STAGE>
-__device__ __forceinline__ void k_river_update_body(MeshView<R> m, StateView<R> in, StateView<R> in0,
-                                                      StateView<R> out, Work<R> w, TurbineView T, double *wlim,
-                                                      DragView d, Control *ctl, int slot, double dt, double drag_dt,
-                                                      int lb)
+//
+// The update of the workgroup's cells [base, base + blockDim.x).  rec(k, f,
+// own) gives the record of face f at incidence k; far_theta(k, donor) the
+// inventory factor of a donor outside the workgroup; before_sync() runs after
+// the workgroup's factors are in th_s and before the barrier that publishes
+// them.
+template <class R, int STAGE, class Rec, class FarTheta, class BeforeSync>
+__device__ __forceinline__ void river_update_cells(const MeshView<R> &m, StateView<R> in, StateView<R> in0,
+                                                   StateView<R> out, const Work<R> &w, const TurbineView &T,
+                                                   double *wlim, const DragView &d, Control *ctl, int slot, double dt,
+                                                   double drag_dt, int base, double *th_s, Rec rec,
+                                                   FarTheta far_theta, BeforeSync before_sync)
 {
-    __shared__ double th_s[256];
     const int nc = m.nc, nf = m.nf;
-    const int base = lb * blockDim.x;
     const int c = base + threadIdx.x;
     double thc = 1., freq = 0.;
     // The cell's state is read ahead of the barrier (loads do not cross it).
@@ -371,7 +380,10 @@ __device__ __forceinline__ void k_river_update_body(MeshView<R> m, StateView<R> 
 #pragma unroll
             for (int i = 0; i < ADJ_CHUNK; ++i) adj_entry(m, kb + i < k1 ? kb + i : k1 - 1, f[i], fl[i]);
 #pragma unroll
-            for (int i = 0; i < ADJ_CHUNK; ++i) { F1[i] = w.F[f[i]]; L[i] = m.length[f[i]]; S[i] = w.F[5 * nf + f[i]]; }
+            for (int i = 0; i < ADJ_CHUNK; ++i) {
+                const SideRecord<R> r = rec(kb + i < k1 ? kb + i : k1 - 1, f[i], (fl[i] & ADJ_OWNER) != 0);
+                F1[i] = r.F1; L[i] = m.length[f[i]]; S[i] = r.S;
+            }
 #pragma unroll
             for (int i = 0; i < ADJ_CHUNK; ++i) {
                 if (kb + i >= k1) break;
@@ -387,6 +399,7 @@ __device__ __forceinline__ void k_river_update_body(MeshView<R> m, StateView<R> 
         if (!(freq == freq)) freq = 1e300;
     }
     th_s[threadIdx.x] = thc;
+    before_sync();
     block_max_to(freq, w.rpart + rpart_index(RQ_PEAK, slot));
     __syncthreads();
     int fail = 0;
@@ -398,11 +411,12 @@ __device__ __forceinline__ void k_river_update_body(MeshView<R> m, StateView<R> 
         for (int kb = k0; kb < k1; kb += ADJ_CHUNK) {
             // owner / neighbour of each face from the incidence (the cell and
             // the cell across); an open face is one the cell owns with no cell across
-            int fa[ADJ_CHUNK], fl[ADJ_CHUNK], oa[ADJ_CHUNK], ja[ADJ_CHUNK];
+            int fa[ADJ_CHUNK], fl[ADJ_CHUNK], oa[ADJ_CHUNK], ja[ADJ_CHUNK], ka[ADJ_CHUNK];
             R La[ADJ_CHUNK], F1a[ADJ_CHUNK], G2a[ADJ_CHUNK], G3a[ADJ_CHUNK];
 #pragma unroll
             for (int i = 0; i < ADJ_CHUNK; ++i) {
                 const int k = kb + i < k1 ? kb + i : k1 - 1;
+                ka[i] = k;
                 adj_entry(m, k, fa[i], fl[i]);
                 const int other = m.adj_other[k];
                 const bool own = (fl[i] & ADJ_OWNER) != 0;
@@ -411,9 +425,8 @@ __device__ __forceinline__ void k_river_update_body(MeshView<R> m, StateView<R> 
 #pragma unroll
             for (int i = 0; i < ADJ_CHUNK; ++i) {
                 const int f = fa[i];
-                const bool own = (fl[i] & ADJ_OWNER) != 0;
-                La[i] = m.length[f]; F1a[i] = w.F[f];
-                G2a[i] = w.F[(own ? nf : 3 * nf) + f]; G3a[i] = w.F[(own ? 2 * nf : 4 * nf) + f];
+                const SideRecord<R> r = rec(ka[i], f, (fl[i] & ADJ_OWNER) != 0);
+                La[i] = m.length[f]; F1a[i] = r.F1; G2a[i] = r.G2; G3a[i] = r.G3;
             }
 #pragma unroll
             for (int i = 0; i < ADJ_CHUNK; ++i) {
@@ -443,7 +456,7 @@ __device__ __forceinline__ void k_river_update_body(MeshView<R> m, StateView<R> 
                 const int donor = wl < 0. ? j : o;
                 const int r = donor - base;
                 const double th = donor < 0 ? 1. : (r >= 0 && r < (int)blockDim.x ? th_s[r]
-                                                                               : river_theta<R>(m, w, in, donor, dt));
+                                                                               : far_theta(ka[i], donor));
                 if (donor >= 0 && th != 1.) {
                     if (fl[i] & ADJ_OPEN) fail = RFAIL_BOUNDARY;
                     else {
@@ -486,12 +499,89 @@ __device__ __forceinline__ void k_river_update_body(MeshView<R> m, StateView<R> 
 }
 
 template <class R, int STAGE>
+__device__ __forceinline__ void k_river_update_body(MeshView<R> m, StateView<R> in, StateView<R> in0,
+                                                      StateView<R> out, Work<R> w, TurbineView T, double *wlim,
+                                                      DragView d, Control *ctl, int slot, double dt, double drag_dt,
+                                                      int lb)
+{
+    __shared__ double th_s[256];
+    const int nf = m.nf;
+    const R *F = w.F;
+    river_update_cells<R, STAGE>(
+        m, in, in0, out, w, T, wlim, d, ctl, slot, dt, drag_dt, lb * (int)blockDim.x, th_s,
+        [&](int, int f, bool own) {
+            SideRecord<R> r;
+            r.F1 = F[f]; r.G2 = F[(own ? nf : 3 * nf) + f]; r.G3 = F[(own ? 2 * nf : 4 * nf) + f]; r.S = F[5 * nf + f];
+            return r;
+        },
+        [&](int, int donor) { return river_theta<R>(m, w, in, donor, dt); },
+        [] {});
+}
+
+template <class R, int STAGE>
 __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(sizeof(R) == 4 ? 6 : 1))) k_river_update(MeshView<R> m, StateView<R> in, StateView<R> in0,
                                                       StateView<R> out, Work<R> w, TurbineView T, double *wlim,
                                                       DragView d, Control *ctl, int slot, double dt, double drag_dt,
                                                       int remap)
 {
     k_river_update_body<R, STAGE>(m, in, in0, out, w, T, wlim, d, ctl, slot, dt, drag_dt, logical_block(remap));
+}
+
+// Fused face-and-update launch of a river stage (FP32 form).  Workgroup b
+// computes the records of its faces (FusedPlan: its cells' and ring-1 cells'
+// faces) into LDS - open faces are copied from the boundary launch that ran
+// before - then the inventory factors of its ring-1 cells from those records
+// (river_theta's expression), then river_update_cells with the records and
+// factors from LDS.  No face record goes through HBM.
+template <int SCHEME, int STAGE>
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4)))
+k_river_fused(MeshView<float> m, StateView<float> in, StateView<float> in0, StateView<float> out,
+              const double *__restrict__ z, Work<float> w, TurbineView T, double *wlim, DragView d, Control *ctl,
+              int slot, double dt, double drag_dt, FusedPlan P, int remap)
+{
+    __shared__ float Fs[6][FUSED_MAXF];
+    __shared__ double th_s[256];
+    __shared__ double th_h[FUSED_MAXH];
+    const int b = logical_block(remap);
+    const int nf = m.nf;
+    const int f0 = P.face_off[b], nfl = P.face_off[b + 1] - f0;
+    for (int s = threadIdx.x; s < nfl; s += blockDim.x) {
+        const int f = P.faces[f0 + s];
+        if (m.bid[f] >= 0) {
+            for (int j = 0; j < 6; ++j) Fs[j][s] = w.F[j * nf + f];
+        } else {
+            face_one_f32<SCHEME>(m, in, z, w, f, FaceRecordShared<FUSED_MAXF>{Fs, s});
+        }
+    }
+    __syncthreads();
+    const int h0 = P.halo_off[b], nh = P.halo_off[b + 1] - h0;
+    river_update_cells<float, STAGE>(
+        m, in, in0, out, w, T, wlim, d, ctl, slot, dt, drag_dt, b * (int)blockDim.x, th_s,
+        [&](int k, int, bool own) {
+            const int s = P.adj_slot[k];
+            SideRecord<float> r;
+            r.F1 = Fs[0][s]; r.G2 = Fs[own ? 1 : 3][s]; r.G3 = Fs[own ? 2 : 4][s]; r.S = Fs[5][s];
+            return r;
+        },
+        [&](int k, int) { return th_h[P.adj_halo[k]]; },
+        [&] {
+            // river_theta of the ring-1 cells, in their CSR order
+            for (int e = threadIdx.x; e < nh; e += blockDim.x) {
+                const int hc = P.halo_cell[h0 + e];
+                const int k0 = m.offset[hc], e0 = P.hinc_off[h0 + e];
+                const int deg = P.hinc_off[h0 + e + 1] - e0;
+                double outv = 0.;
+                for (int t = 0; t < deg; ++t) {
+                    const int k = k0 + t;
+                    const double sg = (m.adj_flags[k] & ADJ_OWNER) ? 1. : -1.;
+                    outv += fmax(sg * (double)Fs[0][P.hinc_slot[e0 + t]], 0.) * (double)m.length[m.adj_face[k]];
+                }
+                const double hA = in.h[hc] * (double)m.area[hc];
+                double th = 1.;
+                if (outv > 0. && dt * outv >= hA) th = fmin(1., hA / (dt * outv)) * (1. - 32. * 2.2204460492503131e-16);
+                th_h[e] = th;
+            }
+        });
 }
 
 // Boundary ledger of one substep: ledger[b] = -dt sum_f L (w1 + w2)/2 over

@@ -779,9 +779,21 @@ __global__ void __launch_bounds__(256) k_face(MeshView<R> m, StateView<R> q, con
 //   G = F_dyn + [1/2 g (h_f^2 - h_c^2) + well-balanced correction] n,
 // which for the wet-column correction equals 1/2 g (h_c + h_f)(eta_f - eta_c),
 // a free-surface difference.  A lake at rest therefore stays at rest in FP32.
-template <int SCHEME>
+// Destinations of a face's record (F1, FL2, FL3, FR2, FR3, speed): the
+// face-indexed arrays of Work, or one slot of a workgroup's LDS table.
+struct FaceRecordGlobal {
+    float *F; int nf, f;
+    __device__ void operator()(int j, float v) const { F[j * nf + f] = v; }
+};
+template <int CAP>
+struct FaceRecordShared {
+    float (*F)[CAP]; int s;
+    __device__ void operator()(int j, float v) const { F[j][s] = v; }
+};
+
+template <int SCHEME, class Out>
 __device__ __forceinline__ void face_one_f32(const MeshView<float> &m, const StateView<float> &q,
-                                             const double *__restrict__ z, const Work<float> &w, int f)
+                                             const double *__restrict__ z, const Work<float> &w, int f, Out out)
 {
     typedef float R;
     const int nf = m.nf;
@@ -793,7 +805,7 @@ __device__ __forceinline__ void face_one_f32(const MeshView<float> &m, const Sta
     const double hc = q.h[c];
     if (b < 0 && hc <= dd) {
         if (n < 0 || q.h[n] <= dd) {
-            for (int j = 0; j < 6; ++j) w.F[j * nf + f] = 0.f;
+            for (int j = 0; j < 6; ++j) out(j, 0.f);
             return;
         }
     }
@@ -873,12 +885,19 @@ __device__ __forceinline__ void face_one_f32(const MeshView<float> &m, const Sta
         }
         PR = PL;
     }
-    w.F[f] = fl[0];
-    w.F[nf + f] = (R)((double)fl[1] + PL * nxd);
-    w.F[2 * nf + f] = (R)((double)fl[2] + PL * nyd);
-    w.F[3 * nf + f] = (R)((double)fr[1] + PR * nxd);
-    w.F[4 * nf + f] = (R)((double)fr[2] + PR * nyd);
-    w.F[5 * nf + f] = speed;
+    out(0, fl[0]);
+    out(1, (R)((double)fl[1] + PL * nxd));
+    out(2, (R)((double)fl[2] + PL * nyd));
+    out(3, (R)((double)fr[1] + PR * nxd));
+    out(4, (R)((double)fr[2] + PR * nyd));
+    out(5, speed);
+}
+
+template <int SCHEME>
+__device__ __forceinline__ void face_one_f32(const MeshView<float> &m, const StateView<float> &q,
+                                             const double *__restrict__ z, const Work<float> &w, int f)
+{
+    face_one_f32<SCHEME>(m, q, z, w, f, FaceRecordGlobal{w.F, m.nf, f});
 }
 
 template <int SCHEME>

@@ -9,7 +9,7 @@ namespace braids {
 
 // Kernel kinds timed by the optional event profiler.
 enum : int { KK_PREP = 0, KK_TRACES, KK_FACE, KK_DRAIN, KK_GATHER, KK_LEDGER,
-             KK_RDRAG, KK_RTHETA, KK_TURBINE, KK_RUPDATE,
+             KK_RDRAG, KK_RFUSED, KK_TURBINE, KK_RUPDATE,
              KK_HRECON, KK_HFACE, KK_HBUDGET, KK_HUPDATE, KK_COUNT };
 
 // The first timed launch that left an error (kind + 1, 0: none), for the
@@ -24,6 +24,9 @@ struct LaunchCfg {
     // whose kind bit is set in ev_mask.
     hipEvent_t *ev = nullptr;
     int ev_mask = 0;
+    // FP32 river stages: the fused face-and-update launch with these tables
+    // (null: separate face and update launches)
+    const FusedPlan *fused = nullptr;
 };
 
 // mode: 0,1,2 = SSPRK(3,3) stage index (combination + prepare [+ Manning]);

@@ -141,6 +141,29 @@ static void river_stage(const MeshView<R> &m, StateView<R> in, StateView<R> in0,
                         double dt, double t, const LaunchCfg &cfg, hipStream_t s)
 {
     launch_prep<R, SCHEME>(m, in, z, w, ctl, slot, cfg, s, 0);     // the river path uses no max depth
+    if constexpr (std::is_same<R, float>::value) {
+        if (cfg.fused) {
+            // boundary traces with their open faces and the rotor sampling (no face
+            // workgroups), then the fused face-and-update launch
+            const int lead = face_lead(m.nb, T.nrot), B = 256;
+            if (lead > 0) {
+                if (T.nrot > 0)
+                    timed_launch(cfg, slot, KK_TRACES, k_face_traces<R, SCHEME, true>, dim3(lead), dim3(B), s, m, in,
+                                 z, w, ctl, T, slot, t, 0, lead, cfg.remap);
+                else
+                    timed_launch(cfg, slot, KK_TRACES, k_face_traces<R, SCHEME, false>, dim3(lead), dim3(B), s, m, in,
+                                 z, w, ctl, T, slot, t, 0, lead, cfg.remap);
+            }
+            const FusedPlan P = *cfg.fused;
+            if (slot == 0)
+                timed_launch(cfg, slot, KK_RFUSED, k_river_fused<SCHEME, 0>, dim3(P.nblk), dim3(B), s, m, in, in0, out,
+                             z, w, T, rb.wlim[0], rb.drag, ctl, slot, dt, 0.0, P, cfg.remap);
+            else
+                timed_launch(cfg, slot, KK_RFUSED, k_river_fused<SCHEME, 1>, dim3(P.nblk), dim3(B), s, m, in, in0, out,
+                             z, w, T, rb.wlim[1], rb.drag, ctl, slot, dt, .5 * dt, P, cfg.remap);
+            return;
+        }
+    }
     launch_face_traces<R, SCHEME>(m, in, z, w, ctl, slot, t, cfg, s, T);
     const dim3 gu(grid_for(m.nc, 256)), bu(256);      // k_river_update keeps 256 thetas in LDS
     if (slot == 0)

@@ -53,6 +53,25 @@ constexpr int RPART = RSLOTS * RSTRIDE;      // u64 per (quantity, stage)
 enum : int { RQ_HMAX = 0, RQ_PEAK = 1 };
 __host__ __device__ inline int rpart_index(int quantity, int stage) { return (quantity * 4 + stage) * RPART; }
 
+// Face tables of the fused face-and-update launch (FP32 river path).
+// Workgroup b updates cells [256 b, 256 b + 256).  Its faces - those of its
+// cells and of the cells around them (ring 1), each once - are
+// faces[face_off[b] .. face_off[b+1]), in ascending order.  For an incidence
+// k of one of its cells, adj_slot[k] is the face's slot in that list and
+// adj_halo[k] the ring-1 index of the cell across (-1 when that cell is in
+// the workgroup or there is none).  The ring-1 cells are
+// halo_cell[halo_off[b] ..), and the slots of the faces of ring-1 cell e, in
+// its CSR order, are hinc_slot[hinc_off[e] .. hinc_off[e+1]).
+constexpr int FUSED_MAXF = 1024, FUSED_MAXH = 256;
+struct FusedPlan {
+    int nblk;
+    const int *face_off, *faces;
+    const unsigned short *adj_slot;
+    const short *adj_halo;
+    const int *halo_off, *halo_cell, *hinc_off;
+    const unsigned short *hinc_slot;
+};
+
 // Workgroups ahead of the faces in a face launch: one per open boundary and
 // one per rotor, padded to a multiple of 8.
 __host__ __device__ inline int face_lead(int nb, int nrot) { return (nb + nrot + 7) & ~7; }

@@ -132,6 +132,15 @@ typedef struct braids_river_result {
  * (1 Chezy, 2 Manning, 3/4 White-Colebrook; 0 none), drag_values [nc] host
  * roughness per cell (NULL: the mesh Manning n). */
 int braids_river_configure(braids_solver *s, double cfl, int drag_mode, const double *drag_values);
+/* FP32 solvers: 1 runs each stage as a boundary launch (traces, open faces,
+ * rotors) and one fused face-and-update launch whose face records stay in
+ * LDS (every workgroup of 256 cells, with the cells around it, must have at
+ * most 1024 faces and 256 neighbours); 0 (the default) runs separate face
+ * and update launches, which are faster on MI355X (the fused launch computes
+ * the faces of the ring around each workgroup again).  No reference
+ * counterpart; the results are identical.  BRAIDS_EINVAL on FP64 solvers or
+ * when the mesh exceeds the tables. */
+int braids_river_fused(braids_solver *s, int enable);
 /* One substep of size dt at time t (RiverExplicit._single, river_explicit.py:94-119);
  * commits on success, BRAIDS_REJECTED leaves the state unchanged. */
 int braids_river_step(braids_solver *s, double dt, double t, braids_river_result *res);

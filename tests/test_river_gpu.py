@@ -283,3 +283,37 @@ def test_engine_operators_match_oracle():
         got = dict(frequency=freq, theta=theta, water_out=water, correction=corr, status=status, rate=rate)
         for k, v in got.items():
             assert np.array_equal(v.cpu().numpy(), want[k]), (seed, k)
+
+
+@pytest.mark.parametrize('case', ['channel_turbines', 'braided_s5', 'refined_turbines'])
+def test_fp32_fused_launch_matches_separate_launches(case):
+    """The FP32 form's fused face-and-update launch (face records in LDS)
+    against separate face and update launches, 100 substeps from the same
+    state: the same arithmetic, so the states agree to FP32 rounding (they
+    are bit-identical on MI355X)."""
+    make, scheme, _ = CASES[case]
+    from braids_hip.solver import HydraulicSolver
+    from braids_hip.river import RiverExplicit
+    from braids_hip.turbines import ThetisArray
+    m, rows = make()
+    m = m.renumbered('hilbert')          # the fused tables need a local cell order
+    out = []
+    for fused in (True, False):
+        s = HydraulicSolver(m, precision=32, scheme=scheme)
+        s.set_state(m.q0, prepare=False)
+        RiverExplicit(s, cfl=.5)
+        if rows:
+            ThetisArray(m, rows).install(s)
+        s.river_fused(fused)
+        t = 0.0
+        dt = 0.005 if case == 'refined_turbines' else 0.01     # 1 m cells around the rotors
+        for k in range(100):
+            s.river_step(dt, t)
+            t += dt
+        out.append((s.state(), s.ledger()))
+        s.close()
+    (qa, la), (qb, lb) = out
+    diff = max(np.abs(qa[j] - qb[j]).max() / max(np.abs(qb[j]).max(), 1e-300) for j in range(3))
+    print(f'[river] {case} fused vs separate launches: max rel diff {diff:.3e}', flush=True)
+    assert diff <= 1e-6
+    assert np.allclose(la, lb, rtol=1e-6, atol=1e-9)
