@@ -222,6 +222,8 @@ def main():
     ap.add_argument('--workload', default=None)
     ap.add_argument('--block', type=int, default=256)
     ap.add_argument('--remap', type=int, default=1)
+    ap.add_argument('--autotune', action='store_true',
+                    help='river integrator: time every launch shape on this mesh first and keep the fastest')
     ap.add_argument('--no-cpu-baseline', action='store_true')
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get('WORLD_SIZE', '1')) != 1:
@@ -246,6 +248,7 @@ def main():
     s = HydraulicSolver(m, precision=args.precision, scheme=args.scheme, cfl_target=0.9, dtmax=1.0,
                         block=args.block, remap=args.remap, stream=stream)
     nslot = 0
+    tuned = None
     if args.integrator == 'river':
         from braids_hip.river import RiverExplicit
         from braids_hip.turbines import ThetisArray
@@ -257,6 +260,10 @@ def main():
 
         def run(n):
             return s.river_run(n)
+        if args.autotune:
+            from braids_hip.autotune import tune_river
+            tuned = tune_river(s, substeps=20, repeats=2)
+            print(f'[bench] autotune: {tuned}', file=sys.stderr, flush=True)
     elif args.integrator == 'morph':
         # one accepted hydraulic substep, then the Exner bed update over it
         # (MPM, d50 20 mm, porosity 0.4; the non-erodible layer 2 m below the
@@ -372,7 +379,8 @@ def main():
                                  'differencing and ledgers') if args.precision == 32 else 'fp64',
                    'turbines': len(rows), 'turbines_active': active, 'turbines_inactive': inactive,
                    'simulated_s': simulated, 'renumbering': 'hilbert',
-                   'block': args.block, 'xcd_remap': bool(args.remap),
+                   'block': args.block, 'xcd_remap': bool(s.launch['remap']),
+                   'fused_river_launch': bool(s.launch['river_fused']), 'autotune': tuned,
                    ('retries' if args.integrator == 'ssp33' else 'rejections'): retries,
                    'state_finite_nonnegative': finite},
         'roofline': {'bound': 'hbm', 'kernel': dom, 'achieved': achieved, 'peak': HBM_PEAK_GBS, 'unit': 'GB/s',

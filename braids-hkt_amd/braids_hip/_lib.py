@@ -70,7 +70,7 @@ SIGNATURES = {
     'braids_create': (_I, [ctypes.POINTER(BraidsMesh), ctypes.POINTER(BraidsOptions), _P, ctypes.POINTER(_P)]),
     'braids_destroy': (_I, [_P]),
     'braids_set_stream': (_I, [_P, _P]),
-    'braids_river_fused': (_I, [_P, _I]),
+    'braids_tune': (_I, [_P, _I, _I]),
     'braids_set_state': (_I, [_P, _P, _I]),
     'braids_get_state': (_I, [_P, _P]),
     'braids_set_bed': (_I, [_P, _P]),

@@ -75,6 +75,7 @@ class HydraulicSolver:
                    'braids_create')
         self.h = handle
         self.precision, self.scheme, self.dtmax = int(precision), int(scheme), float(dtmax)
+        self.launch = {'remap': int(remap), 'river_fused': 0}
         self.time = 0.0
         self.step_hint = float(dtmax)
         if stream is not None:
@@ -96,11 +97,20 @@ class HydraulicSolver:
         raw = getattr(stream, 'cuda_stream', stream)
         _lib.check(self.L.braids_set_stream(self.h, ctypes.c_void_p(int(raw) if raw else 0)), 'braids_set_stream')
 
+    TUNE_KNOBS = {'remap': 1, 'river_fused': 2}
+
+    def tune(self, knob, value):
+        """Launch-shape knob of later steps (braids_tune): 'remap' (XCD-aware
+        workgroup order, default 1) or 'river_fused' (FP32: fused
+        face-and-update launch, default 0).  Results do not depend on them."""
+        _lib.check(self.L.braids_tune(self.h, self.TUNE_KNOBS[knob], int(value)), 'braids_tune')
+        self.launch[knob] = int(value)
+
     def river_fused(self, enable=True):
         """FP32 form: run each river stage as a fused face-and-update launch
         (face records in LDS) or as separate face and update launches (the
-        default, faster on MI355X); the results are identical."""
-        _lib.check(self.L.braids_river_fused(self.h, int(bool(enable))), 'braids_river_fused')
+        default); the results are identical."""
+        self.tune('river_fused', int(bool(enable)))
 
     # -- state -------------------------------------------------------------
     def set_state(self, q, prepare=True):

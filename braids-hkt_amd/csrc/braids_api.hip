@@ -113,6 +113,7 @@ struct SolverBase {
     virtual int face_speed(double *speed) = 0;
     virtual int river_configure(double cfl, int drag_mode, const double *drag_values) = 0;
     virtual int river_fused(int enable) = 0;
+    virtual int set_remap(int on) = 0;
     virtual int river_step(double dt, double t, braids_river_result *res) = 0;
     virtual int turbines_set(const braids_turbines *t) = 0;
     virtual int hllc_configure(double cfl, int drag_mode, const double *drag_values, const double *inverse,
@@ -536,6 +537,7 @@ struct Solver : SolverBase {
         fplan = P;
         return true;
     }
+    int set_remap(int on) override { cfg.remap = on; return BRAIDS_OK; }
     int river_fused(int enable) override
     {
         if (!enable) { fused_on = false; return BRAIDS_OK; }
@@ -1288,11 +1290,14 @@ extern "C" int river_get_face_speed(braids_solver *s, double *speed)
 }
 
 // -- river SSPRK(2,2) and turbines ---------------------------------------------
-extern "C" int braids_river_fused(braids_solver *s, int enable)
+extern "C" int braids_tune(braids_solver *s, int knob, int value)
 {
     SETUP_GUARD;
     if (!s) { set_error("null solver"); return BRAIDS_EINVAL; }
-    return s->impl->river_fused(enable);
+    if (knob == BRAIDS_TUNE_REMAP && (value == 0 || value == 1)) return s->impl->set_remap(value);
+    if (knob == BRAIDS_TUNE_RIVER_FUSED && (value == 0 || value == 1)) return s->impl->river_fused(value);
+    set_error("unknown launch knob or value");
+    return BRAIDS_EINVAL;
 }
 
 extern "C" int braids_river_configure(braids_solver *s, double cfl, int drag_mode, const double *drag_values)

@@ -132,15 +132,20 @@ typedef struct braids_river_result {
  * (1 Chezy, 2 Manning, 3/4 White-Colebrook; 0 none), drag_values [nc] host
  * roughness per cell (NULL: the mesh Manning n). */
 int braids_river_configure(braids_solver *s, double cfl, int drag_mode, const double *drag_values);
-/* FP32 solvers: 1 runs each stage as a boundary launch (traces, open faces,
- * rotors) and one fused face-and-update launch whose face records stay in
- * LDS (every workgroup of 256 cells, with the cells around it, must have at
- * most 1024 faces and 256 neighbours); 0 (the default) runs separate face
- * and update launches, which are faster on MI355X (the fused launch computes
- * the faces of the ring around each workgroup again).  No reference
- * counterpart; the results are identical.  BRAIDS_EINVAL on FP64 solvers or
- * when the mesh exceeds the tables. */
-int braids_river_fused(braids_solver *s, int enable);
+/* Launch-shape knobs of later steps; the results do not depend on them (no
+ * reference counterpart: they are this library's launch choices, set by
+ * braids_hip.autotune).
+ * BRAIDS_TUNE_REMAP: 1 (default) XCD-aware workgroup order, 0 hardware order.
+ * BRAIDS_TUNE_RIVER_FUSED (FP32 solvers): 1 runs each river stage as a
+ *   boundary launch (traces, open faces, rotors) and one fused face-and-update
+ *   launch whose face records stay in LDS (every workgroup of 256 cells, with
+ *   the cells around it, must have at most 1024 faces and 256 neighbours);
+ *   0 (default) runs separate face and update launches.
+ * BRAIDS_EINVAL for an unknown knob or value, an FP64 solver (fused), or a
+ * mesh beyond the fused launch's tables. */
+#define BRAIDS_TUNE_REMAP 1
+#define BRAIDS_TUNE_RIVER_FUSED 2
+int braids_tune(braids_solver *s, int knob, int value);
 /* One substep of size dt at time t (RiverExplicit._single, river_explicit.py:94-119);
  * commits on success, BRAIDS_REJECTED leaves the state unchanged. */
 int braids_river_step(braids_solver *s, double dt, double t, braids_river_result *res);

@@ -317,3 +317,46 @@ def test_fp32_fused_launch_matches_separate_launches(case):
     print(f'[river] {case} fused vs separate launches: max rel diff {diff:.3e}', flush=True)
     assert diff <= 1e-6
     assert np.allclose(la, lb, rtol=1e-6, atol=1e-9)
+
+
+@pytest.mark.parametrize('precision', [64, 32])
+def test_autotune_restores_state_and_keeps_results(precision):
+    """braids_hip.autotune times every launch shape from a snapshot of the
+    resident state and restores it: afterwards the tuned solver runs the same
+    substeps to the same state as an untuned one (launch shapes change no
+    result).  Unknown knobs and values are refused."""
+    make, scheme, _ = CASES['channel_turbines']
+    from braids_hip.solver import HydraulicSolver
+    from braids_hip.river import RiverExplicit
+    from braids_hip.turbines import ThetisArray
+    from braids_hip.autotune import tune_river
+    m, rows = make()
+    m = m.renumbered('hilbert')
+    out = []
+    for tuned in (True, False):
+        s = HydraulicSolver(m, precision=precision, scheme=scheme)
+        s.set_state(m.q0, prepare=False)
+        RiverExplicit(s, cfl=.5)
+        ThetisArray(m, rows).install(s)
+        s.river_run(3)
+        before, t_before = s.state(), s.time
+        if tuned:
+            info = tune_river(s, substeps=4, repeats=1)
+            print(f'[river] autotune FP{precision}: {info}', flush=True)
+            assert len(info['ms_per_substep']) == (4 if precision == 32 else 2)
+            assert s.launch == {**{'remap': 1, 'river_fused': 0}, **info['choice']}
+            after = s.state()
+            for j in range(3):
+                assert np.array_equal(after[j], before[j])
+            assert s.time == t_before
+            with pytest.raises(RuntimeError):
+                s.tune('remap', 2)
+            assert s.L.braids_tune(s.h, 99, 0) < 0
+        s.river_run(20)
+        out.append((s.state(), s.time))
+        s.close()
+    (qa, ta), (qb, tb) = out
+    assert ta == tb
+    tol = 0.0 if precision == 64 else 1e-6
+    for j in range(3):
+        assert np.abs(qa[j] - qb[j]).max() <= tol * max(np.abs(qb[j]).max(), 1e-300), j
