@@ -32,7 +32,7 @@ static __global__ void __launch_bounds__(256) kb_zero(BatchPlan P, const int *nz
 }
 
 template <class R>
-__global__ void __launch_bounds__(256) kb_drag(BatchPlan P, const SlotView<R> *V, SlotRun *run)
+__global__ void __launch_bounds__(256) kb_drag(BatchPlan P, const SlotView<R> *__restrict__ V, SlotRun *run)
 {
     const int s = find_slot(P.cell, P.nslot, blockIdx.x);
     if (!run[s].active) return;
@@ -45,7 +45,7 @@ __global__ void __launch_bounds__(256) kb_drag(BatchPlan P, const SlotView<R> *V
 
 template <class R, int SCHEME>
 __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(sizeof(R) == 4 ? 6 : 1)))
-kb_prep(BatchPlan P, const SlotView<R> *V, const SlotRun *run, int stage)
+kb_prep(BatchPlan P, const SlotView<R> *__restrict__ V, const SlotRun *run, int stage)
 {
     const int s = find_slot(P.cell, P.nslot, blockIdx.x);
     if (!run[s].active) return;
@@ -62,7 +62,7 @@ kb_prep(BatchPlan P, const SlotView<R> *V, const SlotRun *run, int stage)
 // Per scenario: one block per open boundary (traces and their faces), padded
 // to 8, then its face blocks (face_launch_block).
 template <class R, int SCHEME>
-__global__ void __launch_bounds__(256) kb_face(BatchPlan P, const SlotView<R> *V, const SlotRun *run, int stage)
+__global__ void __launch_bounds__(256) kb_face(BatchPlan P, const SlotView<R> *__restrict__ V, const SlotRun *run, int stage)
 {
     const int s = find_slot(P.face, P.nslot, blockIdx.x);
     if (!run[s].active) return;
@@ -77,7 +77,7 @@ __global__ void __launch_bounds__(256) kb_face(BatchPlan P, const SlotView<R> *V
 // The rotors of every scenario sampling its stage input (one workgroup each;
 // a launch of its own, one per stage for the whole batch).
 template <class R>
-__global__ void __launch_bounds__(256) kb_rotors(BatchPlan P, const SlotView<R> *V, const SlotRun *run, int stage)
+__global__ void __launch_bounds__(256) kb_rotors(BatchPlan P, const SlotView<R> *__restrict__ V, const SlotRun *run, int stage)
 {
     const int s = find_slot(P.rot, P.nslot, blockIdx.x);
     if (!run[s].active) return;
@@ -88,7 +88,7 @@ __global__ void __launch_bounds__(256) kb_rotors(BatchPlan P, const SlotView<R> 
 
 template <class R, int STAGE>
 __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(sizeof(R) == 4 ? 6 : 1)))
-kb_update(BatchPlan P, const SlotView<R> *V, const SlotRun *run)
+kb_update(BatchPlan P, const SlotView<R> *__restrict__ V, const SlotRun *run)
 {
     const int s = find_slot(P.cell, P.nslot, blockIdx.x);
     if (!run[s].active) return;
@@ -105,7 +105,7 @@ kb_update(BatchPlan P, const SlotView<R> *V, const SlotRun *run)
 }
 
 template <class R>
-__global__ void __launch_bounds__(TRACE_BLOCK) kb_ledger(BatchPlan P, const SlotView<R> *V, const SlotRun *run)
+__global__ void __launch_bounds__(TRACE_BLOCK) kb_ledger(BatchPlan P, const SlotView<R> *__restrict__ V, const SlotRun *run)
 {
     const int s = find_slot(P.bnd, P.nslot, blockIdx.x);
     if (!run[s].active) return;
@@ -117,7 +117,7 @@ __global__ void __launch_bounds__(TRACE_BLOCK) kb_ledger(BatchPlan P, const Slot
 // accept/reject decision in the order of RiverExplicit.rate (spatial, boundary
 // status, CFL, non-finite/negative) and advance the slot's controller.
 template <class R>
-__global__ void __launch_bounds__(64) kb_finalize(const SlotView<R> *V, SlotRun *run, int nslot)
+__global__ void __launch_bounds__(64) kb_finalize(const SlotView<R> *__restrict__ V, SlotRun *run, int nslot)
 {
     const int s = blockIdx.x;
     if (s >= nslot || !run[s].active) return;

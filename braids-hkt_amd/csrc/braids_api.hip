@@ -206,6 +206,8 @@ struct Solver : SolverBase {
     double *qscratch = nullptr;        // [3*nc] double for conversions
     RiverBuffers rb;
     R *vel_buf = nullptr;              // [2*nc] river stage-input velocities (Work::vel)
+    const int *ring_off = nullptr, *ring_cell = nullptr;   // Work::halo_off / halo_cell (river update)
+    const short *ring_adj = nullptr;                        // Work::adj_halo
     TurbineView tv{};
     ExnerView ex{};
     double *zsnap = nullptr;             // bed snapshot (paired with st[4])
@@ -455,6 +457,7 @@ struct Solver : SolverBase {
         if ((rc = alloc(rb.wlim[0], nf))) return rc;
         if ((rc = alloc(rb.wlim[1], nf))) return rc;
         if ((rc = alloc(vel_buf, 2 * (size_t)nc))) return rc;
+        if ((rc = ring_build())) return rc;
         rb.ledger = ledger_dev;
         HIP_TRY(hipMemset(rb.wlim[0], 0, nf * sizeof(double)));
         HIP_TRY(hipMemset(rb.wlim[1], 0, nf * sizeof(double)));
@@ -465,6 +468,36 @@ struct Solver : SolverBase {
     {
         river_cfl = cfl;
         return drag_configure(drag_mode, drag_values);
+    }
+    // The ring-1 cells of every river-update workgroup of 256 cells (Work::halo_*).
+    // Left unset (the update then recomputes outside donors' factors where it
+    // meets them) when a workgroup has more than 256 of them.
+    int ring_build()
+    {
+        const int B = 256, nblk = (nc + B - 1) / B;
+        std::vector<int> off(1, 0), cells, stamp(nc, -1), pos(nc, -1);
+        std::vector<short> adj(faces_host.size(), -1);
+        for (int b = 0; b < nblk; ++b) {
+            const int c0 = b * B, c1 = std::min(nc, c0 + B);
+            int count = 0;
+            for (int c = c0; c < c1; ++c)
+                for (int k = offset_host[c]; k < offset_host[c + 1]; ++k) {
+                    const int f = faces_host[k];
+                    const int o = signs_host[k] > 0 ? neighbour_host[f] : owner_host[f];
+                    if (o < 0 || (o >= c0 && o < c1)) continue;
+                    if (stamp[o] != b) {
+                        stamp[o] = b; pos[o] = count++;
+                        if (count > B) return BRAIDS_OK;
+                        cells.push_back(o);
+                    }
+                    adj[k] = (short)pos[o];
+                }
+            off.push_back((int)cells.size());
+        }
+        if (cells.empty()) cells.push_back(0);
+        int rc = 0;
+        if ((rc = upload(ring_off, off)) || (rc = upload(ring_cell, cells)) || (rc = upload(ring_adj, adj))) return rc;
+        return BRAIDS_OK;
     }
     // The tables of the fused face-and-update launch (FP32 form): per workgroup
     // of 256 cells, its ring-1 cells and the faces of both, within FUSED_MAXF
@@ -648,6 +681,7 @@ struct Solver : SolverBase {
         if (!rb.theta && (rc = river_buffers())) return rc;
         Work<R> wr = w;
         wr.vel = vel_buf;       // the river stages read their input velocities from the producing launch
+        wr.halo_off = ring_off; wr.halo_cell = ring_cell; wr.adj_halo = ring_adj;
         LaunchCfg rc_cfg = cfg;
         rc_cfg.fused = fused_on ? &fplan : nullptr;
         run_river_substep<R>(m, st[0], st[1], st[2], st[3], z, wr, ctl, rb, nrot > 0 ? &tv : nullptr, dt, t, rc_cfg, stream,
@@ -980,6 +1014,7 @@ struct Batch : BatchBase {
             Solver<R> *x = slots[i].s;
             SlotView<R> v{};
             v.m = x->m; v.w = x->w; v.w.vel = x->vel_buf;
+            v.w.halo_off = x->ring_off; v.w.halo_cell = x->ring_cell; v.w.adj_halo = x->ring_adj;
             v.q[0] = x->st[0]; v.q[1] = x->st[3]; v.work = x->st[1]; v.q1 = x->st[2];
             v.z = x->z; v.ctl = x->ctl; v.rb = x->rb;
             v.T = x->tv;

@@ -355,12 +355,92 @@ template <class R> struct SideRecord { R F1, G2, G3, S; };
 // inventory factor of a donor outside the workgroup; before_sync() runs after
 // the workgroup's factors are in th_s and before the barrier that publishes
 // them.
-template <class R, int STAGE, class Rec, class FarTheta, class BeforeSync>
+// A cell's outflow sum sum_k max(+-F1, 0) L and face-speed sum sum_k L S
+// over its incidences (its inventory factor and frequency follow from them).
+template <class R, class Rec>
+__device__ __forceinline__ void own_factors(const MeshView<R> &m, int k0, int k1, Rec rec, double &outv, double &sum)
+{
+    for (int kb = k0; kb < k1; kb += ADJ_CHUNK) {
+        int f[ADJ_CHUNK], fl[ADJ_CHUNK];
+        R F1[ADJ_CHUNK], L[ADJ_CHUNK], S[ADJ_CHUNK];
+#pragma unroll
+        for (int i = 0; i < ADJ_CHUNK; ++i) adj_entry(m, kb + i < k1 ? kb + i : k1 - 1, f[i], fl[i]);
+#pragma unroll
+        for (int i = 0; i < ADJ_CHUNK; ++i) {
+            const SideRecord<R> r = rec(kb + i < k1 ? kb + i : k1 - 1, f[i], (fl[i] & ADJ_OWNER) != 0);
+            F1[i] = r.F1; L[i] = m.length[f[i]]; S[i] = r.S;
+        }
+#pragma unroll
+        for (int i = 0; i < ADJ_CHUNK; ++i) {
+            if (kb + i >= k1) break;
+            const double sg = (fl[i] & ADJ_OWNER) ? 1. : -1.;
+            const double Ld = (double)L[i];
+            outv += fmax(sg * (double)F1[i], 0.) * Ld;
+            sum += Ld * (double)S[i];
+        }
+    }
+}
+
+// The inventory factors of a workgroup's ring-1 cells (Work::halo_*), one
+// per thread, computed side by side with the thread's own cell: both cells'
+// incidences are read in the same passes, so the ring costs no memory
+// latency beyond the cell's own.  Each factor is river_theta's, summed in the
+// same order.  th_h[e] receives the factor of ring cell e; the cell's own
+// outflow sum and face-speed sum are returned as in the plain pass.
+template <class R, class Rec>
+__device__ __forceinline__ void own_and_ring_factors(const MeshView<R> &m, const Work<R> &w, const StateView<R> &in,
+                                                     int c, int hc, int e, double dt, int k0, int k1, Rec rec,
+                                                     double &outv, double &sum, double *th_h)
+{
+    const bool ring = hc >= 0;
+    const int hk0 = ring ? m.offset[hc] : 0, hk1 = ring ? m.offset[hc + 1] : 0;
+    const double hA = ring ? in.h[hc] * (double)m.area[hc] : 0.;
+    const int n0 = k1 - k0, n1 = hk1 - hk0, nmax = n0 > n1 ? n0 : n1;
+    double hout = 0.;
+    for (int j = 0; j < nmax; j += ADJ_CHUNK) {
+        int f[2][ADJ_CHUNK], fl[2][ADJ_CHUNK];
+        R F1[2][ADJ_CHUNK], L[2][ADJ_CHUNK], S[ADJ_CHUNK];
+#pragma unroll
+        for (int i = 0; i < ADJ_CHUNK; ++i) {
+            const int a = j + i < n0 ? k0 + j + i : (n0 > 0 ? k1 - 1 : 0);
+            const int b = j + i < n1 ? hk0 + j + i : (n1 > 0 ? hk1 - 1 : 0);
+            adj_entry(m, a, f[0][i], fl[0][i]);
+            adj_entry(m, b, f[1][i], fl[1][i]);
+        }
+#pragma unroll
+        for (int i = 0; i < ADJ_CHUNK; ++i) {
+            const int a = j + i < n0 ? k0 + j + i : (n0 > 0 ? k1 - 1 : 0);
+            const SideRecord<R> r = rec(a, f[0][i], (fl[0][i] & ADJ_OWNER) != 0);
+            F1[0][i] = r.F1; S[i] = r.S; L[0][i] = m.length[f[0][i]];
+            F1[1][i] = w.F[f[1][i]]; L[1][i] = m.length[f[1][i]];
+        }
+#pragma unroll
+        for (int i = 0; i < ADJ_CHUNK; ++i) {
+            if (j + i < n0) {
+                const double sg = (fl[0][i] & ADJ_OWNER) ? 1. : -1.;
+                const double Ld = (double)L[0][i];
+                outv += fmax(sg * (double)F1[0][i], 0.) * Ld;
+                sum += Ld * (double)S[i];
+            }
+            if (j + i < n1) {
+                const double sg = (fl[1][i] & ADJ_OWNER) ? 1. : -1.;
+                hout += fmax(sg * (double)F1[1][i], 0.) * (double)L[1][i];
+            }
+        }
+    }
+    if (ring) {
+        double th = 1.;
+        if (hout > 0. && dt * hout >= hA) th = fmin(1., hA / (dt * hout)) * (1. - 32. * 2.2204460492503131e-16);
+        th_h[e] = th;
+    }
+}
+
+template <class R, int STAGE, bool RING, class Rec, class FarTheta, class BeforeSync>
 __device__ __forceinline__ void river_update_cells(const MeshView<R> &m, StateView<R> in, StateView<R> in0,
                                                    StateView<R> out, const Work<R> &w, const TurbineView &T,
                                                    double *wlim, const DragView &d, Control *ctl, int slot, double dt,
                                                    double drag_dt, int base, double *th_s, Rec rec,
-                                                   FarTheta far_theta, BeforeSync before_sync)
+                                                   FarTheta far_theta, BeforeSync before_sync, double *th_h = nullptr)
 {
     const int nc = m.nc, nf = m.nf;
     const int c = base + threadIdx.x;
@@ -368,31 +448,29 @@ __device__ __forceinline__ void river_update_cells(const MeshView<R> &m, StateVi
     // The cell's state is read ahead of the barrier (loads do not cross it).
     double h = 0., hu = 0., hv = 0., h0 = 0., hu0 = 0., hv0 = 0., Ad = 1.;
     int k0 = 0, k1 = 0;
+    double outv = 0., sum = 0.;
     if (c < nc) {
         k0 = m.offset[c]; k1 = m.offset[c + 1];
         h = in.h[c]; hu = in.hu[c]; hv = in.hv[c];
         if (STAGE == 1) { h0 = in0.h[c]; hu0 = in0.hu[c]; hv0 = in0.hv[c]; }
         Ad = (double)m.area[c];
-        double outv = 0., sum = 0.;
-        for (int kb = k0; kb < k1; kb += ADJ_CHUNK) {
-            int f[ADJ_CHUNK], fl[ADJ_CHUNK];
-            R F1[ADJ_CHUNK], L[ADJ_CHUNK], S[ADJ_CHUNK];
-#pragma unroll
-            for (int i = 0; i < ADJ_CHUNK; ++i) adj_entry(m, kb + i < k1 ? kb + i : k1 - 1, f[i], fl[i]);
-#pragma unroll
-            for (int i = 0; i < ADJ_CHUNK; ++i) {
-                const SideRecord<R> r = rec(kb + i < k1 ? kb + i : k1 - 1, f[i], (fl[i] & ADJ_OWNER) != 0);
-                F1[i] = r.F1; L[i] = m.length[f[i]]; S[i] = r.S;
-            }
-#pragma unroll
-            for (int i = 0; i < ADJ_CHUNK; ++i) {
-                if (kb + i >= k1) break;
-                const double sg = (fl[i] & ADJ_OWNER) ? 1. : -1.;
-                const double Ld = (double)L[i];
-                outv += fmax(sg * (double)F1[i], 0.) * Ld;
-                sum += Ld * (double)S[i];
-            }
+    }
+    bool ring_wave = false;
+    if constexpr (RING) {
+        // ring cell e = threadIdx.x of this workgroup: the ring cells fill
+        // the first wavefronts, and a wavefront without one takes the plain
+        // pass (the test is uniform over the wavefront)
+        const int b = base / (int)blockDim.x;
+        const int r0 = w.halo_off[b], nh = w.halo_off[b + 1] - r0;
+        const int e = threadIdx.x;
+        ring_wave = (int)(threadIdx.x & ~63u) < nh;
+        if (ring_wave) {
+            const int hc = e < nh ? w.halo_cell[r0 + e] : -1;
+            own_and_ring_factors<R>(m, w, in, c, hc, e, dt, k0, k1, rec, outv, sum, th_h);
         }
+    }
+    if (!ring_wave && c < nc) own_factors<R>(m, k0, k1, rec, outv, sum);
+    if (c < nc) {
         const double A = Ad, hA = h * A;
         if (outv > 0. && dt * outv >= hA) thc = fmin(1., hA / (dt * outv)) * (1. - 32. * 2.2204460492503131e-16);
         freq = sum / A;
@@ -455,8 +533,11 @@ __device__ __forceinline__ void river_update_cells(const MeshView<R> &m, StateVi
             if (wl != 0.) {
                 const int donor = wl < 0. ? j : o;
                 const int r = donor - base;
-                const double th = donor < 0 ? 1. : (r >= 0 && r < (int)blockDim.x ? th_s[r]
-                                                                               : far_theta(ka[i], donor));
+                double th;
+                if (donor < 0) th = 1.;
+                else if (r >= 0 && r < (int)blockDim.x) th = th_s[r];
+                else if constexpr (RING) th = th_h[w.adj_halo[ka[i]]];
+                else th = far_theta(ka[i], donor);
                 if (donor >= 0 && th != 1.) {
                     if (fl[i] & ADJ_OPEN) fail = RFAIL_BOUNDARY;
                     else {
@@ -505,17 +586,21 @@ __device__ __forceinline__ void k_river_update_body(MeshView<R> m, StateView<R> 
                                                       int lb)
 {
     __shared__ double th_s[256];
+    __shared__ double th_h[256];
     const int nf = m.nf;
     const R *F = w.F;
-    river_update_cells<R, STAGE>(
-        m, in, in0, out, w, T, wlim, d, ctl, slot, dt, drag_dt, lb * (int)blockDim.x, th_s,
-        [&](int, int f, bool own) {
-            SideRecord<R> r;
-            r.F1 = F[f]; r.G2 = F[(own ? nf : 3 * nf) + f]; r.G3 = F[(own ? 2 * nf : 4 * nf) + f]; r.S = F[5 * nf + f];
-            return r;
-        },
-        [&](int, int donor) { return river_theta<R>(m, w, in, donor, dt); },
-        [] {});
+    auto rec = [&](int, int f, bool own) {
+        SideRecord<R> r;
+        r.F1 = F[f]; r.G2 = F[(own ? nf : 3 * nf) + f]; r.G3 = F[(own ? 2 * nf : 4 * nf) + f]; r.S = F[5 * nf + f];
+        return r;
+    };
+    auto far = [&](int, int donor) { return river_theta<R>(m, w, in, donor, dt); };
+    if (w.halo_off)
+        river_update_cells<R, STAGE, true>(m, in, in0, out, w, T, wlim, d, ctl, slot, dt, drag_dt,
+                                           lb * (int)blockDim.x, th_s, rec, far, [] {}, th_h);
+    else
+        river_update_cells<R, STAGE, false>(m, in, in0, out, w, T, wlim, d, ctl, slot, dt, drag_dt,
+                                            lb * (int)blockDim.x, th_s, rec, far, [] {});
 }
 
 template <class R, int STAGE>
@@ -555,7 +640,7 @@ k_river_fused(MeshView<float> m, StateView<float> in, StateView<float> in0, Stat
     }
     __syncthreads();
     const int h0 = P.halo_off[b], nh = P.halo_off[b + 1] - h0;
-    river_update_cells<float, STAGE>(
+    river_update_cells<float, STAGE, false>(
         m, in, in0, out, w, T, wlim, d, ctl, slot, dt, drag_dt, b * (int)blockDim.x, th_s,
         [&](int k, int, bool own) {
             const int s = P.adj_slot[k];

@@ -89,6 +89,14 @@ template <class R> struct Work {
     // stage input, written by the launch that produces that input (river
     // path); null: the readers divide the state themselves.
     R *vel;
+    // Ring-1 cells of each river-update workgroup of 256 cells (cells across
+    // a face of one of its cells, outside it): halo_cell[halo_off[b] ..
+    // halo_off[b+1]), at most 256 per workgroup; adj_halo[k] is the position
+    // there of the cell across incidence k (-1 inside the workgroup or at an
+    // open face).  Null: the update recomputes the factor of an outside
+    // donor where it meets it.
+    const int *halo_off, *halo_cell;
+    const short *adj_halo;
 };
 
 // Per-slot reductions; slots 0-2 are the SSPRK(3,3) stages, slot 3 a
