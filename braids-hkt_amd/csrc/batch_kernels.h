@@ -44,7 +44,7 @@ __global__ void __launch_bounds__(256) kb_drag(BatchPlan P, const SlotView<R> *_
 }
 
 template <class R, int SCHEME>
-__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(sizeof(R) == 4 ? 6 : 1)))
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(sizeof(R) == 4 ? 5 : 1)))
 kb_prep(BatchPlan P, const SlotView<R> *__restrict__ V, const SlotRun *run, int stage)
 {
     const int s = find_slot(P.cell, P.nslot, blockIdx.x);

@@ -156,6 +156,17 @@ __device__ __forceinline__ void k_prep_body(MeshView<R> m, StateView<R> q, const
             R uc = R(0), vc = R(0);
             if (w.vel) { uc = w.vel[c]; vc = w.vel[nc + c]; }
             else if (h > (double)dry) { uc = (R)(q.hu[c] / h); vc = (R)(q.hv[c] / h); }
+            // The cell's LS matrix and the displacements of its first
+            // ADJ_CHUNK incidences are read here, alongside the neighbour
+            // gathers, rather than after them.
+            const R a11 = m.ls[c], a12 = m.ls[nc + c], a22 = m.ls[2 * nc + c], det = m.ls[3 * nc + c];
+            R d0p[ADJ_CHUNK], d1p[ADJ_CHUNK];
+            int flp[ADJ_CHUNK];
+#pragma unroll
+            for (int i = 0; i < ADJ_CHUNK; ++i) {
+                const int k = k0 + i < k1 ? k0 + i : (k1 > 0 ? k1 - 1 : 0);
+                d0p[i] = m.adj_d[k]; d1p[i] = m.adj_d[m.nadj + k]; flp[i] = m.adj_flags[k];
+            }
             if (act) {
                 R b0[4] = {R(0), R(0), R(0), R(0)}, b1[4] = {R(0), R(0), R(0), R(0)};
                 // lo/hi as offsets from the centre value for eta and z (FP64 differences)
@@ -201,7 +212,6 @@ __device__ __forceinline__ void k_prep_body(MeshView<R> m, StateView<R> q, const
                     lo[3] = vmin(lo[3], dz); hi[3] = vmax(hi[3], dz);
                 }
                 }
-                const R a11 = m.ls[c], a12 = m.ls[nc + c], a22 = m.ls[2 * nc + c], det = m.ls[3 * nc + c];
                 if (!(det <= R(1e-12) * ((a11 + a22) * (a11 + a22)))) {
                     // The roundoff guard of the reference limiter (64 eps of FP64):
                     // the eta and z differences are formed in FP64, so an increment
@@ -223,7 +233,8 @@ __device__ __forceinline__ void k_prep_body(MeshView<R> m, StateView<R> q, const
 #pragma unroll
                     for (int i = 0; i < ADJ_CHUNK; ++i) {
                         const int k = kb + i < k1 ? kb + i : k1 - 1;
-                        r0a[i] = m.adj_d[k]; r1a[i] = m.adj_d[m.nadj + k]; fla[i] = m.adj_flags[k];
+                        if (kb == k0) { r0a[i] = d0p[i]; r1a[i] = d1p[i]; fla[i] = flp[i]; }
+                        else { r0a[i] = m.adj_d[k]; r1a[i] = m.adj_d[m.nadj + k]; fla[i] = m.adj_flags[k]; }
                     }
 #pragma unroll
                     for (int i = 0; i < ADJ_CHUNK; ++i) {
@@ -259,7 +270,8 @@ __device__ __forceinline__ void k_prep_body(MeshView<R> m, StateView<R> q, const
 #pragma unroll
                     for (int i = 0; i < ADJ_CHUNK; ++i) {
                         const int k = kb + i < k1 ? kb + i : k1 - 1;
-                        r0a[i] = m.adj_d[k]; r1a[i] = m.adj_d[m.nadj + k];
+                        if (kb == k0) { r0a[i] = d0p[i]; r1a[i] = d1p[i]; }
+                        else { r0a[i] = m.adj_d[k]; r1a[i] = m.adj_d[m.nadj + k]; }
                     }
 #pragma unroll
                     for (int i = 0; i < ADJ_CHUNK; ++i) {
@@ -281,7 +293,7 @@ __device__ __forceinline__ void k_prep_body(MeshView<R> m, StateView<R> q, const
 }
 
 template <class R, int SCHEME>
-__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(sizeof(R) == 4 ? 6 : 1))) k_prep(MeshView<R> m, StateView<R> q, const double *__restrict__ z,
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(sizeof(R) == 4 ? 5 : 1))) k_prep(MeshView<R> m, StateView<R> q, const double *__restrict__ z,
                                               Work<R> w, Control *ctl, int stage, int remap, int want_hmax)
 {
     k_prep_body<R, SCHEME>(m, q, z, w, ctl, stage, remap, want_hmax, logical_block(remap));
@@ -791,6 +803,53 @@ struct FaceRecordShared {
     __device__ void operator()(int j, float v) const { F[j][s] = v; }
 };
 
+// What face_value reads of one cell, loaded in one pass so that both sides
+// of a face are gathered together (one memory latency instead of several).
+template <class R, int SCHEME>
+struct FaceSide {
+    double h, z;
+    R u, v;
+    R g[SCHEME == 5 ? 8 : 1];
+    int flags;
+};
+
+template <class R, int SCHEME>
+__device__ __forceinline__ FaceSide<R, SCHEME> face_side_load(const MeshView<R> &m, const StateView<R> &q,
+                                                              const double *z, const Work<R> &w, int c)
+{
+    FaceSide<R, SCHEME> s;
+    s.h = q.h[c]; s.z = z[c];
+    if (w.vel) { s.u = w.vel[c]; s.v = w.vel[m.nc + c]; }
+    else if (s.h > (double)m.dry) { s.u = (R)(q.hu[c] / s.h); s.v = (R)(q.hv[c] / s.h); }
+    else { s.u = R(0); s.v = R(0); }
+    s.flags = w.flags[c];
+    if constexpr (SCHEME == 5) {
+#pragma unroll
+        for (int j = 0; j < 8; ++j) s.g[j] = w.grad[j * m.nc + c];
+    }
+    return s;
+}
+
+// face_value from a loaded side (the same operations in the same order).
+template <class R, int SCHEME>
+__device__ __forceinline__ void face_value_of(const FaceSide<R, SCHEME> &s, R dry, R r0, R r1, double &eta, R &u,
+                                              R &v, double &zb)
+{
+    const double zc = s.z, etac = s.h + zc;
+    const R uc = s.u, vc = s.v;
+    if constexpr (SCHEME == 1) { eta = etac; u = uc; v = vc; zb = zc; return; }
+    else {
+        eta = etac + ((double)(s.g[0] * r0) + (double)(s.g[1] * r1));
+        zb = zc + ((double)(s.g[6] * r0) + (double)(s.g[7] * r1));
+        u = uc + s.g[2] * r0 + s.g[3] * r1;
+        v = vc + s.g[4] * r0 + s.g[5] * r1;
+        if (s.flags & FLAG_HCONST) {
+            eta = etac; u = uc; v = vc;
+            if (etac - zc <= (double)dry) zb = zc;
+        }
+    }
+}
+
 template <int SCHEME, class Out>
 __device__ __forceinline__ void face_one_f32(const MeshView<float> &m, const StateView<float> &q,
                                              const double *__restrict__ z, const Work<float> &w, int f, Out out)
@@ -802,19 +861,24 @@ __device__ __forceinline__ void face_one_f32(const MeshView<float> &m, const Sta
     const int c = m.owner[f], n = m.neighbour[f], b = m.bid[f];
     const R nx = m.normal[f], ny = m.normal[nf + f];
     const double nxd = nx, nyd = ny;
-    const double hc = q.h[c];
+    // both sides' cell data and displacements in one pass (the owner stands
+    // in for a missing neighbour; its copy is then unused)
+    const FaceSide<R, SCHEME> sL = face_side_load<R, SCHEME>(m, q, z, w, c);
+    const FaceSide<R, SCHEME> sR = face_side_load<R, SCHEME>(m, q, z, w, n >= 0 ? n : c);
+    const R dl0 = m.dl[f], dl1 = m.dl[nf + f], dr0 = m.dr[f], dr1 = m.dr[nf + f];
+    const double hc = sL.h;
     if (b < 0 && hc <= dd) {
-        if (n < 0 || q.h[n] <= dd) {
+        if (n < 0 || sR.h <= dd) {
             for (int j = 0; j < 6; ++j) out(j, 0.f);
             return;
         }
     }
     double etaL, zL; R uL, vL;
-    face_value<R, SCHEME>(m, q, z, w, c, m.dl[f], m.dl[nf + f], etaL, uL, vL, zL);
+    face_value_of<R, SCHEME>(sL, dry, dl0, dl1, etaL, uL, vL, zL);
     const double rawL = vmax(etaL - zL, 0.0);
     double depthL = rawL;
     if (hc <= dd) depthL = 0;
-    if ((w.flags[c] & FLAG_HCONST) && depthL <= dd) depthL = 0;
+    if ((sL.flags & FLAG_HCONST) && depthL <= dd) depthL = 0;
     R l[3] = {(R)depthL, (R)depthL * uL, (R)depthL * vL};
     R fl[3] = {0.f, 0.f, 0.f}, fr[3] = {0.f, 0.f, 0.f}, speed = 0.f;
     // pressure brackets (FP64), added along n to each side
@@ -828,23 +892,24 @@ __device__ __forceinline__ void face_one_f32(const MeshView<float> &m, const Sta
         return .5 * gg * (dface * dface - hcell * hcell);
     };
     if (n >= 0) {
-        const double hn = q.h[n];
+        const double hn = sR.h;
         double etaR, zR; R uR, vR;
-        face_value<R, SCHEME>(m, q, z, w, n, m.dr[f], m.dr[nf + f], etaR, uR, vR, zR);
+        face_value_of<R, SCHEME>(sR, dry, dr0, dr1, etaR, uR, vR, zR);
         const double rawR = vmax(etaR - zR, 0.0);
         double depthR = rawR;
         if (hn <= dd) depthR = 0;
-        if ((w.flags[n] & FLAG_HCONST) && depthR <= dd) depthR = 0;
+        if ((sR.flags & FLAG_HCONST) && depthR <= dd) depthR = 0;
         R r[3] = {(R)depthR, (R)depthR * uR, (R)depthR * vR};
         bool closed = false;
         if (vmin(hc, hn) <= dd) {
-            const double ofd = fabs((double)m.dl[f] * nxd + (double)m.dl[nf + f] * nyd);
-            const double nfd = fabs((double)m.dr[f] * nxd + (double)m.dr[nf + f] * nyd);
+            const double ofd = fabs((double)dl0 * nxd + (double)dl1 * nyd);
+            const double nfd = fabs((double)dr0 * nxd + (double)dr1 * nyd);
+            const double zc_ = sL.z, zn_ = sR.z;
             double shared;
             if (ofd + nfd > 2.2250738585072014e-308)
-                shared = (nfd * (z[c] + hc) + ofd * (z[n] + hn)) / (ofd + nfd) - vmax(z[c], z[n]);
+                shared = (nfd * (zc_ + hc) + ofd * (zn_ + hn)) / (ofd + nfd) - vmax(zc_, zn_);
             else
-                shared = .5 * ((z[c] + hc) + (z[n] + hn)) - vmax(z[c], z[n]);
+                shared = .5 * ((zc_ + hc) + (zn_ + hn)) - vmax(zc_, zn_);
             closed = shared <= dd;
             if (closed) closed = !runup_opens_face<R>(l, r, zL, zR, nx, ny, g, dry);
         }
@@ -854,8 +919,8 @@ __device__ __forceinline__ void face_one_f32(const MeshView<float> &m, const Sta
         } else {
             wave_flux_f32(l, r, depthL, depthR, zL, zR, nx, ny, g, dry, fl, fr, speed);
         }
-        PL = wb_bracket(hc, depthL, rawL, zL, z[c], (w.flags[c] & FLAG_HCONST) != 0);
-        PR = wb_bracket(hn, depthR, rawR, zR, z[n], (w.flags[n] & FLAG_HCONST) != 0);
+        PL = wb_bracket(hc, depthL, rawL, zL, sL.z, (sL.flags & FLAG_HCONST) != 0);
+        PR = wb_bracket(hn, depthR, rawR, zR, sR.z, (sR.flags & FLAG_HCONST) != 0);
     } else if (b < 0) {
         R d = l[1] * nx + l[2] * ny;
         R r[3] = {l[0], l[1] - 2.f * d * nx, l[2] - 2.f * d * ny};
@@ -864,7 +929,7 @@ __device__ __forceinline__ void face_one_f32(const MeshView<float> &m, const Sta
         R p = fl[1] * nx + fl[2] * ny;
         fl[1] = p * nx; fl[2] = p * ny;
         fr[0] = fl[0]; fr[1] = fl[1]; fr[2] = fl[2];
-        PL = wb_bracket(hc, depthL, rawL, zL, z[c], (w.flags[c] & FLAG_HCONST) != 0);
+        PL = wb_bracket(hc, depthL, rawL, zL, sL.z, (sL.flags & FLAG_HCONST) != 0);
         PR = PL;
     } else {
         const R t0 = w.trace[f], t1 = w.trace[nf + f], t2 = w.trace[2 * nf + f];
@@ -878,9 +943,9 @@ __device__ __forceinline__ void face_one_f32(const MeshView<float> &m, const Sta
         PL = depth > dry ? .5 * gg * (dt0 * dt0 - hc * hc) : -.5 * gg * hc * hc;
         if (m.mode[b] == 2 && hc > dd) {
             const double qn = (double)q.hu[c] * nxd + (double)q.hv[c] * nyd;
-            if (fabs(zL - z[c]) > 64 * 2.2204460492503131e-16 * vmax(1.0, fabs(z[c])) && fabs(qn) < hc * sqrt(gg * hc)) {
+            if (fabs(zL - sL.z) > 64 * 2.2204460492503131e-16 * vmax(1.0, fabs(sL.z)) && fabs(qn) < hc * sqrt(gg * hc)) {
                 const double d = rawL;
-                PL += (w.flags[c] & FLAG_HCONST) ? .5 * gg * (hc * hc - d * d) : .5 * gg * (hc + d) * (zL - z[c]);
+                PL += (sL.flags & FLAG_HCONST) ? .5 * gg * (hc * hc - d * d) : .5 * gg * (hc + d) * (zL - sL.z);
             }
         }
         PR = PL;
