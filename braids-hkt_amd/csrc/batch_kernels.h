@@ -87,7 +87,7 @@ __global__ void __launch_bounds__(256) kb_rotors(BatchPlan P, const SlotView<R> 
 }
 
 template <class R, int STAGE>
-__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(sizeof(R) == 4 ? 6 : 1)))
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(sizeof(R) == 4 ? 5 : 1)))
 kb_update(BatchPlan P, const SlotView<R> *__restrict__ V, const SlotRun *run)
 {
     const int s = find_slot(P.cell, P.nslot, blockIdx.x);

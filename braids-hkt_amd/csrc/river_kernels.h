@@ -604,7 +604,7 @@ __device__ __forceinline__ void k_river_update_body(MeshView<R> m, StateView<R> 
 }
 
 template <class R, int STAGE>
-__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(sizeof(R) == 4 ? 6 : 1))) k_river_update(MeshView<R> m, StateView<R> in, StateView<R> in0,
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(sizeof(R) == 4 ? 5 : 1))) k_river_update(MeshView<R> m, StateView<R> in, StateView<R> in0,
                                                       StateView<R> out, Work<R> w, TurbineView T, double *wlim,
                                                       DragView d, Control *ctl, int slot, double dt, double drag_dt,
                                                       int remap)
