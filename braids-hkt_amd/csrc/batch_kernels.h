@@ -1,6 +1,6 @@
 // Batched river substeps: every kernel kind of a substep is ONE launch over
 // all scenarios of a batch.  Workgroups map to scenarios ("slots") through
-// per-kind block prefix tables; each slot's arrays are reached through a
+// per-kind block maps (slot and block within the slot's range); each slot's arrays are reached through a
 // device-resident slot table, and each slot carries its own step controller
 // (hint, rejections, accepted count) that k_batch_finalize advances on the
 // device.  A substep therefore needs no host synchronisation and the whole
@@ -12,32 +12,24 @@
 
 namespace braids {
 
-__device__ __forceinline__ int find_slot(const int *prefix, int n, int b)
-{
-    int lo = 0, hi = n;         // prefix[lo] <= b < prefix[hi]
-    while (hi - lo > 1) {
-        const int mid = (lo + hi) >> 1;
-        if (prefix[mid] <= b) lo = mid; else hi = mid;
-    }
-    return lo;
-}
-
 static __global__ void __launch_bounds__(256) kb_zero(BatchPlan P, const int *nzero, unsigned int *const *blocks,
                                                const SlotRun *run)
 {
-    const int s = find_slot(P.zero, P.nslot, blockIdx.x);
+    const int2 e = P.zero_map[blockIdx.x];
+    const int s = e.x;
     if (!run[s].active) return;
-    const int i = (blockIdx.x - P.zero[s]) * blockDim.x + threadIdx.x;
+    const int i = e.y * blockDim.x + threadIdx.x;
     if (i < nzero[s]) blocks[s][i] = 0u;
 }
 
 template <class R>
 __global__ void __launch_bounds__(256) kb_drag(BatchPlan P, const SlotView<R> *__restrict__ V, SlotRun *run)
 {
-    const int s = find_slot(P.cell, P.nslot, blockIdx.x);
+    const int2 e = P.cell_map[blockIdx.x];
+    const int s = e.x;
     if (!run[s].active) return;
     const SlotView<R> &v = V[s];
-    const int lb = blockIdx.x - P.cell[s];
+    const int lb = e.y;
     if (lb >= v.ncb) return;
     k_river_drag_body<R>(v.m.nc, v.q[run[s].parity], v.work, v.rb.drag, .5 * run[s].hint, (double)v.m.grav,
                          (double)v.m.dry, 1, xcd_order(lb, v.ncb), v.w.vel);
@@ -47,10 +39,11 @@ template <class R, int SCHEME>
 __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(sizeof(R) == 4 ? 5 : 1)))
 kb_prep(BatchPlan P, const SlotView<R> *__restrict__ V, const SlotRun *run, int stage)
 {
-    const int s = find_slot(P.cell, P.nslot, blockIdx.x);
+    const int2 e = P.cell_map[blockIdx.x];
+    const int s = e.x;
     if (!run[s].active) return;
     const SlotView<R> &v = V[s];
-    const int lb = blockIdx.x - P.cell[s];
+    const int lb = e.y;
     if (lb >= v.ncb) return;
     const StateView<R> in = stage == 0 ? v.work : v.q1;
     if constexpr (std::is_same<R, double>::value && SCHEME == 5)
@@ -64,10 +57,11 @@ kb_prep(BatchPlan P, const SlotView<R> *__restrict__ V, const SlotRun *run, int 
 template <class R, int SCHEME>
 __global__ void __launch_bounds__(256) kb_face(BatchPlan P, const SlotView<R> *__restrict__ V, const SlotRun *run, int stage)
 {
-    const int s = find_slot(P.face, P.nslot, blockIdx.x);
+    const int2 e = P.face_map[blockIdx.x];
+    const int s = e.x;
     if (!run[s].active) return;
     const SlotView<R> &v = V[s];
-    const int nfb = (v.m.nf + 255) >> 8, lead = face_lead(v.m.nb, 0), lb = blockIdx.x - P.face[s];
+    const int nfb = (v.m.nf + 255) >> 8, lead = face_lead(v.m.nb, 0), lb = e.y;
     if (lb >= lead + nfb) return;
     const StateView<R> in = stage == 0 ? v.work : v.q1;
     const double t = run[s].time + (stage ? run[s].hint : 0.0);
@@ -79,10 +73,11 @@ __global__ void __launch_bounds__(256) kb_face(BatchPlan P, const SlotView<R> *_
 template <class R>
 __global__ void __launch_bounds__(256) kb_rotors(BatchPlan P, const SlotView<R> *__restrict__ V, const SlotRun *run, int stage)
 {
-    const int s = find_slot(P.rot, P.nslot, blockIdx.x);
+    const int2 e = P.rot_map[blockIdx.x];
+    const int s = e.x;
     if (!run[s].active) return;
     const SlotView<R> &v = V[s];
-    const int a = blockIdx.x - P.rot[s];      // one workgroup per rotor
+    const int a = e.y;                        // one workgroup per rotor
     if (a < v.T.nrot) turbine_sample<R>(v.T, v.m, stage == 0 ? v.work : v.q1, v.z, stage, a);
 }
 
@@ -90,10 +85,11 @@ template <class R, int STAGE>
 __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(sizeof(R) == 4 ? 5 : 1)))
 kb_update(BatchPlan P, const SlotView<R> *__restrict__ V, const SlotRun *run)
 {
-    const int s = find_slot(P.cell, P.nslot, blockIdx.x);
+    const int2 e = P.cell_map[blockIdx.x];
+    const int s = e.x;
     if (!run[s].active) return;
     const SlotView<R> &v = V[s];
-    const int lb = blockIdx.x - P.cell[s];
+    const int lb = e.y;
     if (lb >= v.ncb) return;
     const double dt = run[s].hint;
     if (STAGE == 0)
@@ -107,10 +103,11 @@ kb_update(BatchPlan P, const SlotView<R> *__restrict__ V, const SlotRun *run)
 template <class R>
 __global__ void __launch_bounds__(TRACE_BLOCK) kb_ledger(BatchPlan P, const SlotView<R> *__restrict__ V, const SlotRun *run)
 {
-    const int s = find_slot(P.bnd, P.nslot, blockIdx.x);
+    const int2 e = P.bnd_map[blockIdx.x];
+    const int s = e.x;
     if (!run[s].active) return;
     const SlotView<R> &v = V[s];
-    k_river_ledger_body<R>(v.m, v.rb.wlim[0], v.rb.wlim[1], v.rb.ledger, run[s].hint, blockIdx.x - P.bnd[s]);
+    k_river_ledger_body<R>(v.m, v.rb.wlim[0], v.rb.wlim[1], v.rb.ledger, run[s].hint, e.y);
 }
 
 // One workgroup per slot: reduce the stage maxima, then take the substep's

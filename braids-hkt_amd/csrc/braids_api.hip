@@ -962,6 +962,8 @@ struct Batch : BatchBase {
     SlotView<R> *views = nullptr;
     SlotRun *runs = nullptr, *runs_host = nullptr;
     int *prefix = nullptr, *nzero = nullptr;
+    int2 *bmap = nullptr;
+    size_t bmap_cap = 0;
     unsigned int **blocks = nullptr;
     BatchPlan plan{};
     hipGraph_t graph = nullptr;
@@ -972,7 +974,7 @@ struct Batch : BatchBase {
         if (exec) (void)hipGraphExecDestroy(exec);
         if (graph) (void)hipGraphDestroy(graph);
         for (auto &sl : slots) { (void)hipFree(sl.ledger_acc); (void)hipFree(sl.energy); }
-        for (void *p : {(void *)views, (void *)runs, (void *)prefix, (void *)nzero, (void *)blocks})
+        for (void *p : {(void *)views, (void *)runs, (void *)prefix, (void *)nzero, (void *)blocks, (void *)bmap})
             if (p) (void)hipFree(p);
         if (runs_host) (void)hipHostFree(runs_host);
         if (own_stream && stream) (void)hipStreamDestroy(stream);
@@ -1034,6 +1036,19 @@ struct Batch : BatchBase {
             pre[3 * (n + 1) + i + 1] = pre[3 * (n + 1) + i] + (v.nzero + 255) / 256;
             pre[4 * (n + 1) + i + 1] = pre[4 * (n + 1) + i] + nrb;
         }
+        // block maps of the five kinds, back to back
+        std::vector<int2> bm;
+        for (int kind = 0; kind < 5; ++kind)
+            for (size_t i = 0; i < n; ++i)
+                for (int b = pre[kind * (n + 1) + i]; b < pre[kind * (n + 1) + i + 1]; ++b)
+                    bm.push_back(make_int2((int)i, b - pre[kind * (n + 1) + i]));
+        if (bm.size() > bmap_cap) {
+            if (bmap) (void)hipFree(bmap);
+            bmap = nullptr;
+            bmap_cap = std::max(bm.size(), 2 * bmap_cap);
+            HIP_TRY(hipMalloc(&bmap, bmap_cap * sizeof(int2)));
+        }
+        if (!bm.empty()) HIP_TRY(hipMemcpy(bmap, bm.data(), bm.size() * sizeof(int2), hipMemcpyHostToDevice));
         if (n) {
             HIP_TRY(hipMemcpy(views, hv.data(), n * sizeof(SlotView<R>), hipMemcpyHostToDevice));
             HIP_TRY(hipMemcpy(runs, hr.data(), n * sizeof(SlotRun), hipMemcpyHostToDevice));
@@ -1047,6 +1062,8 @@ struct Batch : BatchBase {
         plan.bnd = prefix + 2 * (n + 1); plan.zero = prefix + 3 * (n + 1); plan.rot = prefix + 4 * (n + 1);
         plan.ncell = pre[n]; plan.nface = pre[(n + 1) + n];
         plan.nbnd = pre[2 * (n + 1) + n]; plan.nzero = pre[3 * (n + 1) + n]; plan.nrot = pre[4 * (n + 1) + n];
+        plan.cell_map = bmap; plan.face_map = plan.cell_map + plan.ncell; plan.bnd_map = plan.face_map + plan.nface;
+        plan.zero_map = plan.bnd_map + plan.nbnd; plan.rot_map = plan.zero_map + plan.nzero;
         dirty = false; dirty_layout = false;
         if (n == 0) return BRAIDS_OK;
         // No solver set-up runs on another thread during the capture (the

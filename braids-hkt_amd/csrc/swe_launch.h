@@ -109,6 +109,10 @@ struct BatchPlan {
     const int *rot;             // rotor blocks (one rotor each)
     const int *bnd;             // boundaries (one block each)
     const int *zero;            // control-clearing blocks
+    // Block maps of each kind: entry b = (slot, block within the slot's
+    // range) of the kind's launch block b, so a workgroup finds its slot
+    // with one load.
+    const int2 *cell_map, *face_map, *rot_map, *bnd_map, *zero_map;
 };
 
 // One batched substep for every active slot (no host synchronisation).
