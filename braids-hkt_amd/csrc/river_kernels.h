@@ -139,6 +139,41 @@ __device__ __forceinline__ double group_min(double v, double *sh)
     return s;
 }
 
+// NS workgroup-wide sums and NM minima in one exchange: each value reduced
+// exactly as group_sum / group_min reduce it (the same butterflies, the
+// per-wave partials combined in the same order), with two barriers for all.
+template <int NS, int NM>
+__device__ __forceinline__ void group_sums_mins(double *s, double *mn, double *sh)
+{
+    constexpr int N = NS + NM;
+#pragma unroll
+    for (int j = 0; j < NS; ++j) s[j] = wave_sum_d(s[j]);
+#pragma unroll
+    for (int j = 0; j < NM; ++j) mn[j] = wave_min_d(mn[j]);
+    __syncthreads();
+    if ((threadIdx.x & 63) == 0) {
+        double *p = sh + (threadIdx.x >> 6) * N;
+#pragma unroll
+        for (int j = 0; j < NS; ++j) p[j] = s[j];
+#pragma unroll
+        for (int j = 0; j < NM; ++j) p[NS + j] = mn[j];
+    }
+    __syncthreads();
+    const int nw = (int)(blockDim.x >> 6);
+#pragma unroll
+    for (int j = 0; j < NS; ++j) {
+        double t = 0.;
+        for (int i = 0; i < nw; ++i) t += sh[i * N + j];
+        s[j] = t;
+    }
+#pragma unroll
+    for (int j = 0; j < NM; ++j) {
+        double t = sh[NS + j];
+        for (int i = 1; i < nw; ++i) t = fmin(t, sh[i * N + NS + j]);
+        mn[j] = t;
+    }
+}
+
 // One workgroup per rotor (every thread of the workgroup calls this):
 // CoveredRotor._operating_state gating (river_turbine_flow.py:136-180) on the
 // rotor-plane surface/bed samples and the footprint means,
@@ -151,7 +186,7 @@ template <class R>
 __device__ void turbine_sample(const TurbineView &T, const MeshView<R> &m, StateView<R> q,
                                const double *__restrict__ z, int slot, int a)
 {
-    __shared__ double sh[16];
+    __shared__ double sh[4 * 6];
     const int lane = threadIdx.x, step = blockDim.x;
     const int nc = m.nc;
     const double dry = 1e-6;                 // CoveredRotor(dry=1e-6)
@@ -185,9 +220,7 @@ __device__ void turbine_sample(const TurbineView &T, const MeshView<R> &m, State
         low = fmin(low, fmin(surface, eta));
         high = fmax(high, fmax(bed, z[i]));
     }
-    low = group_min(low, sh);
-    high = -group_min(-high, sh);
-    // footprint means
+    // footprint means (their loads overlap the plane's; one exchange reduces both)
     double u1 = 0., depth = 0., smin = 1e300, anydry = 0.;
     for (int k = T.foot_off[a] + lane; k < T.foot_off[a + 1]; k += step) {
         const int c = T.foot_cell[k];
@@ -198,10 +231,12 @@ __device__ void turbine_sample(const TurbineView &T, const MeshView<R> &m, State
         depth += T.foot_norm[k] * h;
         smin = fmin(smin, sp);
     }
-    u1 = group_sum(u1, sh);
-    depth = group_sum(depth, sh);
-    smin = group_min(smin, sh);
-    anydry = -group_min(-anydry, sh);
+    {
+        double s2[2] = {u1, depth}, m4[4] = {low, -high, smin, -anydry};
+        group_sums_mins<2, 4>(s2, m4, sh);
+        u1 = s2[0]; depth = s2[1];
+        low = m4[0]; high = -m4[1]; smin = m4[2]; anydry = -m4[3];
+    }
     const double margin = low - T.top[a], clearance = T.bottom[a] - high;
     const int t0 = T.table_off[a], nt = T.table_off[a + 1] - t0;
     const double tol = T.tol[a];
@@ -243,8 +278,11 @@ __device__ void turbine_sample(const TurbineView &T, const MeshView<R> &m, State
         T.kslot[k] = kk;
     }
     if (err) atomicOr(T.error, 1 << slot);
-    F = group_sum(F, sh); work = group_sum(work, sh); P = group_sum(P, sh);
-    inc = group_sum(inc, sh); ctm = group_sum(ctm, sh); cpm = group_sum(cpm, sh);
+    {
+        double s6[6] = {F, work, P, inc, ctm, cpm};
+        group_sums_mins<6, 0>(s6, nullptr, sh);
+        F = s6[0]; work = s6[1]; P = s6[2]; inc = s6[3]; ctm = s6[4]; cpm = s6[5];
+    }
     if (lane == 0) {
         double *d = T.diag + ((size_t)slot * T.nrot + a) * TURBINE_DIAG;
         d[0] = reason == 0; d[1] = reason; d[2] = low; d[3] = high; d[4] = u1; d[5] = depth;
