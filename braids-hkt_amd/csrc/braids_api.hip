@@ -10,6 +10,8 @@
 #include <string>
 #include <vector>
 #include <mutex>
+#include <chrono>
+#include <atomic>
 #include <type_traits>
 
 #include "braids_hip.h"
@@ -200,6 +202,8 @@ struct Solver : SolverBase {
     double *ledger_dev = nullptr;
     void *host_block = nullptr;        // pinned readback
     unsigned int *host_block_dev = nullptr;   // its device-visible address (written by the closing kernels)
+    unsigned int *flag_host = nullptr, *flag_dev = nullptr;   // completion word of the river substep's mirror
+    unsigned int ticket = 0;
     double *diag_dev_host = nullptr;          // device-visible address of diag_host
     size_t block_bytes = 0, head_bytes = 0;
     LaunchCfg cfg;
@@ -224,6 +228,7 @@ struct Solver : SolverBase {
     ~Solver() override
     {
         if (host_block) (void)hipHostFree(host_block);
+        if (flag_host) (void)hipHostFree(flag_host);
         if (diag_host) (void)hipHostFree(diag_host);
         if (ex_host) (void)hipHostFree(ex_host);
         for (void *p : allocs) (void)hipFree(p);
@@ -368,13 +373,38 @@ struct Solver : SolverBase {
         ctl = (Control *)blk;
         ledger_dev = (double *)(blk + sizeof(Control));
         w.rpart = (unsigned long long *)(blk + part_off);
-        HIP_TRY(hipHostMalloc(&host_block, head_bytes, hipHostMallocDefault));
+        // coherent (fine-grained) host memory: the device's writes reach it
+        // without caching, so the completion word orders the mirror for the host
+        HIP_TRY(hipHostMalloc(&host_block, head_bytes, hipHostMallocCoherent));
         HIP_TRY(hipHostGetDevicePointer((void **)&host_block_dev, host_block, 0));
+        HIP_TRY(hipHostMalloc((void **)&flag_host, 64, hipHostMallocCoherent));
+        HIP_TRY(hipHostGetDevicePointer((void **)&flag_dev, flag_host, 0));
+        *(volatile unsigned int *)flag_host = 0u;
         ledger.assign(nb, 0.0);
         hstate.resize(nc);
         return BRAIDS_OK;
     }
 
+    // Wait for the closing launch's completion word: a spin on coherent host
+    // memory answers within a few microseconds of the kernel's last write,
+    // where a stream synchronisation takes longer to return.  After 20 ms the
+    // wait falls back to the stream synchronisation (which reports faults).
+    int wait_ticket(unsigned int tk)
+    {
+        const volatile unsigned int *f = flag_host;
+        const auto t0 = std::chrono::steady_clock::now();
+        unsigned int spins = 0;
+        while (*f != tk) {
+            if ((++spins & 1023u) == 0 && std::chrono::steady_clock::now() - t0 > std::chrono::milliseconds(20)) {
+                HIP_TRY(hipStreamSynchronize(stream));
+                if (*f != tk) { set_error("the closing launch did not report its substep"); return BRAIDS_EHIP; }
+                break;
+            }
+            __builtin_ia32_pause();
+        }
+        std::atomic_thread_fence(std::memory_order_acquire);
+        return BRAIDS_OK;
+    }
     HostMirror mirror() const
     {
         HostMirror hm{};
@@ -667,7 +697,7 @@ struct Solver : SolverBase {
         if ((rc = alloc(T.kslot, nslot))) return rc;
         if ((rc = alloc(T.diag, 2 * (size_t)n * TURBINE_DIAG))) return rc;
         HIP_TRY(hipMemset(T.diag, 0, 2 * (size_t)n * TURBINE_DIAG * sizeof(double)));
-        HIP_TRY(hipHostMalloc(&diag_host, 2 * (size_t)n * TURBINE_DIAG * sizeof(double), hipHostMallocDefault));
+        HIP_TRY(hipHostMalloc(&diag_host, 2 * (size_t)n * TURBINE_DIAG * sizeof(double), hipHostMallocCoherent));
         HIP_TRY(hipHostGetDevicePointer((void **)&diag_dev_host, diag_host, 0));
         tv = T;
         nrot = n;
@@ -684,10 +714,13 @@ struct Solver : SolverBase {
         wr.halo_off = ring_off; wr.halo_cell = ring_cell; wr.adj_halo = ring_adj;
         LaunchCfg rc_cfg = cfg;
         rc_cfg.fused = fused_on ? &fplan : nullptr;
+        HostMirror hmv = mirror();
+        hmv.flag = flag_dev; hmv.ticket = ++ticket;
         run_river_substep<R>(m, st[0], st[1], st[2], st[3], z, wr, ctl, rb, nrot > 0 ? &tv : nullptr, dt, t, rc_cfg, stream,
-                             (unsigned int *)ctl, (int)(block_bytes / 4), mirror());
+                             (unsigned int *)ctl, (int)(block_bytes / 4), hmv);
         HIP_TRY(hipGetLastError());
-        HIP_TRY(hipStreamSynchronize(stream));
+        if (profiling) HIP_TRY(hipStreamSynchronize(stream));
+        else if ((rc = wait_ticket(hmv.ticket))) return rc;
         if (profiling) collect(2, fused_on ? 3 : 1);
         const Control *hc = (const Control *)host_block;
         const double *hl = (const double *)((const char *)host_block + sizeof(Control));

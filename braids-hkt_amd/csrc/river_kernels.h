@@ -752,6 +752,10 @@ __device__ __forceinline__ void finalize_block(const unsigned long long *rpart, 
         const volatile double *dg = hm.diag;
         for (int i = t; i < hm.ndiag; i += blockDim.x) hm.diag_host[i] = dg[i];
         __threadfence_system();
+        if (hm.flag) {
+            __syncthreads();
+            if (t == 0) { *(volatile unsigned int *)hm.flag = hm.ticket; __threadfence_system(); }
+        }
     }
 }
 

@@ -120,6 +120,10 @@ struct HostMirror {
     const double *diag;        // rotor diagnostics on the device
     double *diag_host;         // device-visible pointer of their pinned mirror
     int ndiag;
+    // Completion word (NULL: none): written with `ticket` after the mirror,
+    // so a host that sees the ticket there sees the whole mirror.
+    unsigned int *flag;
+    unsigned int ticket;
 };
 
 // River path: roughness of the Strang drag half-steps and the rotor array.
