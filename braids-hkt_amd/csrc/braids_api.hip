@@ -180,6 +180,7 @@ struct SolverBase {
                 if ((k == KK_TRACES || k == KK_LEDGER) && nb == 0 && !(path == 3 && nrot > 0)) continue;
                 if (!((prof_mask >> k) & 1)) continue;
                 float ms = 0;
+                (void)hipEventSynchronize(events[(slot * KK_COUNT + k) * 2 + 1]);
                 const hipError_t e = hipEventElapsedTime(&ms, events[(slot * KK_COUNT + k) * 2],
                                                          events[(slot * KK_COUNT + k) * 2 + 1]);
                 if (e == hipSuccess) { prof_ms[k] += ms; prof_n[k] += 1; }
@@ -719,8 +720,7 @@ struct Solver : SolverBase {
         run_river_substep<R>(m, st[0], st[1], st[2], st[3], z, wr, ctl, rb, nrot > 0 ? &tv : nullptr, dt, t, rc_cfg, stream,
                              (unsigned int *)ctl, (int)(block_bytes / 4), hmv);
         HIP_TRY(hipGetLastError());
-        if (profiling) HIP_TRY(hipStreamSynchronize(stream));
-        else if ((rc = wait_ticket(hmv.ticket))) return rc;
+        if ((rc = wait_ticket(hmv.ticket))) return rc;
         if (profiling) collect(2, fused_on ? 3 : 1);
         const Control *hc = (const Control *)host_block;
         const double *hl = (const double *)((const char *)host_block + sizeof(Control));
