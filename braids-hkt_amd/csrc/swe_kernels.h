@@ -143,10 +143,22 @@ __device__ __forceinline__ void k_prep_body(MeshView<R> m, StateView<R> q, const
         const int k0 = m.offset[c], k1 = m.offset[c + 1];
         bool act = h > (double)dry;
         if (!act) {
-            for (int k = k0; k < k1; ++k) {
-                int fl = m.adj_flags[k], o = m.adj_other[k];
-                if (fl & ADJ_OPEN) { act = true; break; }
-                if (o >= 0 && q.h[o] > (double)dry) { act = true; break; }
+            // a dry cell is active next to an open face or a wet neighbour:
+            // each chunk's incidences, then their neighbours' depths, are read
+            // together (two memory latencies per chunk)
+            for (int kb = k0; kb < k1 && !act; kb += ADJ_CHUNK) {
+                int fla[ADJ_CHUNK], oa[ADJ_CHUNK];
+                double ha[ADJ_CHUNK];
+#pragma unroll
+                for (int i = 0; i < ADJ_CHUNK; ++i) {
+                    const int k = kb + i < k1 ? kb + i : k1 - 1;
+                    fla[i] = m.adj_flags[k]; oa[i] = m.adj_other[k];
+                }
+#pragma unroll
+                for (int i = 0; i < ADJ_CHUNK; ++i) ha[i] = q.h[oa[i] >= 0 ? oa[i] : c];
+#pragma unroll
+                for (int i = 0; i < ADJ_CHUNK; ++i)
+                    if (kb + i < k1 && ((fla[i] & ADJ_OPEN) || (oa[i] >= 0 && ha[i] > (double)dry))) act = true;
             }
         }
         int flags = act ? FLAG_ACTIVE : 0;
