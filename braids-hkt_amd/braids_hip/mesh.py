@@ -109,12 +109,15 @@ class Mesh:
         return int(np.max(np.diff(self.offset)))
 
     # -- renumbering -----------------------------------------------------
-    def renumbered(self, order: str = 'hilbert', bits: int = 16) -> 'Mesh':
-        """Return a copy with cells in Hilbert order and faces sorted by the
-        smaller incident (new) cell index, then the larger one.  Owner,
-        neighbour and normal orientation are preserved per face."""
+    def renumbered(self, order: str = 'hilbert', bits: int = 16, seed: int = 0) -> 'Mesh':
+        """Return a copy with cells in Hilbert order (or kept, 'identity', or
+        scattered, 'random' with `seed`: an order without locality) and faces
+        sorted by the smaller incident (new) cell index, then the larger one.
+        Owner, neighbour and normal orientation are preserved per face."""
         if order == 'identity':
             cperm = np.arange(self.nc)
+        elif order == 'random':
+            cperm = np.random.default_rng(seed).permutation(self.nc)
         elif order == 'hilbert':
             key = hilbert_keys(self.xy[0], self.xy[1], bits)
             cperm = np.lexsort((np.arange(self.nc), key))
