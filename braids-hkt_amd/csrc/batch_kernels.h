@@ -52,33 +52,21 @@ kb_prep(BatchPlan P, const SlotView<R> *__restrict__ V, const SlotRun *run, int 
         k_prep_body<R, SCHEME>(v.m, in, v.z, v.w, v.ctl, stage, 1, 0, xcd_order(lb, v.ncb));
 }
 
-// Per scenario: one block per open boundary (traces and their faces), padded
-// to 8, then its face blocks (face_launch_block).
+// Per scenario: one block per open boundary (traces and their faces) and one
+// per rotor (its sample of the stage input), padded to 8, then its face
+// blocks (face_launch_block).
 template <class R, int SCHEME>
-__global__ void __launch_bounds__(256) kb_face(BatchPlan P, const SlotView<R> *__restrict__ V, const SlotRun *run, int stage)
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(sizeof(R) == 4 ? 5 : 1))) kb_face(BatchPlan P, const SlotView<R> *__restrict__ V, const SlotRun *run, int stage)
 {
     const int2 e = P.face_map[blockIdx.x];
     const int s = e.x;
     if (!run[s].active) return;
     const SlotView<R> &v = V[s];
-    const int nfb = (v.m.nf + 255) >> 8, lead = face_lead(v.m.nb, 0), lb = e.y;
+    const int nfb = (v.m.nf + 255) >> 8, lead = face_lead(v.m.nb, v.T.nrot), lb = e.y;
     if (lb >= lead + nfb) return;
     const StateView<R> in = stage == 0 ? v.work : v.q1;
     const double t = run[s].time + (stage ? run[s].hint : 0.0);
-    face_launch_block<R, SCHEME, false>(v.m, in, v.z, v.w, v.ctl, v.T, stage, t, nfb, lead, lb, 1);
-}
-
-// The rotors of every scenario sampling its stage input (one workgroup each;
-// a launch of its own, one per stage for the whole batch).
-template <class R>
-__global__ void __launch_bounds__(256) kb_rotors(BatchPlan P, const SlotView<R> *__restrict__ V, const SlotRun *run, int stage)
-{
-    const int2 e = P.rot_map[blockIdx.x];
-    const int s = e.x;
-    if (!run[s].active) return;
-    const SlotView<R> &v = V[s];
-    const int a = e.y;                        // one workgroup per rotor
-    if (a < v.T.nrot) turbine_sample<R>(v.T, v.m, stage == 0 ? v.work : v.q1, v.z, stage, a);
+    face_launch_block<R, SCHEME, true>(v.m, in, v.z, v.w, v.ctl, v.T, stage, t, nfb, lead, lb, 1);
 }
 
 template <class R, int STAGE>

@@ -1035,14 +1035,14 @@ struct Batch : BatchBase {
             cap = std::max<size_t>(n, 2 * cap);
             HIP_TRY(hipMalloc(&views, cap * sizeof(SlotView<R>)));
             HIP_TRY(hipMalloc(&runs, cap * sizeof(SlotRun)));
-            HIP_TRY(hipMalloc(&prefix, 5 * (cap + 1) * sizeof(int)));
+            HIP_TRY(hipMalloc(&prefix, 4 * (cap + 1) * sizeof(int)));
             HIP_TRY(hipMalloc(&nzero, cap * sizeof(int)));
             HIP_TRY(hipMalloc(&blocks, cap * sizeof(unsigned int *)));
             HIP_TRY(hipHostMalloc(&runs_host, cap * sizeof(SlotRun), hipHostMallocDefault));
         }
         std::vector<SlotView<R>> hv(n);
         std::vector<SlotRun> hr(n);
-        std::vector<int> pre(5 * (n + 1), 0), nz(n);
+        std::vector<int> pre(4 * (n + 1), 0), nz(n);
         std::vector<unsigned int *> bl(n);
         auto pad8 = [](int x) { return (x + 7) & ~7; };
         for (size_t i = 0; i < n; ++i) {
@@ -1062,16 +1062,14 @@ struct Batch : BatchBase {
             hr[i] = slots[i].run;
             nz[i] = v.nzero;
             bl[i] = (unsigned int *)x->ctl;
-            const int nrb = v.T.nrot;         // kb_rotors: one workgroup per rotor
             pre[0 * (n + 1) + i + 1] = pre[0 * (n + 1) + i] + pad8(v.ncb);
-            pre[1 * (n + 1) + i + 1] = pre[1 * (n + 1) + i] + pad8(face_lead(x->nb, 0) + (x->nf + 255) / 256);
+            pre[1 * (n + 1) + i + 1] = pre[1 * (n + 1) + i] + pad8(face_lead(x->nb, v.T.nrot) + (x->nf + 255) / 256);
             pre[2 * (n + 1) + i + 1] = pre[2 * (n + 1) + i] + x->nb;
             pre[3 * (n + 1) + i + 1] = pre[3 * (n + 1) + i] + (v.nzero + 255) / 256;
-            pre[4 * (n + 1) + i + 1] = pre[4 * (n + 1) + i] + nrb;
         }
-        // block maps of the five kinds, back to back
+        // block maps of the four kinds, back to back
         std::vector<int2> bm;
-        for (int kind = 0; kind < 5; ++kind)
+        for (int kind = 0; kind < 4; ++kind)
             for (size_t i = 0; i < n; ++i)
                 for (int b = pre[kind * (n + 1) + i]; b < pre[kind * (n + 1) + i + 1]; ++b)
                     bm.push_back(make_int2((int)i, b - pre[kind * (n + 1) + i]));
@@ -1092,11 +1090,11 @@ struct Batch : BatchBase {
         plan = BatchPlan{};
         plan.nslot = (int)n; plan.scheme = scheme;
         plan.cell = prefix; plan.face = prefix + (n + 1);
-        plan.bnd = prefix + 2 * (n + 1); plan.zero = prefix + 3 * (n + 1); plan.rot = prefix + 4 * (n + 1);
+        plan.bnd = prefix + 2 * (n + 1); plan.zero = prefix + 3 * (n + 1);
         plan.ncell = pre[n]; plan.nface = pre[(n + 1) + n];
-        plan.nbnd = pre[2 * (n + 1) + n]; plan.nzero = pre[3 * (n + 1) + n]; plan.nrot = pre[4 * (n + 1) + n];
+        plan.nbnd = pre[2 * (n + 1) + n]; plan.nzero = pre[3 * (n + 1) + n];
         plan.cell_map = bmap; plan.face_map = plan.cell_map + plan.ncell; plan.bnd_map = plan.face_map + plan.nface;
-        plan.zero_map = plan.bnd_map + plan.nbnd; plan.rot_map = plan.zero_map + plan.nzero;
+        plan.zero_map = plan.bnd_map + plan.nbnd;
         dirty = false; dirty_layout = false;
         if (n == 0) return BRAIDS_OK;
         // No solver set-up runs on another thread during the capture (the

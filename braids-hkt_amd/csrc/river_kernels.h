@@ -183,7 +183,7 @@ __device__ __forceinline__ void group_sums_mins(double *s, double *mn, double *s
 // statistics the reference moves to the host (5 + 6 scalars per rotor and
 // stage) stay on the device.  A footprint at 0.5 m cells is about 400 cells.
 template <class R>
-__device__ void turbine_sample(const TurbineView &T, const MeshView<R> &m, StateView<R> q,
+__device__ __forceinline__ void turbine_sample(const TurbineView &T, const MeshView<R> &m, StateView<R> q,
                                const double *__restrict__ z, int slot, int a)
 {
     __shared__ double sh[4 * 6];

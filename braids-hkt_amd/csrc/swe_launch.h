@@ -103,16 +103,15 @@ struct SlotRun {
 
 struct BatchPlan {
     int nslot, scheme;
-    int ncell, nface, nbnd, nzero, nrot;       // grid sizes (block totals)
+    int ncell, nface, nbnd, nzero;             // grid sizes (block totals)
     const int *cell;            // [nslot+1] prefix of cell blocks (each padded to a multiple of 8)
-    const int *face;            // face blocks and one block per open boundary (padded to 8)
-    const int *rot;             // rotor blocks (one rotor each)
+    const int *face;            // one block per open boundary and per rotor, then the face blocks (padded to 8)
     const int *bnd;             // boundaries (one block each)
     const int *zero;            // control-clearing blocks
     // Block maps of each kind: entry b = (slot, block within the slot's
     // range) of the kind's launch block b, so a workgroup finds its slot
     // with one load.
-    const int2 *cell_map, *face_map, *rot_map, *bnd_map, *zero_map;
+    const int2 *cell_map, *face_map, *bnd_map, *zero_map;
 };
 
 // One batched substep for every active slot (no host synchronisation).

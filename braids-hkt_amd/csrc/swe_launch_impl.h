@@ -241,8 +241,6 @@ template <class R, int SCHEME>
 static void batch_stage(const BatchPlan &P, const SlotView<R> *V, SlotRun *run, int stage, hipStream_t s)
 {
     hipLaunchKernelGGL((kb_prep<R, SCHEME>), dim3(P.ncell), dim3(256), 0, s, P, V, (const SlotRun *)run, stage);
-    if (P.nrot > 0)
-        hipLaunchKernelGGL((kb_rotors<R>), dim3(P.nrot), dim3(256), 0, s, P, V, (const SlotRun *)run, stage);
     hipLaunchKernelGGL((kb_face<R, SCHEME>), dim3(P.nface), dim3(256), 0, s, P, V, (const SlotRun *)run, stage);
     if (stage == 0) hipLaunchKernelGGL((kb_update<R, 0>), dim3(P.ncell), dim3(256), 0, s, P, V, (const SlotRun *)run);
     else hipLaunchKernelGGL((kb_update<R, 1>), dim3(P.ncell), dim3(256), 0, s, P, V, (const SlotRun *)run);
