@@ -721,7 +721,7 @@ __device__ __forceinline__ void k_river_ledger_body(MeshView<R> m, const double 
         const int f = m.bnd_faces[k];
         s = s - dt * (double)m.length[f] * (.5 * (w1[f] + w2[f]));
     }
-    s = block_sum<TRACE_BLOCK>(s, sh);
+    s = group_sum(s, sh);          // wave butterflies, then the four wave partials in order
     if (threadIdx.x == 0) ledger[b] = s;
 }
 
