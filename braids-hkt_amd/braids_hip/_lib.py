@@ -102,6 +102,7 @@ SIGNATURES = {
     'braids_batch_poll': (_I, [_P, _P, _I, _P]),
     'braids_profile': (_I, [_P, _I]),
     'braids_profile_read': (_I, [_P, _P, _P]),
+    'braids_profile_every': (_I, [_P, _I]),
     'braids_stream_copy': (_I, [_P, _P, ctypes.c_longlong, _P, _I, _P]),
     'braids_hllc_configure': (_I, [_P, _D, _I, _P, _P, _I]),
     'braids_hllc_step': (_I, [_P, _D, _D, _P]),

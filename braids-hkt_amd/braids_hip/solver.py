@@ -264,8 +264,10 @@ class HydraulicSolver:
         return acc.value, rej.value
 
     # -- measurement --------------------------------------------------------
-    def profile(self, kinds=True):
-        """kinds: True (all kernel kinds), False (off) or an iterable of kind names."""
+    def profile(self, kinds=True, every=1):
+        """kinds: True (all kernel kinds), False (off) or an iterable of kind
+        names; every: river substeps stamp their launches on every `every`-th
+        substep only (braids_profile_every)."""
         if kinds is True:
             mask = -1
         elif not kinds:
@@ -273,6 +275,7 @@ class HydraulicSolver:
         else:
             mask = sum(1 << _lib.KERNEL_KINDS.index(k) for k in kinds)
         _lib.check(self.L.braids_profile(self.h, int(mask)), 'braids_profile')
+        _lib.check(self.L.braids_profile_every(self.h, int(every)), 'braids_profile_every')
 
     def profile_read(self):
         nk = len(_lib.KERNEL_KINDS)

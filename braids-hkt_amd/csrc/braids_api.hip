@@ -146,8 +146,11 @@ struct SolverBase {
     long long prof_n[KK_COUNT] = {0};
     virtual void set_events(hipEvent_t *ev, int mask) = 0;
     int prof_mask = 0;
+    int prof_every = 1;             // river substeps: stamp every prof_every-th substep's launches
+    long long prof_step = 0;
     int enable_profile(int on)
     {
+        prof_step = 0;
         prof_mask = on == 0 ? 0 : (on < 0 ? (1 << KK_COUNT) - 1 : on);
         for (int k = 0; k < KK_COUNT; ++k) { prof_ms[k] = 0; prof_n[k] = 0; }
         if (on && events.empty()) {
@@ -715,13 +718,15 @@ struct Solver : SolverBase {
         wr.halo_off = ring_off; wr.halo_cell = ring_cell; wr.adj_halo = ring_adj;
         LaunchCfg rc_cfg = cfg;
         rc_cfg.fused = fused_on ? &fplan : nullptr;
+        const bool sampled = profiling && (prof_step++ % prof_every) == 0;
+        if (profiling && !sampled) rc_cfg.ev = nullptr;
         HostMirror hmv = mirror();
         hmv.flag = flag_dev; hmv.ticket = ++ticket;
         run_river_substep<R>(m, st[0], st[1], st[2], st[3], z, wr, ctl, rb, nrot > 0 ? &tv : nullptr, dt, t, rc_cfg, stream,
                              (unsigned int *)ctl, (int)(block_bytes / 4), hmv);
         HIP_TRY(hipGetLastError());
         if ((rc = wait_ticket(hmv.ticket))) return rc;
-        if (profiling) collect(2, fused_on ? 3 : 1);
+        if (sampled) collect(2, fused_on ? 3 : 1);
         const Control *hc = (const Control *)host_block;
         const double *hl = (const double *)((const char *)host_block + sizeof(Control));
         res->accepted = 0; res->reason = 0; res->failed_stage = -1; res->fail_cell = -1; res->frequency_peak = 0;
@@ -1715,6 +1720,14 @@ extern "C" int braids_profile(braids_solver *s, int enable)
 {
     if (!s) { set_error("null solver"); return BRAIDS_EINVAL; }
     return s->impl->enable_profile(enable);
+}
+
+extern "C" int braids_profile_every(braids_solver *s, int every)
+{
+    if (!s || every < 1) { set_error("invalid profile sampling"); return BRAIDS_EINVAL; }
+    s->impl->prof_every = every;
+    s->impl->prof_step = 0;
+    return BRAIDS_OK;
 }
 
 extern "C" int braids_profile_read(braids_solver *s, double *ms, long long *launches)
